@@ -1,0 +1,330 @@
+"""gpuprm_amd -- MI355X-native exact collision checking for PRM planners.
+
+Python mirror of GPUPRM's configuration-space interface
+(prm/src/lib/prm/configspace.hpp:11-50, implemented by RobotConfigspace,
+prm/src/lib/robotspace/robotconfigspace.h:26-176) over the C ABI of
+``include/gprm.h`` (``libgprm.so``: HIP kernels for gfx950).
+
+The native library is required: importing succeeds without it, but every
+constructor raises ``RuntimeError`` if ``libgprm.so`` was not built, so there
+is never a silent CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+__all__ = ["RobotConfigspace", "GprmError", "EdgeStats", "ProfileStats", "lib", "generate_scene", "LIB_PATH"]
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgprm.so")
+
+GPRM_OK = 0
+_ERRORS = {
+    -1: "GPRM_E_INVALID",
+    -2: "GPRM_E_SCENE",
+    -3: "GPRM_E_HIP",
+    -4: "GPRM_E_CAPACITY",
+    -5: "GPRM_E_REQUEST",
+    -6: "GPRM_E_STATE",
+}
+
+
+class GprmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__("%s (%d): %s" % (_ERRORS.get(code, "error"), code, msg))
+        self.code = code
+
+
+class EdgeStats(C.Structure):
+    _fields_ = [
+        ("samples_total", C.c_longlong),
+        ("samples_scheduled", C.c_longlong),
+        ("max_testnum", C.c_int),
+        ("rounds", C.c_int),
+    ]
+
+
+class ProfileStats(C.Structure):
+    _fields_ = [
+        ("sva_launches", C.c_longlong),
+        ("sva_ms", C.c_double),
+        ("fk_launches", C.c_longlong),
+        ("fk_ms", C.c_double),
+        ("pairs_separated", C.c_longlong),
+        ("pairs_colliding", C.c_longlong),
+        ("units_dropped", C.c_longlong),
+        ("units_done", C.c_longlong),
+        ("iterations", C.c_longlong),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+_FP = C.POINTER(C.c_float)
+_IP = C.POINTER(C.c_int)
+
+
+def lib():
+    """Load libgprm.so (raises RuntimeError when it is missing)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "libgprm.so not found at %s; build it with `python -m gpuprm_amd.build` "
+            "(there is no CPU fallback)" % LIB_PATH
+        )
+    # torch ships its own HIP runtime with the same soname (libamdhip64.so.7).
+    # Loading torch first makes libgprm.so bind to that already-loaded runtime,
+    # so torch tensors and this library share one HIP runtime in the process;
+    # two runtimes in one process cannot both open the device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.gprm_create_from_dir.argtypes = [C.c_char_p, C.c_float, C.c_int, C.POINTER(vp)]
+    L.gprm_create.argtypes = [C.c_int, _FP, _FP, _FP, _FP, _IP, C.c_int, _IP, _IP, _IP, _FP, _IP, _IP, _IP,
+                              _IP, _IP, C.c_int, _FP, _FP, C.c_float, C.c_int, C.POINTER(vp)]
+    L.gprm_init.argtypes = [vp, C.c_int, C.c_int]
+    L.gprm_destroy.argtypes = [vp]
+    L.gprm_destroy.restype = None
+    L.gprm_indicator1.argtypes = [vp, _FP, _IP]
+    L.gprm_indicator.argtypes = [vp, _FP, _IP, C.c_int, C.c_int]
+    L.gprm_indicator2.argtypes = [vp, _FP, _FP, _IP, C.c_int, C.c_int]
+    L.gprm_indicator2_async.argtypes = [vp, _FP, _FP, _IP, C.c_int, C.c_int, _IP]
+    L.gprm_indicator2_wait.argtypes = [vp, C.c_int]
+    L.gprm_indicator2_compressed.argtypes = [vp, _FP, C.c_int, _FP, _IP, _IP, _IP, C.c_int]
+    L.gprm_check_boundaries.argtypes = [vp, _FP, _IP, C.c_int, C.c_int]
+    L.gprm_check_boundaries1.argtypes = [vp, _FP, _IP]
+    for f in ("gprm_min", "gprm_max"):
+        getattr(L, f).argtypes = [vp, C.c_int]
+        getattr(L, f).restype = C.c_float
+    L.gprm_deltaq.argtypes = [vp]
+    L.gprm_deltaq.restype = C.c_float
+    L.gprm_dim.argtypes = [vp]
+    L.gprm_numthreads_all.argtypes = [vp]
+    L.gprm_numthreads_all.restype = C.c_longlong
+    L.gprm_indicator_dev.argtypes = [vp, vp, vp, C.c_int, C.c_longlong, vp]
+    L.gprm_indicator2_dev.argtypes = [vp, vp, vp, vp, C.c_int, C.c_longlong, vp, C.POINTER(EdgeStats)]
+    L.gprm_last_error.argtypes = [vp]
+    L.gprm_last_error.restype = C.c_char_p
+    L.gprm_scene_generate.argtypes = [C.c_int, C.c_ulonglong, C.c_char_p]
+    L.gprm_profile_enable.argtypes = [vp, C.c_int]
+    L.gprm_profile_read.argtypes = [vp, C.POINTER(ProfileStats)]
+    _lib = L
+    return L
+
+
+def _check(rc: int, ctx=None) -> int:
+    if rc < 0:
+        msg = lib().gprm_last_error(ctx)
+        raise GprmError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def generate_scene(workload: int, seed: int, out_dir: str) -> str:
+    """Write the seeded scene of a benchmark workload (1, 2, 3 or 5) to out_dir."""
+    _check(lib().gprm_scene_generate(int(workload), int(seed), out_dir.encode()))
+    return out_dir
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class RobotConfigspace:
+    """Mirror of RobotConfigspace<ndof> (robotconfigspace.h:26-176).
+
+    ``RobotConfigspace(path, dq, nbuf)`` loads a scene directory in the
+    reference's format (robotconfigspace.cpp:100-208).  Methods keep the
+    reference's names and argument meaning; they return 0 on success like the
+    reference and raise ``GprmError`` where the reference would abort.
+    """
+
+    def __init__(self, path: Optional[str] = None, dq: float = 0.01, nbuf: int = 2048, *, arrays=None):
+        L = lib()
+        self._ctx = C.c_void_p()
+        if path is not None:
+            _check(L.gprm_create_from_dir(path.encode(), float(dq), int(nbuf), C.byref(self._ctx)))
+        elif arrays is not None:
+            self._create_from_arrays(arrays, dq, nbuf)
+        else:
+            raise ValueError("need a scene path or arrays")
+        self.ndof = L.gprm_dim(self._ctx)
+        self._reqs = {}
+
+    def _create_from_arrays(self, s, dq, nbuf):
+        # s: dict with the flattened constructor arguments of gprm_create
+        keep = {k: (_i32(v) if k in ("types", "sys", "n", "m", "dsp", "cnt", "dest", "from", "to") else _f32(v))
+                for k, v in s.items() if k not in ("ndof",)}
+        self._keep = keep
+        fp = lambda k: keep[k].ctypes.data_as(_FP)
+        ip = lambda k: keep[k].ctypes.data_as(_IP)
+        _check(lib().gprm_create(
+            int(s["ndof"]), fp("a"), fp("alpha"), fp("q"), fp("d"), ip("types"), len(keep["sys"]), ip("sys"),
+            ip("n"), ip("m"), fp("vertices"), ip("dsp"), ip("cnt"), ip("dest"), ip("from"), ip("to"),
+            len(keep["from"]), fp("mins"), fp("maxs"), float(dq), int(nbuf), C.byref(self._ctx)))
+
+    # -- lifecycle ---------------------------------------------------------
+    def init(self, ressource_rank: int = 0, ressource_size: int = 1) -> int:
+        """Configspace::init (configspace.hpp:16)."""
+        return _check(lib().gprm_init(self._ctx, int(ressource_rank), int(ressource_size)), self._ctx)
+
+    def close(self) -> None:
+        if getattr(self, "_ctx", None) and self._ctx.value:
+            lib().gprm_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- indicator functions ---------------------------------------------
+    def indicator(self, q, res=None, N: Optional[int] = None, offset: Optional[int] = None):
+        """indicator(q) -> code (configspace.hpp:22), or the batched form
+        indicator(q, res, N, offset) -> 0 (configspace.hpp:17-20), SoA with
+        component i of item k at q[k + offset*i]."""
+        L = lib()
+        qa = _f32(q)
+        if res is None:
+            out = C.c_int(0)
+            _check(L.gprm_indicator1(self._ctx, qa.ctypes.data_as(_FP), C.byref(out)), self._ctx)
+            return out.value
+        if N is None or offset is None:
+            raise ValueError("batched indicator needs N and offset")
+        self._check_soa(qa, N, offset)
+        r = self._out_i32(res, N)
+        _check(L.gprm_indicator(self._ctx, qa.ctypes.data_as(_FP), r.ctypes.data_as(_IP), int(N), int(offset)),
+               self._ctx)
+        if r is not res:
+            res[:N] = r[:N]
+        return 0
+
+    def indicator_batch(self, Q) -> np.ndarray:
+        """Convenience: Q of shape (ndof, N) -> codes (N,)."""
+        Qa = _f32(Q)
+        N = Qa.shape[1]
+        res = np.zeros(N, np.int32)
+        self.indicator(Qa.ravel(), res, N, N)
+        return res
+
+    def indicator2(self, qs, qe, res, N: int, offset: int) -> int:
+        """indicator2 (configspace.hpp:23-26)."""
+        req = self.indicator2_async(qs, qe, res, N, offset)
+        return self.indicator2_async_wait(req)
+
+    def indicator2_async(self, qs, qe, res, N: int, offset: int) -> int:
+        """indicator2_async (configspace.hpp:27); returns the request id."""
+        qsa, qea = _f32(qs), _f32(qe)
+        self._check_soa(qsa, N, offset)
+        self._check_soa(qea, N, offset)
+        r = self._out_i32(res, N)
+        req = C.c_int(0)
+        _check(lib().gprm_indicator2_async(self._ctx, qsa.ctypes.data_as(_FP), qea.ctypes.data_as(_FP),
+                                           r.ctypes.data_as(_IP), int(N), int(offset), C.byref(req)), self._ctx)
+        self._reqs[req.value] = (qsa, qea, r, res)
+        return req.value
+
+    def indicator2_async_wait(self, request: int) -> int:
+        """indicator2_async_wait (configspace.hpp:28)."""
+        _check(lib().gprm_indicator2_wait(self._ctx, int(request)), self._ctx)
+        qsa, qea, r, res = self._reqs.pop(request)
+        if r is not res:
+            n = min(len(res), len(r))
+            res[:n] = r[:n]
+        return 0
+
+    def indicator2_compressed(self, qs, M: int, qe, res, posqe, numqe, offset: int) -> int:
+        """Compressed indicator2 (configspace.hpp:30-32)."""
+        qsa, qea = _f32(qs), _f32(qe)
+        pa, na = _i32(posqe), _i32(numqe)
+        total = int(max((pa[i] + na[i] for i in range(M)), default=0))
+        r = self._out_i32(res, total)
+        _check(lib().gprm_indicator2_compressed(self._ctx, qsa.ctypes.data_as(_FP), int(M), qea.ctypes.data_as(_FP),
+                                                r.ctypes.data_as(_IP), pa.ctypes.data_as(_IP), na.ctypes.data_as(_IP),
+                                                int(offset)), self._ctx)
+        if r is not res:
+            res[:total] = r[:total]
+        return 0
+
+    def check_boundaries(self, q, res=None, N: Optional[int] = None, offset: Optional[int] = None):
+        """check_boundaries (configspace.hpp:33-37)."""
+        qa = _f32(q)
+        if res is None:
+            out = C.c_int(0)
+            _check(lib().gprm_check_boundaries1(self._ctx, qa.ctypes.data_as(_FP), C.byref(out)), self._ctx)
+            return out.value
+        self._check_soa(qa, N, offset)
+        r = self._out_i32(res, N)
+        _check(lib().gprm_check_boundaries(self._ctx, qa.ctypes.data_as(_FP), r.ctypes.data_as(_IP), int(N),
+                                           int(offset)), self._ctx)
+        if r is not res:
+            res[:N] = r[:N]
+        return 0
+
+    # -- getters (configspace.hpp:39-47) -----------------------------------
+    def min(self, i: int) -> float:
+        return lib().gprm_min(self._ctx, int(i))
+
+    def max(self, i: int) -> float:
+        return lib().gprm_max(self._ctx, int(i))
+
+    def deltaq(self) -> float:
+        return lib().gprm_deltaq(self._ctx)
+
+    def dim(self) -> int:
+        return self.ndof
+
+    def get_numthreads_all(self) -> int:
+        return lib().gprm_numthreads_all(self._ctx)
+
+    # -- device-resident variants (torch tensors on the GPU) -------------
+    def indicator_dev(self, q, res, stream=None) -> None:
+        """q: float32 device tensor (ndof, N) contiguous; res: int32 device tensor (N,)."""
+        N = int(q.shape[1])
+        _check(lib().gprm_indicator_dev(self._ctx, C.c_void_p(q.data_ptr()), C.c_void_p(res.data_ptr()), N,
+                                        N, C.c_void_p(stream)), self._ctx)
+
+    def indicator2_dev(self, qs, qe, res, stream=None) -> EdgeStats:
+        """qs, qe: float32 device tensors (ndof, N); res: int32 device tensor (N,)."""
+        N = int(qs.shape[1])
+        st = EdgeStats()
+        _check(lib().gprm_indicator2_dev(self._ctx, C.c_void_p(qs.data_ptr()), C.c_void_p(qe.data_ptr()),
+                                         C.c_void_p(res.data_ptr()), N, N, C.c_void_p(stream), C.byref(st)),
+               self._ctx)
+        return st
+
+    # -- kernel timing (HIP events) and work counters ---------------------
+    def profile_enable(self, enable: bool = True) -> None:
+        _check(lib().gprm_profile_enable(self._ctx, int(bool(enable))), self._ctx)
+
+    def profile_read(self) -> ProfileStats:
+        st = ProfileStats()
+        _check(lib().gprm_profile_read(self._ctx, C.byref(st)), self._ctx)
+        return st
+
+    # -- helpers -----------------------------------------------------------
+    def _check_soa(self, a, N, offset):
+        if N is None or offset is None or N < 0 or offset < 0:
+            raise ValueError("bad N/offset")
+        if N > 0 and (self.ndof - 1) * offset + N > a.size:
+            raise ValueError("array too small for N=%d offset=%d ndof=%d" % (N, offset, self.ndof))
+
+    @staticmethod
+    def _out_i32(res, N):
+        if isinstance(res, np.ndarray) and res.dtype == np.int32 and res.flags.c_contiguous and res.size >= N:
+            return res
+        return np.zeros(max(N, 1), np.int32)

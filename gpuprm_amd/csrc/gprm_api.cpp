@@ -1,0 +1,705 @@
+// gprm_api.cpp -- host runtime behind the C ABI of include/gprm.h.
+//
+// One context = one scene on one device (the reference's RobotConfigspace,
+// prm/src/lib/robotspace/robotconfigspace.{h,cpp}).  The scene is packed once
+// into a device image that the separating-vector kernel stages in LDS.  Work
+// is split into chunks of at most kItemBudget configurations, each chunk
+// being one kinematics launch and one persistent separating-vector launch.
+// Edge queries run in rounds of growing sample windows; after every round the
+// samples of edges that are already blocked are no longer scheduled.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gprm.h"
+#include "runtime.h"
+#include "scene.h"
+
+namespace gprm {
+namespace {
+
+constexpr int kItemBudget = 1 << 23;   // configurations per kinematics/SVA chunk
+constexpr int kFirstWindow = 2;        // sample positions in the first edge round
+constexpr int kPairsPerUnit = 8;       // pairs a lane claims at once
+
+thread_local std::string g_create_error;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  hipError_t ensure(size_t b) {
+    if (b <= bytes) return hipSuccess;
+    if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+    hipError_t e = hipMalloc(&p, b);
+    if (e == hipSuccess) bytes = b;
+    return e;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Buffers of one in-flight request (the reference keeps one buffer set and
+// one stream per outstanding request, robotconfigspace.cpp:627-650).
+struct Slot {
+  hipStream_t stream = nullptr;
+  DevBuf frames, status, item_edge, item_pos, counters, qs, qe, res, testnum, stride, misc, tally;
+  int* host_res = nullptr;
+  int N = 0;
+  bool busy = false;
+  // profiling: (start, end) event pairs of this call's launches; kind 0 = SVA, 1 = kinematics
+  std::vector<hipEvent_t> ev_start, ev_end;
+  std::vector<int> ev_kind;
+  size_t ev_used = 0;
+  bool tally_pending = false;
+  ~Slot() {
+    for (auto e : ev_start) (void)hipEventDestroy(e);
+    for (auto e : ev_end) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+}  // namespace
+}  // namespace gprm
+
+struct gprm_ctx {
+  gprm::Scene scene;
+  gprm::DevRobot rb{};
+  float dq = 0.01f;
+  int nbuf = 0;
+  int device = -1;
+  bool inited = false;
+  gprm::DevBuf img_buf;
+  gprm::SceneImg img{};
+  int grid = 0;
+  std::vector<std::unique_ptr<gprm::Slot>> slots;
+  std::map<int, int> requests;  // request id -> slot index
+  int next_request = 0;
+  long long numthreads_all = 0;
+  bool profile = false;
+  gprm_profile_stats prof{};
+  std::string err;
+  std::mutex mu;
+};
+
+namespace gprm {
+namespace {
+
+int fail(gprm_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  else g_create_error = msg;
+  return code;
+}
+
+int hip_fail(gprm_ctx* c, hipError_t e, const char* what) {
+  return fail(c, GPRM_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define GPRM_HIP(call, what)                      \
+  do {                                            \
+    hipError_t e_ = (call);                       \
+    if (e_ != hipSuccess) return hip_fail(c, e_, what); \
+  } while (0)
+
+void fill_robot(const Scene& s, DevRobot* rb) {
+  std::memset(rb, 0, sizeof(*rb));
+  rb->ndof = s.ndof;
+  for (int i = 0; i < s.ndof; ++i) {
+    rb->a[i] = s.a[i]; rb->alpha[i] = s.alpha[i]; rb->q[i] = s.q[i]; rb->d[i] = s.d[i];
+    rb->mins[i] = s.mins[i]; rb->maxs[i] = s.maxs[i]; rb->types[i] = s.types[i];
+  }
+}
+
+size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+// Pack the scene image (layout documented at SceneImg).  Pair order is the
+// reference's CRS order (polytope4data.h:343-357): by `from`, then as given.
+int build_image(gprm_ctx* c, std::vector<char>* out) {
+  const Scene& s = c->scene;
+  const int N = (int)s.polys.size();
+  std::vector<int> vbase(N), abase(N);
+  int sumn = 0, summ = 0;
+  for (int k = 0; k < N; ++k) {
+    vbase[k] = sumn; abase[k] = summ;
+    sumn += s.polys[k].n(); summ += s.polys[k].m();
+  }
+  if (sumn >= 65536) return fail(c, GPRM_E_CAPACITY, "scene has 65536 or more vertices");
+  if (summ >= (1 << 23)) return fail(c, GPRM_E_CAPACITY, "scene adjacency too large");
+  const int M = (int)s.from.size();
+  SceneImg& img = c->img;
+  img.sumn = sumn; img.summ = summ; img.M = M; img.N = N;
+  size_t off = 0;
+  img.off_verts = (int)off; off = align16(off + 16 * (size_t)sumn);
+  img.off_pairs = (int)off; off = align16(off + 16 * (size_t)M);
+  img.off_vinfo = (int)off; off = align16(off + 4 * (size_t)sumn);
+  img.off_adj = (int)off;   off = align16(off + 2 * (size_t)summ);
+  img.bytes = (int)off;
+  out->assign(off, 0);
+  char* b = out->data();
+  for (int k = 0; k < N; ++k) {
+    const Polytope& p = s.polys[k];
+    for (int i = 0; i < p.n(); ++i) {
+      float v[4] = {p.verts[3 * i], p.verts[3 * i + 1], p.verts[3 * i + 2], 0.0f};
+      std::memcpy(b + img.off_verts + 16 * (size_t)(vbase[k] + i), v, 16);
+      if (p.cnt[i] > 255) return fail(c, GPRM_E_CAPACITY, "vertex degree above 255");
+      int info = ((abase[k] + p.dsp[i]) << 8) | p.cnt[i];
+      std::memcpy(b + img.off_vinfo + 4 * (size_t)(vbase[k] + i), &info, 4);
+    }
+    for (int j = 0; j < p.m(); ++j) {
+      unsigned short g = (unsigned short)(vbase[k] + p.dest[j]);
+      std::memcpy(b + img.off_adj + 2 * (size_t)(abase[k] + j), &g, 2);
+    }
+  }
+  for (int i = 0; i < M; ++i) {
+    int pr[4] = {vbase[s.from[i]], vbase[s.to[i]], s.sys[s.from[i]], s.sys[s.to[i]]};
+    std::memcpy(b + img.off_pairs + 16 * (size_t)i, pr, 16);
+  }
+  if (sva_lds_bytes(img) > (size_t)sva_max_lds_bytes())
+    return fail(c, GPRM_E_CAPACITY, "scene image of " + std::to_string(img.bytes) +
+                                        " bytes does not fit in LDS with the per-lane scratch");
+  return GPRM_OK;
+}
+
+int new_ctx(Scene&& s, float dq, int nbuf, gprm_ctx** out) {
+  std::string err = validate(s);
+  if (!err.empty()) return fail(nullptr, GPRM_E_SCENE, err);
+  if (!(dq > 0.0f) || !std::isfinite(dq)) return fail(nullptr, GPRM_E_INVALID, "dq must be positive");
+  gprm_ctx* c = new gprm_ctx();
+  c->scene = std::move(s);
+  c->dq = dq;
+  c->nbuf = nbuf;
+  fill_robot(c->scene, &c->rb);
+  *out = c;
+  return GPRM_OK;
+}
+
+int acquire_slot(gprm_ctx* c, int* idx) {
+  for (size_t i = 0; i < c->slots.size(); ++i)
+    if (!c->slots[i]->busy) { *idx = (int)i; c->slots[i]->busy = true; return GPRM_OK; }
+  std::unique_ptr<Slot> s(new Slot());
+  GPRM_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+  s->busy = true;
+  c->slots.push_back(std::move(s));
+  *idx = (int)c->slots.size() - 1;
+  return GPRM_OK;
+}
+
+// Profiling helpers: bracket a launch with events on its stream.
+hipError_t prof_begin(gprm_ctx* c, Slot& sl, int kind, hipStream_t st, size_t* idx) {
+  if (!c->profile) return hipSuccess;
+  if (sl.ev_used == sl.ev_start.size()) {
+    hipEvent_t a, b;
+    hipError_t e = hipEventCreate(&a);
+    if (e != hipSuccess) return e;
+    e = hipEventCreate(&b);
+    if (e != hipSuccess) { (void)hipEventDestroy(a); return e; }
+    sl.ev_start.push_back(a);
+    sl.ev_end.push_back(b);
+    sl.ev_kind.push_back(0);
+  }
+  *idx = sl.ev_used++;
+  sl.ev_kind[*idx] = kind;
+  return hipEventRecord(sl.ev_start[*idx], st);
+}
+hipError_t prof_end(gprm_ctx* c, Slot& sl, size_t idx, hipStream_t st) {
+  if (!c->profile) return hipSuccess;
+  return hipEventRecord(sl.ev_end[idx], st);
+}
+unsigned long long* tally_ptr(gprm_ctx* c, Slot& sl) {
+  return c->profile ? sl.tally.as<unsigned long long>() : nullptr;
+}
+int prof_prepare(gprm_ctx* c, Slot& sl, hipStream_t st) {
+  sl.ev_used = 0;
+  sl.tally_pending = false;
+  if (!c->profile) return GPRM_OK;
+  GPRM_HIP(sl.tally.ensure(sizeof(unsigned long long) * kTallyCount), "hipMalloc tally");
+  GPRM_HIP(launch_fill_u32(sl.tally.as<unsigned>(), 0u, 2 * kTallyCount, st), "fill");
+  sl.tally_pending = true;
+  return GPRM_OK;
+}
+// After the slot's stream has completed: fold event times and counters into
+// the context totals.
+int prof_harvest(gprm_ctx* c, Slot& sl) {
+  for (size_t i = 0; i < sl.ev_used; ++i) {
+    float ms = 0.0f;
+    GPRM_HIP(hipEventElapsedTime(&ms, sl.ev_start[i], sl.ev_end[i]), "hipEventElapsedTime");
+    if (sl.ev_kind[i] == 0) { c->prof.sva_launches++; c->prof.sva_ms += ms; }
+    else { c->prof.fk_launches++; c->prof.fk_ms += ms; }
+  }
+  sl.ev_used = 0;
+  if (sl.tally_pending) {
+    unsigned long long t[kTallyCount];
+    GPRM_HIP(hipMemcpy(t, sl.tally.p, sizeof(t), hipMemcpyDeviceToHost), "copy tally");
+    c->prof.pairs_separated += (long long)t[0];
+    c->prof.pairs_colliding += (long long)t[1];
+    c->prof.units_dropped += (long long)t[2];
+    c->prof.units_done += (long long)t[3];
+    c->prof.iterations += (long long)t[4];
+    sl.tally_pending = false;
+  }
+  return GPRM_OK;
+}
+
+// Point checks of n items on the device (q SoA with offset, res[n]).
+int run_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n, int* d_res,
+               hipStream_t st) {
+  if (n <= 0) return GPRM_OK;
+  const int nd = c->scene.ndof;
+  const int chunk = std::min(n, kItemBudget);
+  GPRM_HIP(sl.frames.ensure(sizeof(float) * 12 * nd * (size_t)chunk), "hipMalloc frames");
+  GPRM_HIP(sl.status.ensure(sizeof(int) * (size_t)chunk), "hipMalloc words");
+  const int nchunks = (n + chunk - 1) / chunk;
+  GPRM_HIP(sl.counters.ensure(sizeof(unsigned) * (size_t)nchunks), "hipMalloc counters");
+  GPRM_HIP(launch_fill_u32(sl.counters.as<unsigned>(), 0u, nchunks, st), "fill");
+  for (int ci = 0; ci < nchunks; ++ci) {
+    int c0 = ci * chunk, cn = std::min(chunk, n - c0);
+    size_t ev = 0;
+    GPRM_HIP(prof_begin(c, sl, 1, st, &ev), "event");
+    GPRM_HIP(launch_fk_points(c->rb, d_q + c0, offset, cn, sl.frames.as<float>(), sl.status.as<int>(), 1, st),
+             "kinematics launch");
+    GPRM_HIP(prof_end(c, sl, ev, st), "event");
+    SvaArgs a{};
+    a.img = c->img;
+    a.ndof = nd;
+    a.frames = sl.frames.as<float>();
+    a.nitems = cn;
+    a.counter = sl.counters.as<unsigned>() + ci;
+    a.pairs_per_unit = kPairsPerUnit;
+    a.units_per_item = (c->img.M + kPairsPerUnit - 1) / kPairsPerUnit;
+    a.word = sl.status.as<int>();
+    a.tally = tally_ptr(c, sl);
+    GPRM_HIP(prof_begin(c, sl, 0, st, &ev), "event");
+    GPRM_HIP(launch_sva(a, 0, c->grid, st), "separating-vector launch");
+    GPRM_HIP(prof_end(c, sl, ev, st), "event");
+    GPRM_HIP(launch_finalize_points(sl.status.as<int>(), d_res + c0, cn, st), "finalize launch");
+  }
+  return GPRM_OK;
+}
+
+// Sample-window rounds: positions [0,w0), then doubling widths.
+std::vector<std::pair<int, int>> windows(int maxT) {
+  std::vector<std::pair<int, int>> w;
+  int lo = 0, width = kFirstWindow;
+  while (lo < maxT) {
+    int hi = (int)std::min<long long>((long long)lo + width, maxT);
+    w.push_back({lo, hi});
+    lo = hi;
+    if (width < (1 << 20)) width *= 2;
+  }
+  return w;
+}
+
+// Edge checks on the device.  testnum/stride/res are computed by the
+// prologue kernel; maxT (host-known or read back) sizes the rounds.
+int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long long offset, int N,
+              int* d_res, hipStream_t st, int host_maxT, gprm_edge_stats* stats) {
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (N <= 0) return GPRM_OK;
+  const int nd = c->scene.ndof;
+  GPRM_HIP(sl.testnum.ensure(sizeof(int) * (size_t)N), "hipMalloc testnum");
+  GPRM_HIP(sl.stride.ensure(sizeof(int) * (size_t)N), "hipMalloc stride");
+  GPRM_HIP(sl.misc.ensure(64), "hipMalloc misc");
+  int* d_max = sl.misc.as<int>();
+  unsigned long long* d_total = reinterpret_cast<unsigned long long*>(sl.misc.as<char>() + 16);
+  GPRM_HIP(launch_fill_u32(sl.misc.as<unsigned>(), 0u, 16, st), "fill");
+  EdgeArgs ea{};
+  ea.qs = d_qs; ea.qe = d_qe; ea.offset = offset; ea.nedges = N;
+  ea.testnum = sl.testnum.as<int>(); ea.stride = sl.stride.as<int>(); ea.edge_res = d_res;
+  GPRM_HIP(launch_edge_prologue(c->rb, c->dq, ea, sl.testnum.as<int>(), sl.stride.as<int>(), d_max, d_total, st),
+           "edge prologue launch");
+  int maxT = host_maxT;
+  unsigned long long total = 0;
+  if (maxT < 0 || stats) {
+    int hmax = 0;
+    GPRM_HIP(hipMemcpyAsync(&hmax, d_max, 4, hipMemcpyDeviceToHost, st), "copy max");
+    GPRM_HIP(hipMemcpyAsync(&total, d_total, 8, hipMemcpyDeviceToHost, st), "copy total");
+    GPRM_HIP(hipStreamSynchronize(st), "sync");
+    maxT = hmax;
+  }
+  std::vector<std::pair<int, int>> win = windows(maxT);
+  // chunking: at most kItemBudget items per launch triple
+  struct Launch { int e0, e1, lo, hi; };
+  std::vector<Launch> plan;
+  for (auto& w : win) {
+    int width = w.second - w.first;
+    int per = std::max(1, kItemBudget / width);
+    for (int e0 = 0; e0 < N; e0 += per) plan.push_back({e0, std::min(N, e0 + per), w.first, w.second});
+  }
+  int cap = 0;
+  for (auto& p : plan) cap = std::max(cap, (p.e1 - p.e0) * (p.hi - p.lo));
+  GPRM_HIP(sl.frames.ensure(sizeof(float) * 12 * nd * (size_t)std::max(cap, 1)), "hipMalloc frames");
+  GPRM_HIP(sl.item_edge.ensure(sizeof(int) * (size_t)std::max(cap, 1)), "hipMalloc items");
+  GPRM_HIP(sl.item_pos.ensure(sizeof(int) * (size_t)std::max(cap, 1)), "hipMalloc items");
+  // per launch: [0] item count, [1] work counter
+  GPRM_HIP(sl.counters.ensure(sizeof(int) * 2 * std::max<size_t>(plan.size(), 1)), "hipMalloc counters");
+  GPRM_HIP(launch_fill_u32(sl.counters.as<unsigned>(), 0u, 2 * (long long)plan.size(), st), "fill");
+  for (size_t i = 0; i < plan.size(); ++i) {
+    const Launch& p = plan[i];
+    int* cnt = sl.counters.as<int>() + 2 * i;
+    GPRM_HIP(launch_edge_compact(ea, p.e0, p.e1, p.lo, p.hi, sl.item_edge.as<int>(), sl.item_pos.as<int>(), cnt, st),
+             "edge compaction launch");
+    int pcap = (p.e1 - p.e0) * (p.hi - p.lo);
+    size_t ev = 0;
+    GPRM_HIP(prof_begin(c, sl, 1, st, &ev), "event");
+    GPRM_HIP(launch_edge_fk(c->rb, ea, sl.item_edge.as<int>(), sl.item_pos.as<int>(), cnt, pcap,
+                            sl.frames.as<float>(), st),
+             "edge kinematics launch");
+    GPRM_HIP(prof_end(c, sl, ev, st), "event");
+    SvaArgs a{};
+    a.img = c->img;
+    a.ndof = nd;
+    a.frames = sl.frames.as<float>();
+    a.nitems = -1;
+    a.nitems_dev = cnt;
+    a.counter = reinterpret_cast<unsigned*>(cnt + 1);
+    a.item_edge = sl.item_edge.as<int>();
+    a.edge_res = d_res;
+    a.pairs_per_unit = kPairsPerUnit;
+    a.units_per_item = (c->img.M + kPairsPerUnit - 1) / kPairsPerUnit;
+    a.tally = tally_ptr(c, sl);
+    GPRM_HIP(prof_begin(c, sl, 0, st, &ev), "event");
+    GPRM_HIP(launch_sva(a, 1, c->grid, st), "separating-vector launch");
+    GPRM_HIP(prof_end(c, sl, ev, st), "event");
+  }
+  if (stats) {
+    std::vector<int> h(2 * plan.size());
+    if (!plan.empty()) {
+      GPRM_HIP(hipMemcpyAsync(h.data(), sl.counters.p, sizeof(int) * h.size(), hipMemcpyDeviceToHost, st), "copy");
+      GPRM_HIP(hipStreamSynchronize(st), "sync");
+    }
+    long long sched = 0;
+    for (size_t i = 0; i < plan.size(); ++i) sched += h[2 * i];
+    stats->samples_total = (long long)total;
+    stats->samples_scheduled = sched;
+    stats->max_testnum = maxT;
+    stats->rounds = (int)plan.size();
+  }
+  return GPRM_OK;
+}
+
+// Host-side testnum (robotconfigspace.cpp:660-673), same float operations as
+// the prologue kernel; gives the round plan without a device round trip.
+long long host_testnum(const gprm_ctx* c, const float* qs, const float* qe, int N, int offset, int* maxT) {
+  long long tot = 0;
+  int mx = 0;
+  for (int l = 0; l < N; ++l) {
+    float norm = 0.0f;
+    for (int i = 0; i < c->scene.ndof; ++i) {
+      float diff = qe[l + (long long)offset * i] - qs[l + (long long)offset * i];
+      norm += diff * diff;
+    }
+    norm = std::sqrt(norm);
+    int T = (int)(norm / c->dq) + 1;
+    tot += T;
+    mx = std::max(mx, T);
+  }
+  *maxT = mx;
+  return tot;
+}
+
+// Copy a caller SoA array (component i at [k + offset*i]) to device SoA with
+// stride N, as the reference does (robotconfigspace.cpp:678-682).
+int upload_soa(gprm_ctx* c, DevBuf& buf, const float* h, int N, int offset, hipStream_t st) {
+  const int nd = c->scene.ndof;
+  GPRM_HIP(buf.ensure(sizeof(float) * (size_t)nd * std::max(N, 1)), "hipMalloc");
+  for (int i = 0; i < nd; ++i)
+    GPRM_HIP(hipMemcpyAsync(buf.as<float>() + (size_t)N * i, h + (long long)offset * i, sizeof(float) * N,
+                            hipMemcpyHostToDevice, st),
+             "copy H2D");
+  return GPRM_OK;
+}
+
+int check_ready(gprm_ctx* c) {
+  if (!c) return GPRM_E_INVALID;
+  if (!c->inited) return fail(c, GPRM_E_STATE, "context not initialised");
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+  return GPRM_OK;
+}
+
+}  // namespace
+}  // namespace gprm
+
+using namespace gprm;
+
+extern "C" {
+
+int gprm_create_from_dir(const char* path, float dq, int nbuf, gprm_ctx** out) {
+  if (!path || !out) return fail(nullptr, GPRM_E_INVALID, "null argument");
+  *out = nullptr;
+  Scene s;
+  std::string err = load_dir(path, &s);
+  if (!err.empty()) return fail(nullptr, GPRM_E_SCENE, err);
+  return new_ctx(std::move(s), dq, nbuf, out);
+}
+
+int gprm_create(int ndof, const float* a, const float* alpha, const float* q, const float* d,
+                const int* types, int N, const int* sys, const int* n, const int* m,
+                const float* vertices, const int* dsp, const int* cnt, const int* dest,
+                const int* from, const int* to, int M, const float* mins, const float* maxs,
+                float dq, int nbuf, gprm_ctx** out) {
+  if (!out || !a || !alpha || !q || !d || !types || !sys || !n || !m || !vertices || !dsp || !cnt ||
+      !from || !to || !mins || !maxs)
+    return fail(nullptr, GPRM_E_INVALID, "null argument");
+  *out = nullptr;
+  if (ndof < 1 || ndof > kMaxDof || N < 1 || M < 1) return fail(nullptr, GPRM_E_INVALID, "bad sizes");
+  Scene s;
+  s.ndof = ndof;
+  for (int i = 0; i < ndof; ++i) {
+    s.a[i] = a[i]; s.alpha[i] = alpha[i]; s.q[i] = q[i]; s.d[i] = d[i]; s.types[i] = types[i];
+    s.mins[i] = mins[i]; s.maxs[i] = maxs[i];
+  }
+  long long vo = 0, eo = 0;
+  for (int k = 0; k < N; ++k) {
+    if (n[k] < 1 || m[k] < 0) return fail(nullptr, GPRM_E_INVALID, "bad polytope size");
+    Polytope p;
+    p.verts.assign(vertices + 3 * vo, vertices + 3 * (vo + n[k]));
+    p.dsp.assign(dsp + vo, dsp + vo + n[k]);
+    p.cnt.assign(cnt + vo, cnt + vo + n[k]);
+    if (m[k] > 0 && !dest) return fail(nullptr, GPRM_E_INVALID, "null dest");
+    if (m[k] > 0) p.dest.assign(dest + eo, dest + eo + m[k]);
+    vo += n[k];
+    eo += m[k];
+    s.polys.push_back(std::move(p));
+    s.sys.push_back(sys[k]);
+  }
+  s.from.assign(from, from + M);
+  s.to.assign(to, to + M);
+  return new_ctx(std::move(s), dq, nbuf, out);
+}
+
+int gprm_init(gprm_ctx* c, int rank, int size) {
+  if (!c) return GPRM_E_INVALID;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (size < 1 || rank < 0 || rank >= size) return fail(c, GPRM_E_INVALID, "bad rank/size");
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count < 1) return fail(c, GPRM_E_HIP, "no HIP device available");
+  // robotconfigspace.cpp:229: device = rank*devcount/size
+  c->device = (int)((long long)rank * count / size);
+  GPRM_HIP(hipSetDevice(c->device), "hipSetDevice");
+  std::vector<char> image;
+  int rc = build_image(c, &image);
+  if (rc != GPRM_OK) return rc;
+  GPRM_HIP(c->img_buf.ensure(image.size()), "hipMalloc scene");
+  GPRM_HIP(hipMemcpy(c->img_buf.p, image.data(), image.size(), hipMemcpyHostToDevice), "copy scene");
+  c->img.blob = c->img_buf.as<int4>();
+  c->grid = sva_grid(sva_lds_bytes(c->img), c->device);
+  c->inited = true;
+  return GPRM_OK;
+}
+
+void gprm_destroy(gprm_ctx* c) {
+  if (!c) return;
+  if (c->device >= 0) (void)hipSetDevice(c->device);
+  for (auto& s : c->slots)
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+  delete c;
+}
+
+int gprm_indicator(gprm_ctx* c, const float* q, int* res, int N, int offset) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (N < 0 || (N > 0 && (!q || !res)) || offset < 0) return fail(c, GPRM_E_INVALID, "bad arguments");
+  if (N == 0) return GPRM_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  int si;
+  if ((rc = acquire_slot(c, &si))) return rc;
+  Slot& sl = *c->slots[si];
+  struct Release { Slot& s; ~Release() { s.busy = false; } } rel{sl};
+  hipStream_t st = sl.stream;
+  if ((rc = prof_prepare(c, sl, st))) return rc;
+  if ((rc = upload_soa(c, sl.qs, q, N, offset, st))) return rc;
+  GPRM_HIP(sl.res.ensure(sizeof(int) * (size_t)N), "hipMalloc res");
+  if ((rc = run_points(c, sl, sl.qs.as<float>(), N, N, sl.res.as<int>(), st))) return rc;
+  GPRM_HIP(hipMemcpyAsync(res, sl.res.p, sizeof(int) * N, hipMemcpyDeviceToHost, st), "copy D2H");
+  GPRM_HIP(hipStreamSynchronize(st), "sync");
+  return prof_harvest(c, sl);
+}
+
+int gprm_indicator1(gprm_ctx* c, const float* q, int* code) {
+  if (!q || !code) return c ? fail(c, GPRM_E_INVALID, "null argument") : GPRM_E_INVALID;
+  return gprm_indicator(c, q, code, 1, 1);
+}
+
+int gprm_indicator_dev(gprm_ctx* c, const float* d_q, int* d_res, int N, long long offset, void* stream) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (N < 0 || (N > 0 && (!d_q || !d_res)) || offset < 0) return fail(c, GPRM_E_INVALID, "bad arguments");
+  if (N == 0) return GPRM_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  int si;
+  if ((rc = acquire_slot(c, &si))) return rc;
+  Slot& sl = *c->slots[si];
+  struct Release { Slot& s; ~Release() { s.busy = false; } } rel{sl};
+  hipStream_t st = stream ? (hipStream_t)stream : sl.stream;
+  if ((rc = prof_prepare(c, sl, st))) return rc;
+  if ((rc = run_points(c, sl, d_q, offset, N, d_res, st))) return rc;
+  GPRM_HIP(hipStreamSynchronize(st), "sync");
+  return prof_harvest(c, sl);
+}
+
+int gprm_indicator2_async(gprm_ctx* c, const float* qs, const float* qe, int* res, int N, int offset,
+                          int* request) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (N < 0 || (N > 0 && (!qs || !qe || !res)) || offset < 0 || !request)
+    return fail(c, GPRM_E_INVALID, "bad arguments");
+  std::lock_guard<std::mutex> lk(c->mu);
+  int si;
+  if ((rc = acquire_slot(c, &si))) return rc;
+  Slot& sl = *c->slots[si];
+  hipStream_t st = sl.stream;
+  int maxT = 0;
+  long long total = host_testnum(c, qs, qe, N, offset, &maxT);
+  if (N > 0) {
+    rc = prof_prepare(c, sl, st);
+    if (!rc) rc = upload_soa(c, sl.qs, qs, N, offset, st);
+    if (!rc) rc = upload_soa(c, sl.qe, qe, N, offset, st);
+    if (!rc && sl.res.ensure(sizeof(int) * (size_t)N) != hipSuccess) rc = fail(c, GPRM_E_HIP, "hipMalloc res");
+    if (!rc) rc = run_edges(c, sl, sl.qs.as<float>(), sl.qe.as<float>(), N, N, sl.res.as<int>(), st, maxT, nullptr);
+    if (rc) {
+      sl.busy = false;
+      return rc;
+    }
+  }
+  sl.host_res = res;
+  sl.N = N;
+  int id = ++c->next_request;
+  c->requests[id] = si;
+  *request = id;
+  c->numthreads_all += total;  // robotconfigspace.cpp:720
+  return GPRM_OK;
+}
+
+int gprm_indicator2_wait(gprm_ctx* c, int request) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(c->mu);
+  auto it = c->requests.find(request);
+  if (it == c->requests.end()) return fail(c, GPRM_E_REQUEST, "unknown request id");
+  Slot& sl = *c->slots[it->second];
+  c->requests.erase(it);
+  struct Release { Slot& s; ~Release() { s.busy = false; } } rel{sl};
+  if (sl.N > 0)
+    GPRM_HIP(hipMemcpyAsync(sl.host_res, sl.res.p, sizeof(int) * sl.N, hipMemcpyDeviceToHost, sl.stream),
+             "copy D2H");
+  GPRM_HIP(hipStreamSynchronize(sl.stream), "sync");
+  return prof_harvest(c, sl);
+}
+
+int gprm_indicator2(gprm_ctx* c, const float* qs, const float* qe, int* res, int N, int offset) {
+  int req = 0;
+  int rc = gprm_indicator2_async(c, qs, qe, res, N, offset, &req);
+  if (rc) return rc;
+  return gprm_indicator2_wait(c, req);
+}
+
+int gprm_indicator2_dev(gprm_ctx* c, const float* d_qs, const float* d_qe, int* d_res, int N,
+                        long long offset, void* stream, gprm_edge_stats* stats) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (N < 0 || (N > 0 && (!d_qs || !d_qe || !d_res)) || offset < 0) return fail(c, GPRM_E_INVALID, "bad arguments");
+  std::lock_guard<std::mutex> lk(c->mu);
+  int si;
+  if ((rc = acquire_slot(c, &si))) return rc;
+  Slot& sl = *c->slots[si];
+  struct Release { Slot& s; ~Release() { s.busy = false; } } rel{sl};
+  hipStream_t st = stream ? (hipStream_t)stream : sl.stream;
+  gprm_edge_stats local;
+  if ((rc = prof_prepare(c, sl, st))) return rc;
+  if ((rc = run_edges(c, sl, d_qs, d_qe, offset, N, d_res, st, -1, &local))) return rc;
+  GPRM_HIP(hipStreamSynchronize(st), "sync");
+  c->numthreads_all += local.samples_total;
+  if (stats) *stats = local;
+  return prof_harvest(c, sl);
+}
+
+int gprm_indicator2_compressed(gprm_ctx* c, const float* qs, int M, const float* qe, int* res,
+                               const int* posqe, const int* numqe, int offset) {
+  if (!c) return GPRM_E_INVALID;
+  if (M < 0 || (M > 0 && (!qs || !qe || !res || !posqe || !numqe)) || offset < 0)
+    return fail(c, GPRM_E_INVALID, "bad arguments");
+  const int nd = c->scene.ndof;
+  long long E = 0;
+  for (int i = 0; i < M; ++i) {
+    if (numqe[i] < 0 || posqe[i] < 0) return fail(c, GPRM_E_INVALID, "bad posqe/numqe");
+    E += numqe[i];
+  }
+  if (E > (1LL << 30)) return fail(c, GPRM_E_INVALID, "too many edges");
+  std::vector<float> S((size_t)nd * E), T((size_t)nd * E);
+  std::vector<int> idx(E), r(E);
+  long long e = 0;
+  for (int i = 0; i < M; ++i)
+    for (int j = 0; j < numqe[i]; ++j, ++e) {
+      idx[e] = posqe[i] + j;
+      for (int d = 0; d < nd; ++d) {
+        S[(size_t)E * d + e] = qs[i + (long long)offset * d];
+        T[(size_t)E * d + e] = qe[idx[e] + (long long)offset * d];
+      }
+    }
+  int rc = gprm_indicator2(c, S.data(), T.data(), r.data(), (int)E, (int)E);
+  if (rc) return rc;
+  for (long long k = 0; k < E; ++k) res[idx[k]] = r[k];
+  return GPRM_OK;
+}
+
+int gprm_check_boundaries(gprm_ctx* c, const float* q, int* res, int N, int offset) {
+  if (!c) return GPRM_E_INVALID;
+  if (N < 0 || (N > 0 && (!q || !res)) || offset < 0) return fail(c, GPRM_E_INVALID, "bad arguments");
+  for (int ix = 0; ix < N; ++ix) {
+    int r = 0;
+    for (int i = 0; i < c->scene.ndof; ++i) {
+      float qi = q[ix + (long long)i * offset];
+      if (qi < c->scene.mins[i] || qi > c->scene.maxs[i]) { r = 1; break; }
+    }
+    res[ix] = r;
+  }
+  return GPRM_OK;
+}
+
+int gprm_check_boundaries1(gprm_ctx* c, const float* q, int* out) {
+  if (!c || !q || !out) return GPRM_E_INVALID;
+  return gprm_check_boundaries(c, q, out, 1, 1);
+}
+
+float gprm_min(gprm_ctx* c, int i) { return (c && i >= 0 && i < c->scene.ndof) ? c->scene.mins[i] : NAN; }
+float gprm_max(gprm_ctx* c, int i) { return (c && i >= 0 && i < c->scene.ndof) ? c->scene.maxs[i] : NAN; }
+float gprm_deltaq(gprm_ctx* c) { return c ? c->dq : NAN; }
+int gprm_dim(gprm_ctx* c) { return c ? c->scene.ndof : GPRM_E_INVALID; }
+long long gprm_numthreads_all(gprm_ctx* c) { return c ? c->numthreads_all : GPRM_E_INVALID; }
+
+int gprm_profile_enable(gprm_ctx* c, int enable) {
+  if (!c) return GPRM_E_INVALID;
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->profile = enable != 0;
+  c->prof = gprm_profile_stats{};
+  return GPRM_OK;
+}
+
+int gprm_profile_read(gprm_ctx* c, gprm_profile_stats* out) {
+  if (!c || !out) return GPRM_E_INVALID;
+  std::lock_guard<std::mutex> lk(c->mu);
+  *out = c->prof;
+  return GPRM_OK;
+}
+
+const char* gprm_last_error(gprm_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int gprm_scene_generate(int workload, unsigned long long seed, const char* out_dir) {
+  if (!out_dir) return fail(nullptr, GPRM_E_INVALID, "null argument");
+  Scene s;
+  std::string err = generate(workload, seed, &s);
+  if (!err.empty()) return fail(nullptr, GPRM_E_INVALID, err);
+  err = write_dir(s, out_dir);
+  if (!err.empty()) return fail(nullptr, GPRM_E_SCENE, err);
+  return GPRM_OK;
+}
+
+}  // extern "C"

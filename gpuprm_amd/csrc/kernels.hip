@@ -1,0 +1,704 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the exact collision-check path.
+//
+//   fk_points / fk_edges : forward kinematics, one lane per configuration,
+//                          frames kept in VGPRs and written once to HBM.
+//   sva                  : persistent separating-vector kernel; the scene is
+//                          staged in LDS, every lane is an independent worker
+//                          that steps one iteration of the Chung-Wang test per
+//                          trip and refills itself from a wave-level work queue
+//                          (ballot + one atomic per 64 items).
+//   edge_prologue/compact: edge sample counts, sample order, and per-round
+//                          compaction of the samples of still-open edges.
+//
+// Arithmetic follows the reference's host path operation by operation:
+// the file is compiled with -ffp-contract=off (no FMA contraction), IEEE
+// division and sqrt, denormals kept.  Comparisons keep the reference's
+// direction so NaN behaves as it does there.
+#include <hip/hip_runtime.h>
+
+#include "glibc_trig.h"
+#include "runtime.h"
+
+namespace gprm {
+
+namespace {
+
+// ----------------------------------------------------------------------------
+// geometry (prm/src/lib/collision4/geo4.h)
+// ----------------------------------------------------------------------------
+struct V3 {
+  float x, y, z;
+};
+struct V2 {
+  float x, y;
+};
+// trafo4 (geo4.h:22-24): columns 0..3, rows x,y,z.
+struct T12 {
+  float m[12];  // col c, row r at m[3*c + r]
+};
+
+__device__ __forceinline__ V3 v3(float4 a) { return V3{a.x, a.y, a.z}; }
+
+// sprod (geo4.h:130-132)
+__device__ __forceinline__ float dot3(const V3& u, const V3& v) {
+  return (u.x * v.x + u.y * v.y) + u.z * v.z;
+}
+// normalize(float4&) (geo4.h:165-171)
+__device__ __forceinline__ float normalize3(V3& u) {
+  float norm = __builtin_sqrtf((u.x * u.x + u.y * u.y) + u.z * u.z);
+  float factor = 1.0f / norm;
+  if (factor == __builtin_inff()) factor = 0.0f;
+  u.x *= factor;
+  u.y *= factor;
+  u.z *= factor;
+  return norm;
+}
+// normalize(float2&) (geo4.h:80-83), no zero guard
+__device__ __forceinline__ void normalize2(V2& u) {
+  float factor = 1.0f / __builtin_sqrtf(u.x * u.x + u.y * u.y);
+  u.x *= factor;
+  u.y *= factor;
+}
+// cross2 (geo4.h:66-68)
+__device__ __forceinline__ float cross2(const V2& v, const V2& w) { return v.x * w.y - v.y * w.x; }
+
+// trafo4::apply(u, Tu) (geo4.h:215-221)
+__device__ __forceinline__ V3 t_apply(const T12& t, const V3& u) {
+  const float* c = t.m;
+  V3 r;
+  r.x = ((c[0] * u.x + c[3] * u.y) + c[6] * u.z) + c[9];
+  r.y = ((c[1] * u.x + c[4] * u.y) + c[7] * u.z) + c[10];
+  r.z = ((c[2] * u.x + c[5] * u.y) + c[8] * u.z) + c[11];
+  return r;
+}
+// trafo4::apply_rot(u) (geo4.h:223-230)
+__device__ __forceinline__ V3 t_apply_rot(const T12& t, const V3& u) {
+  const float* c = t.m;
+  V3 r;
+  r.x = (c[0] * u.x + c[3] * u.y) + c[6] * u.z;
+  r.y = (c[1] * u.x + c[4] * u.y) + c[7] * u.z;
+  r.z = (c[2] * u.x + c[5] * u.y) + c[8] * u.z;
+  return r;
+}
+// trafo4::apply_rot_inv(u, Rtu) (geo4.h:249-255)
+__device__ __forceinline__ V3 t_apply_rot_inv(const T12& t, const V3& u) {
+  const float* c = t.m;
+  V3 r;
+  r.x = (c[0] * u.x + c[1] * u.y) + c[2] * u.z;
+  r.y = (c[3] * u.x + c[4] * u.y) + c[5] * u.z;
+  r.z = (c[6] * u.x + c[7] * u.y) + c[8] * u.z;
+  return r;
+}
+// trafo4::set(a, alpha, q, d) (geo4.h:194-203)
+__device__ __forceinline__ T12 t_set(float a, float alpha, float q, float d) {
+  float cq = gprm_trig::cosf_ref(q), sq = gprm_trig::sinf_ref(q);
+  float ca = gprm_trig::cosf_ref(alpha), sa = gprm_trig::sinf_ref(alpha);
+  T12 t;
+  t.m[0] = cq;      t.m[3] = -sq;     t.m[6] = 0.0f;  t.m[9] = a;
+  t.m[1] = sq * ca; t.m[4] = cq * ca; t.m[7] = -sa;   t.m[10] = -sa * d;
+  t.m[2] = sq * sa; t.m[5] = cq * sa; t.m[8] = ca;    t.m[11] = ca * d;
+  return t;
+}
+// this = T * this (trafo4::lapply, geo4.h:272-279)
+__device__ __forceinline__ T12 t_lapply(const T12& self, const T12& T) {
+  T12 r;
+  for (int c = 0; c < 3; ++c) {
+    V3 col = t_apply_rot(T, V3{self.m[3 * c], self.m[3 * c + 1], self.m[3 * c + 2]});
+    r.m[3 * c] = col.x; r.m[3 * c + 1] = col.y; r.m[3 * c + 2] = col.z;
+  }
+  V3 col = t_apply(T, V3{self.m[9], self.m[10], self.m[11]});
+  r.m[9] = col.x; r.m[10] = col.y; r.m[11] = col.z;
+  return r;
+}
+
+// Kinematics<ndof>::calculate (kinematics.h:27-37); frames 1..ndof to out.
+template <int NDOF>
+__device__ __forceinline__ void fk_store(const DevRobot& rb, const float* q, float* __restrict__ out) {
+  const int nd = NDOF > 0 ? NDOF : rb.ndof;
+  T12 T = t_set(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+  for (int i = 0; i < (NDOF > 0 ? NDOF : kMaxDofDev); ++i) {
+    if (NDOF == 0 && i >= nd) break;
+    float qset = rb.q[i], dset = rb.d[i];
+    if (rb.types[i] == 0) qset = q[i];
+    else dset = q[i];
+    T12 A = t_set(rb.a[i], rb.alpha[i], qset, dset);
+    T = t_lapply(A, T);
+    float4* o = reinterpret_cast<float4*>(out + 12 * i);
+    o[0] = make_float4(T.m[0], T.m[1], T.m[2], T.m[3]);
+    o[1] = make_float4(T.m[4], T.m[5], T.m[6], T.m[7]);
+    o[2] = make_float4(T.m[8], T.m[9], T.m[10], T.m[11]);
+  }
+}
+
+__device__ __forceinline__ T12 load_frame(const float* __restrict__ frames, int item, int ndof, int sys) {
+  const float4* f = reinterpret_cast<const float4*>(frames + ((size_t)item * ndof + (sys - 1)) * 12);
+  float4 a = f[0], b = f[1], c = f[2];
+  T12 t;
+  t.m[0] = a.x; t.m[1] = a.y; t.m[2] = a.z;  t.m[3] = a.w;
+  t.m[4] = b.x; t.m[5] = b.y; t.m[6] = b.z;  t.m[7] = b.w;
+  t.m[8] = c.x; t.m[9] = c.y; t.m[10] = c.z; t.m[11] = c.w;
+  return t;
+}
+
+// `norm < collision_eps` and `dp >= -collision_eps` compare a float with the
+// double 1e-6 (collision4.h:20,242,296).  float(1e-6) = 0x1.0c6f7ap-20 lies
+// below 1e-6, so both become exact float comparisons with it.
+constexpr float kEpsBelow = 0x1.0c6f7ap-20f;
+
+// ----------------------------------------------------------------------------
+// kinematics kernels
+// ----------------------------------------------------------------------------
+// Point mode result word of one configuration: ((pair+1) << 4) | (code+1)
+// of the first colliding pair in CRS order (updated by atomicMin), kWordFree
+// while none is known; out of bounds is (0 << 4) | (1+1).
+constexpr int kWordFree = 0x7fffffff;
+constexpr int kWordOutOfBounds = 2;
+
+template <int NDOF>
+__global__ void __launch_bounds__(kFkBlock)
+    fk_points_kernel(DevRobot rb, const float* __restrict__ q, long long offset, int n,
+                     float* __restrict__ frames, int* __restrict__ word, int check_bounds) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int nd = NDOF > 0 ? NDOF : rb.ndof;
+  float qq[kMaxDofDev];
+#pragma unroll
+  for (int j = 0; j < (NDOF > 0 ? NDOF : kMaxDofDev); ++j)
+    if (NDOF > 0 || j < nd) qq[j] = q[i + offset * j];
+  if (check_bounds) {
+    // check_boundaries(q) (robotconfigspace.cpp:795-802)
+    int st = 0;
+    for (int j = 0; j < nd; ++j)
+      if (qq[j] < rb.mins[j] || qq[j] > rb.maxs[j]) { st = 1; break; }
+    word[i] = st ? kWordOutOfBounds : kWordFree;
+    if (st) return;
+  } else {
+    word[i] = kWordFree;
+  }
+  fk_store<NDOF>(rb, qq, frames + (size_t)i * nd * 12);
+}
+
+// codes[i] from the point mode result word.
+__global__ void finalize_points_kernel(const int* __restrict__ word, int* __restrict__ codes, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int w = word[i];
+  codes[i] = w == kWordFree ? 0 : (w & 15) - 1;
+}
+
+// q of the sample at order position pos of edge e (robotconfigspace.cpp:409-420).
+template <int NDOF>
+__global__ void __launch_bounds__(kFkBlock)
+    fk_edges_kernel(DevRobot rb, EdgeArgs ea, const int* __restrict__ item_edge,
+                    const int* __restrict__ item_pos, const int* __restrict__ nitems_dev, int cap,
+                    float* __restrict__ frames) {
+  int n = *nitems_dev;
+  if (n > cap) n = cap;
+  const int nd = NDOF > 0 ? NDOF : rb.ndof;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int e = item_edge[i];
+    int pos = item_pos[i];
+    int T = ea.testnum[e];
+    int s = (int)(((long long)pos * ea.stride[e]) % T);
+    float c1, c2;
+    if (T > 0) {
+      c1 = (float)s / (float)(T - 1);
+      c2 = 1.0f - c1;
+    } else {
+      c1 = 1.0f;
+      c2 = 0.0f;
+    }
+    float qq[kMaxDofDev];
+#pragma unroll
+    for (int j = 0; j < (NDOF > 0 ? NDOF : kMaxDofDev); ++j)
+      if (NDOF > 0 || j < nd) qq[j] = c1 * ea.qs[e + ea.offset * j] + c2 * ea.qe[e + ea.offset * j];
+    fk_store<NDOF>(rb, qq, frames + (size_t)i * nd * 12);
+  }
+}
+
+__device__ int gcd_i(int a, int b) {
+  while (b) {
+    int t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+// testnum (robotconfigspace.cpp:660-673) and the sample order stride: a
+// golden-ratio step coprime with testnum, so that positions 0,1,2,... visit
+// every sample once, spread along the edge.
+__global__ void __launch_bounds__(256)
+    edge_prologue_kernel(int ndof, float dq, EdgeArgs ea, int* __restrict__ testnum,
+                         int* __restrict__ stride, int* __restrict__ max_testnum,
+                         unsigned long long* __restrict__ total) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int T = 0;
+  if (k < ea.nedges) {
+    float norm = 0.0f;
+    for (int i = 0; i < ndof; ++i) {
+      float diff = ea.qe[k + ea.offset * i] - ea.qs[k + ea.offset * i];
+      norm += diff * diff;
+    }
+    norm = __builtin_sqrtf(norm);
+    T = (int)(norm / dq) + 1;
+    int g = 1;
+    if (T > 2) {
+      g = (int)((double)T * 0.6180339887498949);
+      if (g < 1) g = 1;
+      while (gcd_i(g, T) != 1) ++g;
+    }
+    testnum[k] = T;
+    stride[k] = g;
+    ea.edge_res[k] = 0;
+    atomicMax(max_testnum, T);
+    atomicAdd(total, (unsigned long long)T);
+  }
+}
+
+// Items for order positions [lo, hi) of every open edge in [e0, e1).
+// One atomic per wave: lanes reserve contiguous slots by prefix count.
+__global__ void __launch_bounds__(256)
+    edge_compact_kernel(EdgeArgs ea, int e0, int e1, int lo, int hi, int* __restrict__ item_edge,
+                        int* __restrict__ item_pos, int* __restrict__ nitems) {
+  int e = e0 + blockIdx.x * blockDim.x + threadIdx.x;
+  int cnt = 0;
+  if (e < e1) {
+    int T = ea.testnum[e];
+    int open = __hip_atomic_load(&ea.edge_res[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+    if (open && T > lo) cnt = (T < hi ? T : hi) - lo;
+  }
+  // wave-inclusive prefix sum of cnt
+  const int lane = threadIdx.x & 63;
+  int incl = cnt;
+  for (int d = 1; d < 64; d <<= 1) {
+    int v = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += v;
+  }
+  int wave_total = __shfl(incl, 63, 64);
+  int base = 0;
+  if (lane == 63 && wave_total > 0) base = atomicAdd(nitems, wave_total);
+  base = __shfl(base, 63, 64);
+  int o = base + incl - cnt;
+  for (int t = 0; t < cnt; ++t) {
+    item_edge[o + t] = e;
+    item_pos[o + t] = lo + t;
+  }
+}
+
+__global__ void fill_i32_kernel(int* p, int v, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void fill_u32_kernel(unsigned int* p, unsigned int v, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// ----------------------------------------------------------------------------
+// separating-vector kernel
+// ----------------------------------------------------------------------------
+
+struct Lds {
+  const float4* verts;
+  const int4* pairs;
+  const int* vinfo;
+  const unsigned short* adj;
+  float* R;   // [3][kMaxIter][kSvaBlock]
+  int* C;     // [kMaxIter][kSvaBlock]
+};
+
+// support_vertex_searcher::search_support_vertex (collision4.h:66-103):
+// first improvement over the stored neighbour order, restart at the new vertex.
+__device__ __forceinline__ int climb(const Lds& L, int p, const V3& dir) {
+  float mx = dot3(v3(L.verts[p]), dir);
+  int info = L.vinfo[p];
+  int i = info >> 8;
+  int end = i + (info & 255);
+  while (i < end) {
+    int v = L.adj[i];
+    float d = dot3(v3(L.verts[v]), dir);
+    if (d > mx) {
+      mx = d;
+      p = v;
+      info = L.vinfo[v];
+      i = info >> 8;
+      end = i + (info & 255);
+    } else {
+      ++i;
+    }
+  }
+  return p;
+}
+
+__device__ __forceinline__ V3 getR(const Lds& L, int j, int tid) {
+  return V3{L.R[(0 * kMaxIter + j) * kSvaBlock + tid], L.R[(1 * kMaxIter + j) * kSvaBlock + tid],
+            L.R[(2 * kMaxIter + j) * kSvaBlock + tid]};
+}
+__device__ __forceinline__ void putR(const Lds& L, int j, int tid, const V3& r) {
+  L.R[(0 * kMaxIter + j) * kSvaBlock + tid] = r.x;
+  L.R[(1 * kMaxIter + j) * kSvaBlock + tid] = r.y;
+  L.R[(2 * kMaxIter + j) * kSvaBlock + tid] = r.z;
+}
+
+// orthtrafo23 (collision4.h:111-134)
+struct Orth {
+  float M0, M1, M2, M3, M4, M5;
+};
+__device__ __forceinline__ V2 mult23(const Orth& o, const V3& x) {
+  return V2{(o.M0 * x.x + o.M1 * x.y) + o.M2 * x.z, (o.M3 * x.x + o.M4 * x.y) + o.M5 * x.z};
+}
+
+// find_half_plane (collision4.h:136-223), R[0..k-1] in LDS.
+__device__ __forceinline__ bool find_half_plane(const Lds& L, int tid, int k, V3& w, const V3& rk) {
+  float dp = dot3(w, rk);
+  if (dp > 0) return true;
+  Orth o;
+  {
+    float x = rk.x, y = rk.y, z = rk.z;
+    float d2 = x * x + y * y;
+    float d = __builtin_sqrtf(d2);
+    o.M0 = x * z / d; o.M1 = y * z / d; o.M2 = -d;
+    o.M3 = -y / d;    o.M4 = x / d;     o.M5 = 0.0f;
+  }
+  V2 ra = mult23(o, getR(L, 0, tid));
+  V2 rb = mult23(o, getR(L, 1, tid));
+  if (cross2(ra, rb) < 0) {
+    V2 t = ra;
+    ra = rb;
+    rb = t;
+  }
+  for (int i = 2; i < k; ++i) {
+    V2 T = mult23(o, getR(L, i, tid));
+    float rbcross = cross2(rb, T);
+    if (cross2(ra, T) > 0) {
+      if (rbcross > 0) rb = T;
+    } else {
+      if (rbcross < 0) ra = T;
+      else return false;
+    }
+  }
+  normalize2(ra);
+  normalize2(rb);
+  V2 T{ra.x + rb.x, ra.y + rb.y};
+  normalize2(T);
+  w.x = o.M0 * T.x + o.M3 * T.y;
+  w.y = o.M1 * T.x + o.M4 * T.y;
+  w.z = o.M2 * T.x + o.M5 * T.y;
+  return true;
+}
+
+template <int EDGE>
+__global__ void __launch_bounds__(kSvaBlock) sva_kernel(SvaArgs A) {
+  extern __shared__ int4 smem[];
+  // stage the scene image
+  {
+    const int n16 = A.img.bytes >> 4;
+    for (int i = threadIdx.x; i < n16; i += kSvaBlock) smem[i] = A.img.blob[i];
+  }
+  const char* base = reinterpret_cast<const char*>(smem);
+  Lds L;
+  L.verts = reinterpret_cast<const float4*>(base + A.img.off_verts);
+  L.pairs = reinterpret_cast<const int4*>(base + A.img.off_pairs);
+  L.vinfo = reinterpret_cast<const int*>(base + A.img.off_vinfo);
+  L.adj = reinterpret_cast<const unsigned short*>(base + A.img.off_adj);
+  L.R = reinterpret_cast<float*>(reinterpret_cast<char*>(smem) + A.img.bytes);
+  L.C = reinterpret_cast<int*>(L.R + 3 * kMaxIter * kSvaBlock);
+  __syncthreads();
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const unsigned long long lt_mask = (1ull << lane) - 1ull;
+  const int nitems = A.nitems >= 0 ? A.nitems : *A.nitems_dev;
+  const int M = A.img.M;
+  const int ndof = A.ndof;
+  const int PC = A.pairs_per_unit;
+  const int nch = A.units_per_item;
+  const unsigned nunits = (unsigned)nitems * (unsigned)nch;
+
+  // lane state: the current unit (item, pairs [pair, pend)) and its SVA state
+  int item = -1, edge = 0, pair = 0, pend = 0;
+  bool retired = false, need_init = false;
+  int k = 0, l = 0, p = 0, q = 0, vbP = 0, vbQ = 0;
+  bool sysP0 = true, sysQ0 = true;
+  V3 S{0, 0, 0}, w{0, 0, 0};
+  T12 tp, tq;
+#pragma unroll
+  for (int i = 0; i < 12; ++i) tp.m[i] = tq.m[i] = 0.0f;
+
+  unsigned q_base = 0, q_left = 0;
+  bool exhausted = false;
+  unsigned n_sep = 0, n_col = 0, n_dropped = 0, n_units = 0, n_iter = 0;
+
+  for (;;) {
+    // ---- refill idle lanes from the wave's queue of units -----------------
+    bool need = (item < 0) && !retired;
+    unsigned long long m = __ballot(need);
+    while (m != 0ull && !exhausted) {
+      if (q_left == 0) {
+        unsigned b = 0;
+        if (lane == 0) b = atomicAdd(A.counter, (unsigned)kChunk);
+        b = __shfl(b, 0, 64);
+        if (b >= nunits) {
+          exhausted = true;
+          break;
+        }
+        q_base = b;
+        q_left = nunits - b < (unsigned)kChunk ? nunits - b : (unsigned)kChunk;
+      }
+      unsigned rank = (unsigned)__popcll(m & lt_mask);
+      unsigned avail = (unsigned)__popcll(m);
+      unsigned take = avail < q_left ? avail : q_left;
+      if (need && rank < take) {
+        unsigned u = q_base + rank;
+        item = (int)(u / (unsigned)nch);
+        int ch = (int)(u - (unsigned)item * (unsigned)nch);
+        pair = ch * PC;
+        pend = min(M, pair + PC);
+        need = false;
+        need_init = true;
+        if (EDGE) edge = A.item_edge[item];
+      }
+      q_base += take;
+      q_left -= take;
+      m = __ballot(need);
+    }
+    if (need && exhausted) retired = true;
+    if (__ballot(item >= 0) == 0ull) break;
+    if (item < 0) continue;
+
+    // ---- start a pair: early exit, frames, S0 (collision4.h:228-244) -------
+    if (need_init) {
+      // Early exit: in edge mode once any sample of the edge is blocked; in
+      // point mode once a pair earlier in CRS order is known to collide (the
+      // reference returns the first colliding pair's code, so later pairs
+      // cannot change the result).
+      int gate;
+      if (EDGE) gate = __hip_atomic_load(&A.edge_res[edge], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+      else gate = (__hip_atomic_load(&A.word[item], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 4) <= pair;
+      if (gate) {
+        item = -1;
+        ++n_dropped;
+        continue;
+      }
+      int4 pd = L.pairs[pair];
+      vbP = pd.x;
+      vbQ = pd.y;
+      sysP0 = pd.z == 0;
+      sysQ0 = pd.w == 0;
+      if (!sysP0) tp = load_frame(A.frames, item, ndof, pd.z);
+      if (!sysQ0) tq = load_frame(A.frames, item, ndof, pd.w);
+      V3 vq = v3(L.verts[vbQ]), vp = v3(L.verts[vbP]);
+      V3 Pq = sysQ0 ? vq : t_apply(tq, vq);
+      V3 Pp = sysP0 ? vp : t_apply(tp, vp);
+      S = V3{Pq.x - Pp.x, Pq.y - Pp.y, Pq.z - Pp.z};
+      float norm = normalize3(S);
+      if (norm <= kEpsBelow) S.x = 1.0f;
+      p = vbP;
+      q = vbQ;
+      k = 0;
+      l = 0;
+      w = V3{0.0f, 0.0f, 0.0f};
+      need_init = false;
+    }
+    ++n_iter;
+
+    // ---- one iteration (collision4.h:266-327) ------------------------------
+    V3 Sp = sysP0 ? S : t_apply_rot_inv(tp, S);
+    V3 Sn{-S.x, -S.y, -S.z};
+    V3 Sq = sysQ0 ? Sn : t_apply_rot_inv(tq, Sn);
+    p = climb(L, p, Sp);
+    q = climb(L, q, Sq);
+    V3 vp = v3(L.verts[p]), vq = v3(L.verts[q]);
+    V3 p0 = sysP0 ? vp : t_apply(tp, vp);
+    V3 q0 = sysQ0 ? vq : t_apply(tq, vq);
+    V3 r{q0.x - p0.x, q0.y - p0.y, q0.z - p0.z};
+    normalize3(r);
+    float dp = dot3(S, r);
+
+    bool fin = false;
+    int code = 0;
+    if (dp >= -kEpsBelow) {
+      fin = true;
+    } else {
+      int pq = (p - vbP) | ((q - vbQ) << 16);
+      L.C[k * kSvaBlock + tid] = pq;
+      bool rep = false;
+      for (int j = 0; j < k; ++j)
+        if (L.C[j * kSvaBlock + tid] == pq) {
+          rep = true;
+          break;
+        }
+      if (rep) {
+        S = w;
+      } else {
+        if (k == 1) {
+          V3 r0 = getR(L, 0, tid);
+          w = V3{r0.x + r.x, r0.y + r.y, r0.z + r.z};
+          normalize3(w);
+        }
+        if (k >= 2 && !find_half_plane(L, tid, k, w, r)) {
+          fin = true;
+          code = l + 1;
+        } else {
+          putR(L, k, tid, r);
+          float b = -2.0f * dp;
+          S.x = S.x + b * r.x;
+          S.y = S.y + b * r.y;
+          S.z = S.z + b * r.z;
+          ++k;
+        }
+      }
+      if (!fin) {
+        ++l;
+        if (l >= kMaxIter) {
+          fin = true;
+          code = -1;
+        }
+      }
+    }
+
+    if (fin) {
+      if (code != 0) {
+        ++n_col;
+        if (EDGE) __hip_atomic_store(&A.edge_res[edge], code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else atomicMin(&A.word[item], ((pair + 1) << 4) | (code + 1));
+        item = -1;
+        ++n_units;
+      } else {
+        ++n_sep;
+        if (++pair == pend) {
+          item = -1;
+          ++n_units;
+        } else {
+          need_init = true;
+        }
+      }
+    }
+  }
+
+  if (A.tally != nullptr) {
+    // per-block sums in the (no longer used) per-lane scratch, one atomic per counter
+    __syncthreads();
+    unsigned* t = reinterpret_cast<unsigned*>(L.C);
+    if (tid < kTallyCount) t[tid] = 0u;
+    __syncthreads();
+    atomicAdd(&t[0], n_sep);
+    atomicAdd(&t[1], n_col);
+    atomicAdd(&t[2], n_dropped);
+    atomicAdd(&t[3], n_units);
+    atomicAdd(&t[4], n_iter);
+    __syncthreads();
+    if (tid < kTallyCount) atomicAdd(&A.tally[tid], (unsigned long long)t[tid]);
+  }
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// launchers
+// ----------------------------------------------------------------------------
+
+size_t sva_lds_bytes(const SceneImg& img) {
+  return (size_t)img.bytes + sizeof(float) * 3 * kMaxIter * kSvaBlock + sizeof(int) * kMaxIter * kSvaBlock;
+}
+
+int sva_max_lds_bytes() { return 160 * 1024; }
+
+int sva_grid(size_t lds_bytes, int device) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 1024;
+  int per_cu = (int)(prop.maxSharedMemoryPerMultiProcessor / (lds_bytes ? lds_bytes : 1));
+  if (per_cu < 1) per_cu = 1;
+  if (per_cu > 8) per_cu = 8;
+  return prop.multiProcessorCount * per_cu;
+}
+
+#define GPRM_FK_DISPATCH(KERNEL, GRID, BLOCK, ST, ...)                                      \
+  switch (rb.ndof) {                                                                        \
+    case 2: hipLaunchKernelGGL(KERNEL<2>, GRID, BLOCK, 0, ST, __VA_ARGS__); break;         \
+    case 3: hipLaunchKernelGGL(KERNEL<3>, GRID, BLOCK, 0, ST, __VA_ARGS__); break;         \
+    case 4: hipLaunchKernelGGL(KERNEL<4>, GRID, BLOCK, 0, ST, __VA_ARGS__); break;         \
+    case 5: hipLaunchKernelGGL(KERNEL<5>, GRID, BLOCK, 0, ST, __VA_ARGS__); break;         \
+    case 6: hipLaunchKernelGGL(KERNEL<6>, GRID, BLOCK, 0, ST, __VA_ARGS__); break;         \
+    case 7: hipLaunchKernelGGL(KERNEL<7>, GRID, BLOCK, 0, ST, __VA_ARGS__); break;         \
+    case 8: hipLaunchKernelGGL(KERNEL<8>, GRID, BLOCK, 0, ST, __VA_ARGS__); break;         \
+    default: hipLaunchKernelGGL(KERNEL<0>, GRID, BLOCK, 0, ST, __VA_ARGS__); break;        \
+  }
+
+hipError_t launch_fk_points(const DevRobot& rb, const float* q, long long offset, int n, float* frames,
+                            int* word, int check_bounds, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  dim3 grid((n + kFkBlock - 1) / kFkBlock), block(kFkBlock);
+  GPRM_FK_DISPATCH(fk_points_kernel, grid, block, st, rb, q, offset, n, frames, word, check_bounds);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize_points(const int* word, int* codes, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(finalize_points_kernel, dim3((n + 255) / 256), dim3(256), 0, st, word, codes, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_edge_fk(const DevRobot& rb, EdgeArgs ea, const int* item_edge, const int* item_pos,
+                          const int* nitems, int cap, float* frames, hipStream_t st) {
+  if (cap <= 0) return hipSuccess;
+  int blocks = (cap + kFkBlock - 1) / kFkBlock;
+  if (blocks > 4096) blocks = 4096;
+  dim3 grid(blocks), block(kFkBlock);
+  GPRM_FK_DISPATCH(fk_edges_kernel, grid, block, st, rb, ea, item_edge, item_pos, nitems, cap, frames);
+  return hipGetLastError();
+}
+
+hipError_t launch_sva(const SvaArgs& a, int edge_mode, int grid, hipStream_t st) {
+  size_t lds = sva_lds_bytes(a.img);
+  if (lds > (size_t)sva_max_lds_bytes()) return hipErrorInvalidValue;
+  const void* fn = edge_mode ? reinterpret_cast<const void*>(&sva_kernel<1>)
+                             : reinterpret_cast<const void*>(&sva_kernel<0>);
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  if (edge_mode)
+    hipLaunchKernelGGL(sva_kernel<1>, dim3(grid), dim3(kSvaBlock), lds, st, a);
+  else
+    hipLaunchKernelGGL(sva_kernel<0>, dim3(grid), dim3(kSvaBlock), lds, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_edge_prologue(const DevRobot& rb, float dq, EdgeArgs ea, int* testnum_out,
+                                int* stride_out, int* max_testnum, unsigned long long* total,
+                                hipStream_t st) {
+  if (ea.nedges <= 0) return hipSuccess;
+  dim3 grid((ea.nedges + 255) / 256), block(256);
+  hipLaunchKernelGGL(edge_prologue_kernel, grid, block, 0, st, rb.ndof, dq, ea, testnum_out, stride_out,
+                     max_testnum, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_edge_compact(EdgeArgs ea, int e0, int e1, int lo, int hi, int* item_edge,
+                               int* item_pos, int* nitems, hipStream_t st) {
+  if (e1 <= e0) return hipSuccess;
+  dim3 grid((e1 - e0 + 255) / 256), block(256);
+  hipLaunchKernelGGL(edge_compact_kernel, grid, block, 0, st, ea, e0, e1, lo, hi, item_edge, item_pos, nitems);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(int* p, int v, long long n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  long long b = (n + 255) / 256;
+  if (b > 4096) b = 4096;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)b), dim3(256), 0, st, p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(unsigned int* p, unsigned int v, long long n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  long long b = (n + 255) / 256;
+  if (b > 4096) b = 4096;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)b), dim3(256), 0, st, p, v, n);
+  return hipGetLastError();
+}
+
+}  // namespace gprm

@@ -1,0 +1,156 @@
+/*
+ * gprm.h -- C ABI of the MI355X collision-check path (libgprm.so).
+ *
+ * Drop-in for the configuration-space interface of GPUPRM: every entry point
+ * below replaces one member of Configspace<ndof> / RobotConfigspace<ndof>
+ * (paths relative to the reference repository root).  Layout conventions are
+ * the reference's:
+ *   - single configuration: AoS float[ndof];
+ *   - batches and edges: SoA, component i of item k at q[k + offset*i]
+ *     (prm/src/lib/prm/configspace.hpp:17-26);
+ *   - results: int per item; 0 = free, nonzero = blocked.  Codes are the
+ *     reference's: 1 out of bounds, l+1 no separating half plane found in
+ *     iteration l, -1 iteration cap reached (collision4.h:296-331).
+ * All entry points return GPRM_OK or a negative GPRM_E_* code; they never
+ * abort.  A context is used by one host thread at a time (the reference's
+ * contract, robotconfigspace.cpp:621-650); different contexts are independent.
+ * Host arrays are owned by the caller and copied at call time; result arrays
+ * of asynchronous requests are written by gprm_indicator2_wait.
+ */
+#ifndef GPRM_H
+#define GPRM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPRM_OK 0
+#define GPRM_E_INVALID (-1)   /* bad argument (null pointer, N < 0, bad index) */
+#define GPRM_E_SCENE (-2)     /* scene could not be read or is malformed */
+#define GPRM_E_HIP (-3)       /* HIP runtime error (no device, launch failure) */
+#define GPRM_E_CAPACITY (-4)  /* scene exceeds what the kernels stage in LDS */
+#define GPRM_E_REQUEST (-5)   /* unknown or already completed request id */
+#define GPRM_E_STATE (-6)     /* context not initialised (call gprm_init) */
+
+typedef struct gprm_ctx gprm_ctx;
+
+/* RobotConfigspace(std::string path, float dq, int nbuf)
+ * (prm/src/lib/robotspace/robotconfigspace.h:42, .cpp:84-98; scene directory
+ * format of load_config, robotconfigspace.cpp:100-208).  nbuf is the
+ * reference's per-call edge capacity; here larger calls are split, so it is
+ * only a sizing hint. */
+int gprm_create_from_dir(const char* path, float dq, int nbuf, gprm_ctx** out);
+
+/* RobotConfigspace(robot, polys, sys, N, from, to, M, mins, maxs, dq, nbuf)
+ * (robotconfigspace.h:34-41), flattened: the N polytopes' vertices (3 floats
+ * each), dsp/cnt (one per vertex, local offsets) and dest (local vertex ids)
+ * back to back; n[k], m[k] their sizes.  Pairs (from, to) sorted by from
+ * (polytope4data.h:225).  All arrays are copied. */
+int gprm_create(int ndof, const float* a, const float* alpha, const float* q, const float* d,
+                const int* types, int N, const int* sys, const int* n, const int* m,
+                const float* vertices, const int* dsp, const int* cnt, const int* dest,
+                const int* from, const int* to, int M, const float* mins, const float* maxs,
+                float dq, int nbuf, gprm_ctx** out);
+
+/* Configspace::init(ressource_rank, ressource_size) (configspace.hpp:16;
+ * robotconfigspace.cpp:212-287): selects device rank*count/size, uploads the
+ * scene. */
+int gprm_init(gprm_ctx* ctx, int rank, int size);
+
+/* Frees every device and host resource (the reference's clear() is
+ * unimplemented, robotconfigspace.cpp:289-295). */
+void gprm_destroy(gprm_ctx* ctx);
+
+/* Configspace::indicator(const float* q) (configspace.hpp:22;
+ * robotconfigspace.cpp:312-346): bounds check, then all pairs; *code gets the
+ * reference's return value. */
+int gprm_indicator1(gprm_ctx* ctx, const float* q, int* code);
+
+/* Configspace::indicator(q, res, N, offset) (configspace.hpp:17-20; the
+ * reference leaves it unimplemented, robotconfigspace.cpp:304-309): res[k]
+ * gets indicator1 of item k. */
+int gprm_indicator(gprm_ctx* ctx, const float* q, int* res, int N, int offset);
+
+/* Configspace::indicator2(qs, qe, res, N, offset) (configspace.hpp:23-26;
+ * robotconfigspace.cpp:616-752): res[k] != 0 iff some sample of edge k,
+ * discretised as the reference does (testnum = (int)(|qe-qs|_2/dq)+1 samples,
+ * robotconfigspace.cpp:660-673,409-420), is blocked.  No bounds check, as in
+ * the reference. */
+int gprm_indicator2(gprm_ctx* ctx, const float* qs, const float* qe, int* res, int N, int offset);
+
+/* indicator2_async / indicator2_async_wait (configspace.hpp:27-28;
+ * robotconfigspace.cpp:616-742).  Several requests may be in flight, each on
+ * its own stream; res is written when wait returns. */
+int gprm_indicator2_async(gprm_ctx* ctx, const float* qs, const float* qe, int* res, int N, int offset,
+                          int* request);
+int gprm_indicator2_wait(gprm_ctx* ctx, int request);
+
+/* Compressed indicator2 (configspace.hpp:30-32; unimplemented in the
+ * reference, robotconfigspace.cpp:766-772): for i < M and j < numqe[i],
+ * res[posqe[i]+j] is the indicator2 result of the edge from qs item i to qe
+ * item posqe[i]+j (qs has M items, stride offset; qe stride offset). */
+int gprm_indicator2_compressed(gprm_ctx* ctx, const float* qs, int M, const float* qe, int* res,
+                               const int* posqe, const int* numqe, int offset);
+
+/* check_boundaries, batch and single (configspace.hpp:33-37;
+ * robotconfigspace.cpp:779-802). */
+int gprm_check_boundaries(gprm_ctx* ctx, const float* q, int* res, int N, int offset);
+int gprm_check_boundaries1(gprm_ctx* ctx, const float* q, int* out);
+
+/* min(i), max(i), deltaq(), dim() (configspace.hpp:39-47) and
+ * get_numthreads_all() (robotconfigspace.h:112): samples checked by edge
+ * queries since creation, counted as the reference counts them (sum of
+ * testnum). */
+float gprm_min(gprm_ctx* ctx, int i);
+float gprm_max(gprm_ctx* ctx, int i);
+float gprm_deltaq(gprm_ctx* ctx);
+int gprm_dim(gprm_ctx* ctx);
+long long gprm_numthreads_all(gprm_ctx* ctx);
+
+/* Device-resident variants (no reference counterpart): the same computations
+ * on arrays already in device memory, enqueued on `stream` (a hipStream_t,
+ * NULL = the context's stream) and complete when the call returns.  For
+ * callers that keep configurations in HBM (the planner's sample buffers). */
+int gprm_indicator_dev(gprm_ctx* ctx, const float* d_q, int* d_res, int N, long long offset,
+                       void* stream);
+typedef struct {
+  long long samples_total;     /* sum of testnum: the reference's work for the call */
+  long long samples_scheduled; /* samples handed to the kernels after early exit */
+  int max_testnum;
+  int rounds;                  /* compaction rounds launched */
+} gprm_edge_stats;
+int gprm_indicator2_dev(gprm_ctx* ctx, const float* d_qs, const float* d_qe, int* d_res, int N,
+                        long long offset, void* stream, gprm_edge_stats* stats);
+
+/* Kernel timing and work counters (no reference counterpart; the reference's
+ * only counter is numthreads_all).  While enabled, every kinematics and
+ * separating-vector launch is bracketed by HIP events on its own stream and
+ * the kernels add their work counts; totals accumulate until read or
+ * re-enabled.  Enabling resets the totals. */
+typedef struct {
+  long long sva_launches;      /* separating-vector kernel launches */
+  double sva_ms;               /* their summed HIP-event durations */
+  long long fk_launches;       /* kinematics kernel launches */
+  double fk_ms;
+  long long pairs_separated;   /* separating-vector tests that ended separated (code 0) */
+  long long pairs_colliding;   /* ... that ended with a nonzero code */
+  long long units_dropped;     /* work units (a configuration's run of up to 8 pairs)
+                                  skipped by early exit */
+  long long units_done;        /* work units run to their end */
+  long long iterations;        /* separating-vector iterations run */
+} gprm_profile_stats;
+int gprm_profile_enable(gprm_ctx* ctx, int enable);
+int gprm_profile_read(gprm_ctx* ctx, gprm_profile_stats* out);
+
+/* Last error message of the context (or of creation when ctx is NULL). */
+const char* gprm_last_error(gprm_ctx* ctx);
+
+/* Scene generators of the benchmark workloads (1: 4-DOF boxes, 2: 6-DOF
+ * hulls, 3: 7-DOF hulls, 5: 7-DOF with random obstacle/hull counts), written
+ * in the reference's directory format. */
+int gprm_scene_generate(int workload, unsigned long long seed, const char* out_dir);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPRM_H */
