@@ -226,6 +226,7 @@ def main():
                                 "colliding": round(fpp["colliding"], 1)},
         "pair_tests_per_step": (prof.pairs_separated + prof.pairs_colliding) / args.steps,
         "iterations_per_step": prof.iterations / args.steps,
+        "bound_decided_per_step": prof.bound_separated / args.steps,
     }
 
     cpu = None

@@ -17,7 +17,8 @@ from typing import Optional
 
 import numpy as np
 
-__all__ = ["RobotConfigspace", "GprmError", "EdgeStats", "ProfileStats", "lib", "generate_scene", "LIB_PATH"]
+__all__ = ["RobotConfigspace", "Surrogate", "GprmError", "EdgeStats", "ProfileStats", "PrmParams", "PrmResult", "lib",
+           "generate_scene", "LIB_PATH"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgprm.so")
 
@@ -58,6 +59,44 @@ class ProfileStats(C.Structure):
         ("units_dropped", C.c_longlong),
         ("units_done", C.c_longlong),
         ("iterations", C.c_longlong),
+        ("bound_separated", C.c_longlong),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class PrmParams(C.Structure):
+    _fields_ = [
+        ("num", C.c_int),
+        ("D", C.c_float),
+        ("max_nodes", C.c_int),
+        ("max_steps", C.c_int),
+        ("seed", C.c_ulonglong),
+        ("nbuf", C.c_int),
+        ("surrogate", C.c_void_p),
+        ("bias_sampling", C.c_int),
+        ("order_edges", C.c_int),
+    ]
+
+
+class PrmResult(C.Structure):
+    _fields_ = [
+        ("found", C.c_int),
+        ("steps", C.c_int),
+        ("path_len", C.c_int),
+        ("nodes_left", C.c_longlong),
+        ("nodes_right", C.c_longlong),
+        ("point_checks", C.c_longlong),
+        ("edges_checked", C.c_longlong),
+        ("edges_free", C.c_longlong),
+        ("edge_samples", C.c_longlong),
+        ("model_predictions", C.c_longlong),
+        ("ms_total", C.c_double),
+        ("ms_sampling", C.c_double),
+        ("ms_near", C.c_double),
+        ("ms_edges", C.c_double),
+        ("ms_host", C.c_double),
     ]
 
     def as_dict(self):
@@ -116,6 +155,16 @@ def lib():
     L.gprm_last_error.argtypes = [vp]
     L.gprm_last_error.restype = C.c_char_p
     L.gprm_scene_generate.argtypes = [C.c_int, C.c_ulonglong, C.c_char_p]
+    L.gprm_prm_solve.argtypes = [vp, _FP, _FP, C.POINTER(PrmParams), C.POINTER(PrmResult), _FP, C.c_int]
+    L.gprm_surrogate_create.argtypes = [C.c_int, C.c_int, C.c_ulonglong, C.POINTER(vp)]
+    L.gprm_surrogate_destroy.argtypes = [vp]
+    L.gprm_surrogate_destroy.restype = None
+    L.gprm_surrogate_param_count.argtypes = []
+    L.gprm_surrogate_get_params.argtypes = [vp, _FP]
+    L.gprm_surrogate_set_params.argtypes = [vp, _FP]
+    L.gprm_surrogate_gradients.argtypes = [vp, vp, C.c_longlong, vp, C.c_int, _FP, _FP]
+    L.gprm_surrogate_train_step.argtypes = [vp, vp, C.c_longlong, vp, C.c_int, C.c_float, _FP]
+    L.gprm_surrogate_predict.argtypes = [vp, vp, C.c_int, C.c_longlong, vp, vp]
     L.gprm_profile_enable.argtypes = [vp, C.c_int]
     L.gprm_profile_read.argtypes = [vp, C.POINTER(ProfileStats)]
     _lib = L
@@ -141,6 +190,64 @@ def _f32(a) -> np.ndarray:
 
 def _i32(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Surrogate:
+    """Collision-probability MLP (7 -> 256 -> 256 -> 256 -> 1) trained on the
+    GPU by the library's own BF16 MFMA forward/backward kernels and Adam.  It
+    ranks configurations; the exact checker decides them."""
+
+    def __init__(self, ndof: int = 7, batch: int = 4096, seed: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().gprm_surrogate_create(int(ndof), int(batch), int(seed), C.byref(self._h)))
+        self.batch = batch
+        self.ndof = ndof
+
+    @property
+    def handle(self):
+        return self._h.value
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().gprm_surrogate_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(lib().gprm_surrogate_param_count(), np.float32)
+        _check(lib().gprm_surrogate_get_params(self._h, out.ctypes.data_as(_FP)))
+        return out
+
+    def set_params(self, p) -> None:
+        pa = _f32(p)
+        assert pa.size == lib().gprm_surrogate_param_count()
+        _check(lib().gprm_surrogate_set_params(self._h, pa.ctypes.data_as(_FP)))
+
+    def gradients(self, q, codes):
+        """q: float32 device tensor (ndof, n); codes: int32 device tensor (n,). Returns (grads, loss)."""
+        n = int(q.shape[1])
+        g = np.zeros(lib().gprm_surrogate_param_count(), np.float32)
+        loss = C.c_float(0)
+        _check(lib().gprm_surrogate_gradients(self._h, C.c_void_p(q.data_ptr()), n, C.c_void_p(codes.data_ptr()), n,
+                                              g.ctypes.data_as(_FP), C.byref(loss)))
+        return g, loss.value
+
+    def train_step(self, q, codes, lr: float = 1e-3, offset: Optional[int] = None) -> float:
+        n = int(q.shape[1]) if offset is None else int(q.shape[-1])
+        loss = C.c_float(0)
+        _check(lib().gprm_surrogate_train_step(self._h, C.c_void_p(q.data_ptr()), int(offset or n),
+                                               C.c_void_p(codes.data_ptr()), n, float(lr), C.byref(loss)))
+        return loss.value
+
+    def predict(self, q, out, stream=None) -> None:
+        n = int(q.shape[1])
+        _check(lib().gprm_surrogate_predict(self._h, C.c_void_p(q.data_ptr()), n, n, C.c_void_p(out.data_ptr()),
+                                            C.c_void_p(stream)))
 
 
 class RobotConfigspace:
@@ -306,6 +413,21 @@ class RobotConfigspace:
                                          C.c_void_p(res.data_ptr()), N, N, C.c_void_p(stream), C.byref(st)),
                self._ctx)
         return st
+
+    # -- single-process PRM ----------------------------------------------
+    def prm_solve(self, qstart, qgoal, num=2048, D=0.5, max_nodes=1 << 18, max_steps=200, seed=1,
+                  nbuf=1 << 21, surrogate=None, bias_sampling=False, order_edges=False, path_cap=1 << 16):
+        """Bidirectional PRM from qstart to qgoal (the reference's PRMSolver
+        process, prm/src/lib/prm/prmsolver.cpp:1973-2187, on one GPU).
+        Returns (PrmResult, path (n, ndof) or None)."""
+        p = PrmParams(int(num), float(D), int(max_nodes), int(max_steps), int(seed), int(nbuf),
+                      C.c_void_p(surrogate.handle if surrogate is not None else None),
+                      int(bool(bias_sampling)), int(bool(order_edges)))
+        r = PrmResult()
+        path = np.zeros((path_cap, self.ndof), np.float32)
+        _check(lib().gprm_prm_solve(self._ctx, _f32(qstart).ctypes.data_as(_FP), _f32(qgoal).ctypes.data_as(_FP),
+                                    C.byref(p), C.byref(r), path.ctypes.data_as(_FP), int(path_cap)), self._ctx)
+        return r, (path[:min(r.path_len, path_cap)].copy() if r.found else None)
 
     # -- kernel timing (HIP events) and work counters ---------------------
     def profile_enable(self, enable: bool = True) -> None:

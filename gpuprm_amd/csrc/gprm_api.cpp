@@ -27,7 +27,7 @@ namespace {
 
 constexpr int kItemBudget = 1 << 23;   // configurations per kinematics/SVA chunk
 constexpr int kFirstWindow = 2;        // sample positions in the first edge round
-constexpr int kPairsPerUnit = 8;       // pairs a lane claims at once
+constexpr int kPairsPerUnit = 32;      // pairs a lane claims at once
 
 thread_local std::string g_create_error;
 
@@ -49,7 +49,7 @@ struct DevBuf {
 // one stream per outstanding request, robotconfigspace.cpp:627-650).
 struct Slot {
   hipStream_t stream = nullptr;
-  DevBuf frames, status, item_edge, item_pos, counters, qs, qe, res, testnum, stride, misc, tally;
+  DevBuf frames, status, item_edge, item_pos, counters, qs, qe, res, testnum, stride, misc, tally, hist;
   int* host_res = nullptr;
   int N = 0;
   bool busy = false;
@@ -118,51 +118,98 @@ void fill_robot(const Scene& s, DevRobot* rb) {
 
 size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-// Pack the scene image (layout documented at SceneImg).  Pair order is the
-// reference's CRS order (polytope4data.h:343-357): by `from`, then as given.
+// Pack the scene image (layout documented at SceneImg).  The kernel visits
+// pairs grouped by the frames they need; each pair keeps its index in the
+// reference's CRS order (polytope4data.h:343-357: by `from`, then as given),
+// which decides which colliding pair's code a configuration returns.
 int build_image(gprm_ctx* c, std::vector<char>* out) {
   const Scene& s = c->scene;
   const int N = (int)s.polys.size();
-  std::vector<int> vbase(N), abase(N);
-  int sumn = 0, summ = 0;
+  std::vector<int> vbase(N);
+  int sumn = 0;
   for (int k = 0; k < N; ++k) {
-    vbase[k] = sumn; abase[k] = summ;
-    sumn += s.polys[k].n(); summ += s.polys[k].m();
+    vbase[k] = sumn;
+    sumn += s.polys[k].n();
   }
   if (sumn >= 65536) return fail(c, GPRM_E_CAPACITY, "scene has 65536 or more vertices");
-  if (summ >= (1 << 23)) return fail(c, GPRM_E_CAPACITY, "scene adjacency too large");
+  if (N >= 65536) return fail(c, GPRM_E_CAPACITY, "scene has 65536 or more polytopes");
+  // adjacency rows padded to multiples of 4 entries
+  std::vector<int> row_off(sumn);
+  size_t nadj = 0;
+  for (int k = 0; k < N; ++k)
+    for (int i = 0; i < s.polys[k].n(); ++i) {
+      if (s.polys[k].cnt[i] > 255) return fail(c, GPRM_E_CAPACITY, "vertex degree above 255");
+      row_off[vbase[k] + i] = (int)nadj;
+      nadj += (size_t)((s.polys[k].cnt[i] + 3) / 4) * 4;
+    }
+  if (nadj >= (1u << 23)) return fail(c, GPRM_E_CAPACITY, "scene adjacency too large");
   const int M = (int)s.from.size();
   SceneImg& img = c->img;
-  img.sumn = sumn; img.summ = summ; img.M = M; img.N = N;
+  img.sumn = sumn; img.summ = (int)nadj; img.M = M; img.N = N;
   size_t off = 0;
   img.off_verts = (int)off; off = align16(off + 16 * (size_t)sumn);
   img.off_pairs = (int)off; off = align16(off + 16 * (size_t)M);
-  img.off_vinfo = (int)off; off = align16(off + 4 * (size_t)sumn);
-  img.off_adj = (int)off;   off = align16(off + 2 * (size_t)summ);
+  img.off_adj = (int)off;   off = align16(off + 2 * nadj);
+  img.off_spheres = (int)off; off = align16(off + 16 * (size_t)N);
   img.bytes = (int)off;
   out->assign(off, 0);
   char* b = out->data();
   for (int k = 0; k < N; ++k) {
     const Polytope& p = s.polys[k];
     for (int i = 0; i < p.n(); ++i) {
+      const int g = vbase[k] + i;
+      int info = (row_off[g] << 8) | p.cnt[i];
       float v[4] = {p.verts[3 * i], p.verts[3 * i + 1], p.verts[3 * i + 2], 0.0f};
-      std::memcpy(b + img.off_verts + 16 * (size_t)(vbase[k] + i), v, 16);
-      if (p.cnt[i] > 255) return fail(c, GPRM_E_CAPACITY, "vertex degree above 255");
-      int info = ((abase[k] + p.dsp[i]) << 8) | p.cnt[i];
-      std::memcpy(b + img.off_vinfo + 4 * (size_t)(vbase[k] + i), &info, 4);
-    }
-    for (int j = 0; j < p.m(); ++j) {
-      unsigned short g = (unsigned short)(vbase[k] + p.dest[j]);
-      std::memcpy(b + img.off_adj + 2 * (size_t)(abase[k] + j), &g, 2);
+      std::memcpy(&v[3], &info, 4);
+      std::memcpy(b + img.off_verts + 16 * (size_t)g, v, 16);
+      const int padded = (p.cnt[i] + 3) / 4 * 4;
+      for (int j = 0; j < padded; ++j) {
+        unsigned short id = (unsigned short)(j < p.cnt[i] ? vbase[k] + p.dest[p.dsp[i] + j] : g);
+        std::memcpy(b + img.off_adj + 2 * (size_t)(row_off[g] + j), &id, 2);
+      }
     }
   }
-  for (int i = 0; i < M; ++i) {
-    int pr[4] = {vbase[s.from[i]], vbase[s.to[i]], s.sys[s.from[i]], s.sys[s.to[i]]};
-    std::memcpy(b + img.off_pairs + 16 * (size_t)i, pr, 16);
+  // bounding spheres: centre of the vertex bounding box, radius the largest
+  // vertex distance (double), enlarged by a relative 1e-6 and rounded up
+  for (int k = 0; k < N; ++k) {
+    const Polytope& p = s.polys[k];
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < p.n(); ++i)
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min(lo[a], (double)p.verts[3 * i + a]);
+        hi[a] = std::max(hi[a], (double)p.verts[3 * i + a]);
+      }
+    float c[3];
+    for (int a = 0; a < 3; ++a) c[a] = (float)(0.5 * (lo[a] + hi[a]));
+    double r2 = 0.0;
+    for (int i = 0; i < p.n(); ++i) {
+      double d2 = 0.0;
+      for (int a = 0; a < 3; ++a) {
+        double dd = (double)p.verts[3 * i + a] - (double)c[a];
+        d2 += dd * dd;
+      }
+      r2 = std::max(r2, d2);
+    }
+    double r = std::sqrt(r2) * (1.0 + 1e-6) + 1e-30;
+    float rf = (float)r;
+    if ((double)rf < r) rf = std::nextafter(rf, INFINITY);
+    float sp[4] = {c[0], c[1], c[2], rf};
+    std::memcpy(b + img.off_spheres + 16 * (size_t)k, sp, 16);
+  }
+  std::vector<int> order(M);
+  for (int i = 0; i < M; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    int kx = s.sys[s.to[x]] * 256 + s.sys[s.from[x]], ky = s.sys[s.to[y]] * 256 + s.sys[s.from[y]];
+    return kx < ky;
+  });
+  for (int j = 0; j < M; ++j) {
+    const int i = order[j];
+    int pr[4] = {vbase[s.from[i]] | (vbase[s.to[i]] << 16), s.sys[s.from[i]] | (s.sys[s.to[i]] << 8), i,
+                 s.from[i] | (s.to[i] << 16)};
+    std::memcpy(b + img.off_pairs + 16 * (size_t)j, pr, 16);
   }
   if (sva_lds_bytes(img) > (size_t)sva_max_lds_bytes())
-    return fail(c, GPRM_E_CAPACITY, "scene image of " + std::to_string(img.bytes) +
-                                        " bytes does not fit in LDS with the per-lane scratch");
+    return fail(c, GPRM_E_CAPACITY, "scene image of " + std::to_string(img.bytes) + " bytes does not fit in LDS");
   return GPRM_OK;
 }
 
@@ -241,6 +288,7 @@ int prof_harvest(gprm_ctx* c, Slot& sl) {
     c->prof.units_dropped += (long long)t[2];
     c->prof.units_done += (long long)t[3];
     c->prof.iterations += (long long)t[4];
+    c->prof.bound_separated += (long long)t[5];
     sl.tally_pending = false;
   }
   return GPRM_OK;
@@ -255,6 +303,7 @@ int run_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n,
   GPRM_HIP(sl.frames.ensure(sizeof(float) * 12 * nd * (size_t)chunk), "hipMalloc frames");
   GPRM_HIP(sl.status.ensure(sizeof(int) * (size_t)chunk), "hipMalloc words");
   const int nchunks = (n + chunk - 1) / chunk;
+  GPRM_HIP(sl.hist.ensure(sizeof(float) * 4 * kMaxIter * (size_t)c->grid * sva_block_for(c->img)), "hipMalloc history");
   GPRM_HIP(sl.counters.ensure(sizeof(unsigned) * (size_t)nchunks), "hipMalloc counters");
   GPRM_HIP(launch_fill_u32(sl.counters.as<unsigned>(), 0u, nchunks, st), "fill");
   for (int ci = 0; ci < nchunks; ++ci) {
@@ -268,6 +317,7 @@ int run_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n,
     a.img = c->img;
     a.ndof = nd;
     a.frames = sl.frames.as<float>();
+    a.hist = sl.hist.as<float4>();
     a.nitems = cn;
     a.counter = sl.counters.as<unsigned>() + ci;
     a.pairs_per_unit = kPairsPerUnit;
@@ -336,6 +386,7 @@ int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long 
   GPRM_HIP(sl.frames.ensure(sizeof(float) * 12 * nd * (size_t)std::max(cap, 1)), "hipMalloc frames");
   GPRM_HIP(sl.item_edge.ensure(sizeof(int) * (size_t)std::max(cap, 1)), "hipMalloc items");
   GPRM_HIP(sl.item_pos.ensure(sizeof(int) * (size_t)std::max(cap, 1)), "hipMalloc items");
+  GPRM_HIP(sl.hist.ensure(sizeof(float) * 4 * kMaxIter * (size_t)c->grid * sva_block_for(c->img)), "hipMalloc history");
   // per launch: [0] item count, [1] work counter
   GPRM_HIP(sl.counters.ensure(sizeof(int) * 2 * std::max<size_t>(plan.size(), 1)), "hipMalloc counters");
   GPRM_HIP(launch_fill_u32(sl.counters.as<unsigned>(), 0u, 2 * (long long)plan.size(), st), "fill");
@@ -355,6 +406,7 @@ int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long 
     a.img = c->img;
     a.ndof = nd;
     a.frames = sl.frames.as<float>();
+    a.hist = sl.hist.as<float4>();
     a.nitems = -1;
     a.nitems_dev = cnt;
     a.counter = reinterpret_cast<unsigned*>(cnt + 1);
@@ -490,7 +542,7 @@ int gprm_init(gprm_ctx* c, int rank, int size) {
   GPRM_HIP(c->img_buf.ensure(image.size()), "hipMalloc scene");
   GPRM_HIP(hipMemcpy(c->img_buf.p, image.data(), image.size(), hipMemcpyHostToDevice), "copy scene");
   c->img.blob = c->img_buf.as<int4>();
-  c->grid = sva_grid(sva_lds_bytes(c->img), c->device);
+  c->grid = sva_grid(c->img, c->device);
   c->inited = true;
   return GPRM_OK;
 }

@@ -172,7 +172,6 @@ __global__ void __launch_bounds__(kFkBlock)
     for (int j = 0; j < nd; ++j)
       if (qq[j] < rb.mins[j] || qq[j] > rb.maxs[j]) { st = 1; break; }
     word[i] = st ? kWordOutOfBounds : kWordFree;
-    if (st) return;
   } else {
     word[i] = kWordFree;
   }
@@ -287,6 +286,46 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Radius neighbours (get_near_vertices, prm/src/lib/prm/prmsolver.cpp:
+// 2532-2574): for query j and stored node l, |q_l - query_j|^2 < D^2 in float
+// with the reference's summation order.  One block per query; matches are
+// appended (query, node, distance) with one atomic per wave.
+__global__ void __launch_bounds__(256)
+    near_kernel(int ndof, const float* __restrict__ queries, int nq, long long qoff,
+                const float* __restrict__ nodes, int nn, long long noff, float D2, int* __restrict__ out_q,
+                int* __restrict__ out_node, float* __restrict__ out_d2, int* __restrict__ count, int cap) {
+  const int j = blockIdx.x;
+  if (j >= nq) return;
+  float qr[kMaxDofDev];
+  for (int i = 0; i < ndof; ++i) qr[i] = queries[j + qoff * i];
+  const int lane = threadIdx.x & 63;
+  for (int base = 0; base < nn; base += blockDim.x) {
+    const int l = base + threadIdx.x;
+    float normsq = 0.0f;
+    bool hit = false;
+    if (l < nn) {
+      for (int i = 0; i < ndof; ++i) {
+        float diff = nodes[l + noff * i] - qr[i];
+        normsq += diff * diff;
+      }
+      hit = normsq < D2;
+    }
+    unsigned long long m = __ballot(hit);
+    if (m == 0ull) continue;
+    int b = 0;
+    if (lane == 0) b = atomicAdd(count, __popcll(m));
+    b = __shfl(b, 0, 64);
+    if (hit) {
+      int o = b + __popcll(m & ((1ull << lane) - 1ull));
+      if (o < cap) {
+        out_q[o] = j;
+        out_node[o] = l;
+        out_d2[o] = normsq;
+      }
+    }
+  }
+}
+
 __global__ void fill_i32_kernel(int* p, int v, long long n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     p[i] = v;
@@ -300,46 +339,57 @@ __global__ void fill_u32_kernel(unsigned int* p, unsigned int v, long long n) {
 // separating-vector kernel
 // ----------------------------------------------------------------------------
 
+// Scene image as staged in LDS (layout documented at SceneImg).
 struct Lds {
-  const float4* verts;
-  const int4* pairs;
-  const int* vinfo;
-  const unsigned short* adj;
-  float* R;   // [3][kMaxIter][kSvaBlock]
-  int* C;     // [kMaxIter][kSvaBlock]
+  const float4* verts;          // (x, y, z, info bits): info = row offset << 8 | degree
+  const int4* pairs;            // (vbP | vbQ << 16, sysP | sysQ << 8, CRS index, kP | kQ << 16)
+  const float4* spheres;        // per polytope: bounding sphere (centre, radius), own frame
+  const unsigned short* adj;    // rows of 4k neighbour ids, padded with the row's own vertex
 };
 
+// Per-lane iteration history R[k] (x, y, z) and combs[k] (w: p | q << 16),
+// in global memory: only iterations that do not separate touch it.
+struct Hist {
+  float4* h;  // [kMaxIter] for this lane
+};
+
+__device__ __forceinline__ int info_of(const float4& r) { return __float_as_int(r.w); }
+
 // support_vertex_searcher::search_support_vertex (collision4.h:66-103):
-// first improvement over the stored neighbour order, restart at the new vertex.
+// first improvement over the stored neighbour order, restart at the new
+// vertex.  A vertex's neighbours are read 4 at a time with independent loads;
+// the first of them (in stored order) whose value exceeds the current maximum
+// is the sequential scan's move, found as the lowest set bit of a compare
+// mask.  The new maximum is recomputed from the same operands, so it is the
+// same float the scan would keep.
 __device__ __forceinline__ int climb(const Lds& L, int p, const V3& dir) {
-  float mx = dot3(v3(L.verts[p]), dir);
-  int info = L.vinfo[p];
-  int i = info >> 8;
-  int end = i + (info & 255);
-  while (i < end) {
-    int v = L.adj[i];
-    float d = dot3(v3(L.verts[v]), dir);
-    if (d > mx) {
-      mx = d;
-      p = v;
-      info = L.vinfo[v];
-      i = info >> 8;
-      end = i + (info & 255);
+  float4 rec = L.verts[p];
+  float mx = dot3(v3(rec), dir);
+  int info = info_of(rec);
+  int c = 0;
+  for (;;) {
+    const int off = info >> 8, cnt = info & 255;
+    if (c >= cnt) break;
+    const uint2 ids = *reinterpret_cast<const uint2*>(L.adj + off + c);
+    const float4 r0 = L.verts[ids.x & 0xffff], r1 = L.verts[ids.x >> 16];
+    const float4 r2 = L.verts[ids.y & 0xffff], r3 = L.verts[ids.y >> 16];
+    // padding entries are the vertex itself: same operands, same value, never > mx
+    unsigned bits = (dot3(v3(r0), dir) > mx) ? 1u : 0u;
+    bits |= (dot3(v3(r1), dir) > mx) ? 2u : 0u;
+    bits |= (dot3(v3(r2), dir) > mx) ? 4u : 0u;
+    bits |= (dot3(v3(r3), dir) > mx) ? 8u : 0u;
+    if (bits) {
+      const unsigned long long packed = ((unsigned long long)ids.y << 32) | ids.x;
+      p = (int)((packed >> (16 * __builtin_ctz(bits))) & 0xffff);
+      rec = L.verts[p];
+      mx = dot3(v3(rec), dir);
+      info = info_of(rec);
+      c = 0;
     } else {
-      ++i;
+      c += 4;
     }
   }
   return p;
-}
-
-__device__ __forceinline__ V3 getR(const Lds& L, int j, int tid) {
-  return V3{L.R[(0 * kMaxIter + j) * kSvaBlock + tid], L.R[(1 * kMaxIter + j) * kSvaBlock + tid],
-            L.R[(2 * kMaxIter + j) * kSvaBlock + tid]};
-}
-__device__ __forceinline__ void putR(const Lds& L, int j, int tid, const V3& r) {
-  L.R[(0 * kMaxIter + j) * kSvaBlock + tid] = r.x;
-  L.R[(1 * kMaxIter + j) * kSvaBlock + tid] = r.y;
-  L.R[(2 * kMaxIter + j) * kSvaBlock + tid] = r.z;
 }
 
 // orthtrafo23 (collision4.h:111-134)
@@ -350,8 +400,8 @@ __device__ __forceinline__ V2 mult23(const Orth& o, const V3& x) {
   return V2{(o.M0 * x.x + o.M1 * x.y) + o.M2 * x.z, (o.M3 * x.x + o.M4 * x.y) + o.M5 * x.z};
 }
 
-// find_half_plane (collision4.h:136-223), R[0..k-1] in LDS.
-__device__ __forceinline__ bool find_half_plane(const Lds& L, int tid, int k, V3& w, const V3& rk) {
+// find_half_plane (collision4.h:136-223), R[0..k-1] in the lane's history.
+__device__ __forceinline__ bool find_half_plane(const float4* __restrict__ H, int k, V3& w, const V3& rk) {
   float dp = dot3(w, rk);
   if (dp > 0) return true;
   Orth o;
@@ -362,15 +412,15 @@ __device__ __forceinline__ bool find_half_plane(const Lds& L, int tid, int k, V3
     o.M0 = x * z / d; o.M1 = y * z / d; o.M2 = -d;
     o.M3 = -y / d;    o.M4 = x / d;     o.M5 = 0.0f;
   }
-  V2 ra = mult23(o, getR(L, 0, tid));
-  V2 rb = mult23(o, getR(L, 1, tid));
+  V2 ra = mult23(o, v3(H[0]));
+  V2 rb = mult23(o, v3(H[1]));
   if (cross2(ra, rb) < 0) {
     V2 t = ra;
     ra = rb;
     rb = t;
   }
   for (int i = 2; i < k; ++i) {
-    V2 T = mult23(o, getR(L, i, tid));
+    V2 T = mult23(o, v3(H[i]));
     float rbcross = cross2(rb, T);
     if (cross2(ra, T) > 0) {
       if (rbcross > 0) rb = T;
@@ -389,27 +439,50 @@ __device__ __forceinline__ bool find_half_plane(const Lds& L, int tid, int k, V3
   return true;
 }
 
-template <int EDGE>
-__global__ void __launch_bounds__(kSvaBlock) sva_kernel(SvaArgs A) {
+// Iteration that did not separate (collision4.h:301-323): repetition test,
+// w update, half plane, reflection.  Returns a nonzero code when the pair is
+// decided (l+1: no half plane), else 0.  Taken by few iterations.
+__device__ __forceinline__ int sva_slow(float4* __restrict__ H, int& k, int l, int pq, const V3& r, float dp,
+                                     V3& S, V3& w) {
+  H[k] = make_float4(r.x, r.y, r.z, __int_as_float(pq));
+  for (int j = 0; j < k; ++j)
+    if (__float_as_int(H[j].w) == pq) {
+      S = w;  // (p,q) seen before
+      return 0;
+    }
+  if (k == 1) {
+    V3 r0 = v3(H[0]);
+    w = V3{r0.x + r.x, r0.y + r.y, r0.z + r.z};
+    normalize3(w);
+  }
+  if (k >= 2 && !find_half_plane(H, k, w, r)) return l + 1;
+  float b = -2.0f * dp;
+  S.x = S.x + b * r.x;
+  S.y = S.y + b * r.y;
+  S.z = S.z + b * r.z;
+  ++k;
+  return 0;
+}
+
+template <int EDGE, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) sva_kernel(SvaArgs A) {
   extern __shared__ int4 smem[];
-  // stage the scene image
   {
     const int n16 = A.img.bytes >> 4;
-    for (int i = threadIdx.x; i < n16; i += kSvaBlock) smem[i] = A.img.blob[i];
+    for (int i = threadIdx.x; i < n16; i += BLOCK) smem[i] = A.img.blob[i];
   }
   const char* base = reinterpret_cast<const char*>(smem);
   Lds L;
   L.verts = reinterpret_cast<const float4*>(base + A.img.off_verts);
   L.pairs = reinterpret_cast<const int4*>(base + A.img.off_pairs);
-  L.vinfo = reinterpret_cast<const int*>(base + A.img.off_vinfo);
   L.adj = reinterpret_cast<const unsigned short*>(base + A.img.off_adj);
-  L.R = reinterpret_cast<float*>(reinterpret_cast<char*>(smem) + A.img.bytes);
-  L.C = reinterpret_cast<int*>(L.R + 3 * kMaxIter * kSvaBlock);
+  L.spheres = reinterpret_cast<const float4*>(base + A.img.off_spheres);
   __syncthreads();
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
+  float4* __restrict__ H = A.hist + ((size_t)blockIdx.x * BLOCK + tid) * kMaxIter;
   const int nitems = A.nitems >= 0 ? A.nitems : *A.nitems_dev;
   const int M = A.img.M;
   const int ndof = A.ndof;
@@ -417,10 +490,13 @@ __global__ void __launch_bounds__(kSvaBlock) sva_kernel(SvaArgs A) {
   const int nch = A.units_per_item;
   const unsigned nunits = (unsigned)nitems * (unsigned)nch;
 
-  // lane state: the current unit (item, pairs [pair, pend)) and its SVA state
-  int item = -1, edge = 0, pair = 0, pend = 0;
+  // lane state: the current unit (item, pairs [pair, pend) of the stored
+  // order) and the separating-vector state of its current pair
+  int item = -1, edge = 0, pair = 0, pstart = 0, pend = 0, gate = 0x7fffffff;
   bool retired = false, need_init = false;
-  int k = 0, l = 0, p = 0, q = 0, vbP = 0, vbQ = 0;
+  int k = 0, l = 0, p = 0, q = 0, vbP = 0, vbQ = 0, crs = 0, fsP = -1, fsQ = -1, ckQ = -1;
+  V3 cQw{0, 0, 0};
+  float rQ = 0.0f;
   bool sysP0 = true, sysQ0 = true;
   V3 S{0, 0, 0}, w{0, 0, 0};
   T12 tp, tq;
@@ -429,7 +505,7 @@ __global__ void __launch_bounds__(kSvaBlock) sva_kernel(SvaArgs A) {
 
   unsigned q_base = 0, q_left = 0;
   bool exhausted = false;
-  unsigned n_sep = 0, n_col = 0, n_dropped = 0, n_units = 0, n_iter = 0;
+  unsigned n_sep = 0, n_col = 0, n_dropped = 0, n_units = 0, n_iter = 0, n_bound = 0;
 
   for (;;) {
     // ---- refill idle lanes from the wave's queue of units -----------------
@@ -451,14 +527,26 @@ __global__ void __launch_bounds__(kSvaBlock) sva_kernel(SvaArgs A) {
       unsigned avail = (unsigned)__popcll(m);
       unsigned take = avail < q_left ? avail : q_left;
       if (need && rank < take) {
+        // chunk-major order: the lanes of a wave run the same pairs on
+        // consecutive items, so they walk the same polytopes in step
         unsigned u = q_base + rank;
-        item = (int)(u / (unsigned)nch);
-        int ch = (int)(u - (unsigned)item * (unsigned)nch);
+        int ch = (int)(u / (unsigned)nitems);
+        item = (int)(u - (unsigned)ch * (unsigned)nitems);
         pair = ch * PC;
+        pstart = pair;
         pend = min(M, pair + PC);
         need = false;
         need_init = true;
-        if (EDGE) edge = A.item_edge[item];
+        fsP = fsQ = ckQ = -1;
+        // early exit: in edge mode once any sample of the edge is blocked; in
+        // point mode for pairs after (in CRS order) a pair known to collide,
+        // since the reference returns the first colliding pair's code
+        if (EDGE) {
+          edge = A.item_edge[item];
+          gate = __hip_atomic_load(&A.edge_res[edge], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ? 0 : 0x7fffffff;
+        } else {
+          gate = __hip_atomic_load(&A.word[item], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 4;
+        }
       }
       q_base += take;
       q_left -= take;
@@ -468,27 +556,24 @@ __global__ void __launch_bounds__(kSvaBlock) sva_kernel(SvaArgs A) {
     if (__ballot(item >= 0) == 0ull) break;
     if (item < 0) continue;
 
-    // ---- start a pair: early exit, frames, S0 (collision4.h:228-244) -------
+    // ---- start a pair: gate, frames, S0 (collision4.h:228-244) -------------
     if (need_init) {
-      // Early exit: in edge mode once any sample of the edge is blocked; in
-      // point mode once a pair earlier in CRS order is known to collide (the
-      // reference returns the first colliding pair's code, so later pairs
-      // cannot change the result).
-      int gate;
-      if (EDGE) gate = __hip_atomic_load(&A.edge_res[edge], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-      else gate = (__hip_atomic_load(&A.word[item], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 4) <= pair;
-      if (gate) {
-        item = -1;
+      int4 pd = L.pairs[pair];
+      crs = pd.z;
+      // the gate value loaded at the unit's start is consulted from its second
+      // pair on, so its memory latency overlaps the first pair's work
+      if (pair != pstart && (EDGE ? gate == 0 : gate <= crs)) {
         ++n_dropped;
+        if (++pair == pend) item = -1;
         continue;
       }
-      int4 pd = L.pairs[pair];
-      vbP = pd.x;
-      vbQ = pd.y;
-      sysP0 = pd.z == 0;
-      sysQ0 = pd.w == 0;
-      if (!sysP0) tp = load_frame(A.frames, item, ndof, pd.z);
-      if (!sysQ0) tq = load_frame(A.frames, item, ndof, pd.w);
+      vbP = pd.x & 0xffff;
+      vbQ = pd.x >> 16;
+      const int sP = pd.y & 255, sQ = pd.y >> 8;
+      sysP0 = sP == 0;
+      sysQ0 = sQ == 0;
+      if (!sysP0 && sP != fsP) { tp = load_frame(A.frames, item, ndof, sP); fsP = sP; }
+      if (!sysQ0 && sQ != fsQ) { tq = load_frame(A.frames, item, ndof, sQ); fsQ = sQ; }
       V3 vq = v3(L.verts[vbQ]), vp = v3(L.verts[vbP]);
       V3 Pq = sysQ0 ? vq : t_apply(tq, vq);
       V3 Pp = sysP0 ? vp : t_apply(tp, vp);
@@ -501,10 +586,46 @@ __global__ void __launch_bounds__(kSvaBlock) sva_kernel(SvaArgs A) {
       l = 0;
       w = V3{0.0f, 0.0f, 0.0f};
       need_init = false;
+
+      // Iteration 0 returns 0 whenever S.normalize(q0 - p0) >= -1e-6 for the
+      // support vertices p0, q0 the climbs find.  If the bounding spheres show
+      // S.(q - p) > 0 with a wide margin for every vertex p of P and q of Q,
+      // that holds whichever vertices the climbs would pick, so the pair test
+      // ends here exactly as the reference's does (code 0, one iteration).
+      // The margin, 2^-10 of the magnitudes involved, exceeds the rounding
+      // error of every quantity in the test and in the reference's iteration
+      // by orders of magnitude; a NaN direction never passes.
+      {
+        const int kP = pd.w & 0xffff, kQ = pd.w >> 16;
+        if (kQ != ckQ) {
+          const float4 sq = L.spheres[kQ];
+          cQw = sysQ0 ? v3(sq) : t_apply(tq, v3(sq));
+          rQ = sq.w;
+          ckQ = kQ;
+        }
+        const float4 sp = L.spheres[kP];
+        const V3 cPw = sysP0 ? v3(sp) : t_apply(tp, v3(sp));
+        const float rad = sp.w + rQ;
+        const float sn1 = fabsf(S.x) + fabsf(S.y) + fabsf(S.z);
+        const float gap = dot3(S, V3{cQw.x - cPw.x, cQw.y - cPw.y, cQw.z - cPw.z}) - sn1 * rad;
+        const float mag = fabsf(cQw.x) + fabsf(cQw.y) + fabsf(cQw.z) + fabsf(cPw.x) + fabsf(cPw.y) +
+                          fabsf(cPw.z) + rad;
+        if (gap > 0x1p-10f * mag * sn1) {
+          ++n_bound;
+          ++n_sep;
+          if (++pair == pend) {
+            item = -1;
+            ++n_units;
+          } else {
+            need_init = true;
+          }
+          continue;
+        }
+      }
     }
     ++n_iter;
 
-    // ---- one iteration (collision4.h:266-327) ------------------------------
+    // ---- one iteration (collision4.h:266-300) ------------------------------
     V3 Sp = sysP0 ? S : t_apply_rot_inv(tp, S);
     V3 Sn{-S.x, -S.y, -S.z};
     V3 Sq = sysQ0 ? Sn : t_apply_rot_inv(tq, Sn);
@@ -522,66 +643,41 @@ __global__ void __launch_bounds__(kSvaBlock) sva_kernel(SvaArgs A) {
     if (dp >= -kEpsBelow) {
       fin = true;
     } else {
-      int pq = (p - vbP) | ((q - vbQ) << 16);
-      L.C[k * kSvaBlock + tid] = pq;
-      bool rep = false;
-      for (int j = 0; j < k; ++j)
-        if (L.C[j * kSvaBlock + tid] == pq) {
-          rep = true;
-          break;
-        }
-      if (rep) {
-        S = w;
-      } else {
-        if (k == 1) {
-          V3 r0 = getR(L, 0, tid);
-          w = V3{r0.x + r.x, r0.y + r.y, r0.z + r.z};
-          normalize3(w);
-        }
-        if (k >= 2 && !find_half_plane(L, tid, k, w, r)) {
-          fin = true;
-          code = l + 1;
-        } else {
-          putR(L, k, tid, r);
-          float b = -2.0f * dp;
-          S.x = S.x + b * r.x;
-          S.y = S.y + b * r.y;
-          S.z = S.z + b * r.z;
-          ++k;
-        }
-      }
-      if (!fin) {
-        ++l;
-        if (l >= kMaxIter) {
-          fin = true;
-          code = -1;
-        }
+      code = sva_slow(H, k, l, (p - vbP) | ((q - vbQ) << 16), r, dp, S, w);
+      if (code != 0) {
+        fin = true;
+      } else if (++l >= kMaxIter) {
+        fin = true;
+        code = -1;
       }
     }
 
     if (fin) {
       if (code != 0) {
         ++n_col;
-        if (EDGE) __hip_atomic_store(&A.edge_res[edge], code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        else atomicMin(&A.word[item], ((pair + 1) << 4) | (code + 1));
+        if (EDGE) {
+          __hip_atomic_store(&A.edge_res[edge], code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          gate = 0;
+        } else {
+          atomicMin(&A.word[item], ((crs + 1) << 4) | (code + 1));
+          gate = min(gate, crs + 1);
+        }
+      } else {
+        ++n_sep;
+      }
+      if (++pair == pend) {
         item = -1;
         ++n_units;
       } else {
-        ++n_sep;
-        if (++pair == pend) {
-          item = -1;
-          ++n_units;
-        } else {
-          need_init = true;
-        }
+        need_init = true;
       }
     }
   }
 
   if (A.tally != nullptr) {
-    // per-block sums in the (no longer used) per-lane scratch, one atomic per counter
+    // per-block sums through LDS (the scene is no longer read), one atomic per counter
     __syncthreads();
-    unsigned* t = reinterpret_cast<unsigned*>(L.C);
+    unsigned* t = reinterpret_cast<unsigned*>(smem);
     if (tid < kTallyCount) t[tid] = 0u;
     __syncthreads();
     atomicAdd(&t[0], n_sep);
@@ -589,6 +685,7 @@ __global__ void __launch_bounds__(kSvaBlock) sva_kernel(SvaArgs A) {
     atomicAdd(&t[2], n_dropped);
     atomicAdd(&t[3], n_units);
     atomicAdd(&t[4], n_iter);
+    atomicAdd(&t[5], n_bound);
     __syncthreads();
     if (tid < kTallyCount) atomicAdd(&A.tally[tid], (unsigned long long)t[tid]);
   }
@@ -600,19 +697,26 @@ __global__ void __launch_bounds__(kSvaBlock) sva_kernel(SvaArgs A) {
 // launchers
 // ----------------------------------------------------------------------------
 
-size_t sva_lds_bytes(const SceneImg& img) {
-  return (size_t)img.bytes + sizeof(float) * 3 * kMaxIter * kSvaBlock + sizeof(int) * kMaxIter * kSvaBlock;
-}
+size_t sva_lds_bytes(const SceneImg& img) { return (size_t)img.bytes; }
 
 int sva_max_lds_bytes() { return 160 * 1024; }
 
-int sva_grid(size_t lds_bytes, int device) {
+int sva_block_for(const SceneImg& img) {
+  // four 256-thread blocks per CU when four scene copies fit in LDS, else one
+  // 1024-thread block sharing a single copy
+  return 4 * sva_lds_bytes(img) <= (size_t)sva_max_lds_bytes() ? 256 : 1024;
+}
+
+int sva_grid(const SceneImg& img, int device) {
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 1024;
-  int per_cu = (int)(prop.maxSharedMemoryPerMultiProcessor / (lds_bytes ? lds_bytes : 1));
+  int cus = 256;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) cus = prop.multiProcessorCount;
+  const int block = sva_block_for(img);
+  int per_cu = (int)(sva_max_lds_bytes() / (sva_lds_bytes(img) ? sva_lds_bytes(img) : 1));
+  int by_waves = 32 / (block / 64);  // at most 8 waves per SIMD
+  if (per_cu > by_waves) per_cu = by_waves;
   if (per_cu < 1) per_cu = 1;
-  if (per_cu > 8) per_cu = 8;
-  return prop.multiProcessorCount * per_cu;
+  return cus * per_cu;
 }
 
 #define GPRM_FK_DISPATCH(KERNEL, GRID, BLOCK, ST, ...)                                      \
@@ -651,20 +755,24 @@ hipError_t launch_edge_fk(const DevRobot& rb, EdgeArgs ea, const int* item_edge,
   return hipGetLastError();
 }
 
-hipError_t launch_sva(const SvaArgs& a, int edge_mode, int grid, hipStream_t st) {
-  size_t lds = sva_lds_bytes(a.img);
-  if (lds > (size_t)sva_max_lds_bytes()) return hipErrorInvalidValue;
-  const void* fn = edge_mode ? reinterpret_cast<const void*>(&sva_kernel<1>)
-                             : reinterpret_cast<const void*>(&sva_kernel<0>);
+template <int EDGE, int BLOCK>
+static hipError_t launch_sva_t(const SvaArgs& a, int grid, size_t lds, hipStream_t st) {
   if (lds > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&sva_kernel<EDGE, BLOCK>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  if (edge_mode)
-    hipLaunchKernelGGL(sva_kernel<1>, dim3(grid), dim3(kSvaBlock), lds, st, a);
-  else
-    hipLaunchKernelGGL(sva_kernel<0>, dim3(grid), dim3(kSvaBlock), lds, st, a);
+  hipLaunchKernelGGL((sva_kernel<EDGE, BLOCK>), dim3(grid), dim3(BLOCK), lds, st, a);
   return hipGetLastError();
+}
+
+hipError_t launch_sva(const SvaArgs& a, int edge_mode, int grid, hipStream_t st) {
+  size_t lds = sva_lds_bytes(a.img);
+  if (lds > (size_t)sva_max_lds_bytes() || a.hist == nullptr) return hipErrorInvalidValue;
+  const int block = sva_block_for(a.img);
+  if (block == 256)
+    return edge_mode ? launch_sva_t<1, 256>(a, grid, lds, st) : launch_sva_t<0, 256>(a, grid, lds, st);
+  return edge_mode ? launch_sva_t<1, 1024>(a, grid, lds, st) : launch_sva_t<0, 1024>(a, grid, lds, st);
 }
 
 hipError_t launch_edge_prologue(const DevRobot& rb, float dq, EdgeArgs ea, int* testnum_out,
@@ -682,6 +790,15 @@ hipError_t launch_edge_compact(EdgeArgs ea, int e0, int e1, int lo, int hi, int*
   if (e1 <= e0) return hipSuccess;
   dim3 grid((e1 - e0 + 255) / 256), block(256);
   hipLaunchKernelGGL(edge_compact_kernel, grid, block, 0, st, ea, e0, e1, lo, hi, item_edge, item_pos, nitems);
+  return hipGetLastError();
+}
+
+hipError_t launch_near(int ndof, const float* queries, int nq, long long qoff, const float* nodes, int nn,
+                       long long noff, float D2, int* out_q, int* out_node, float* out_d2, int* count, int cap,
+                       hipStream_t st) {
+  if (nq <= 0 || nn <= 0) return hipSuccess;
+  hipLaunchKernelGGL(near_kernel, dim3(nq), dim3(256), 0, st, ndof, queries, nq, qoff, nodes, nn, noff, D2, out_q,
+                     out_node, out_d2, count, cap);
   return hipGetLastError();
 }
 

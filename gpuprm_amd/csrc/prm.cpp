@@ -1,0 +1,348 @@
+// prm.cpp -- single-process bidirectional PRM on one GPU.
+//
+// The planner of the reference (PRMSolver<ndof>, prm/src/lib/prm/prmsolver.
+// {h,cpp}) restated for one process and one device: its MPI rank split
+// (prmsolver.cpp:1765-1895) becomes one batched step per iteration.  Per
+// step (processing_step_part1a..3a, prmsolver.cpp:1973-2187):
+//   1. sample num/2 new nodes around random nodes of each tree, accepting a
+//      base node m with probability 1/(1+surrnum[m])^3 and drawing uniformly
+//      in the cube of half-width D around it (get_random_nodes, :2584-2624);
+//      candidates are decided by the exact point check in one batch instead
+//      of one host call each;
+//   2. list every stored node of either tree closer than D (L2) to each new
+//      node (find_neighbours / get_near_vertices, :2532-2574, :2701-2748) with
+//      a GPU radius search over all nodes instead of the q[0] block index;
+//   3. check all candidate edges with the exact edge check (indicator2);
+//   4. insert each new node into every tree it has a free edge to, with all
+//      its free edges (calc_conn, part3a); a node joining both trees connects
+//      them, and Dijkstra on each tree (do_dijkstra, :2804-2846) gives the path.
+// An optional collision-probability model (surrogate.hip) biases sampling
+// and orders edge checks; the exact checks above still decide every node and
+// edge that enters a tree.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <queue>
+#include <vector>
+
+#include "../../include/gprm.h"
+#include "runtime.h"
+#include "scene.h"
+#include "surrogate.h"
+
+namespace gprm {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double ms_since(Clock::time_point t) {
+  return std::chrono::duration<double, std::milli>(Clock::now() - t).count();
+}
+
+struct DevArr {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevArr() { if (p) (void)hipFree(p); }
+  bool ensure(size_t b) {
+    if (b <= bytes) return true;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (hipMalloc(&p, b) != hipSuccess) return false;
+    bytes = b;
+    return true;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// One tree of the bidirectional search (the reference's `graph`,
+// prmsolver.h:40-58, without the block index).
+struct Tree {
+  int ndof = 0, cap = 0, n = 0;
+  std::vector<float> q;                 // AoS ndof*n
+  std::vector<int> surrnum;             // neighbours per node
+  std::vector<std::vector<std::pair<int, float>>> adj;
+  DevArr dq;                            // SoA ndof x cap on the device
+};
+
+
+int insert(Tree& t, const float* q, hipStream_t st) {
+  if (t.n >= t.cap) return -1;
+  int pos = t.n++;
+  t.q.insert(t.q.end(), q, q + t.ndof);
+  t.surrnum.push_back(0);
+  t.adj.emplace_back();
+  for (int i = 0; i < t.ndof; ++i)
+    if (hipMemcpyAsync(t.dq.as<float>() + (size_t)t.cap * i + pos, q + i, 4, hipMemcpyHostToDevice, st) != hipSuccess)
+      return -1;
+  return pos;
+}
+
+// Dijkstra on one tree (do_dijkstra, prmsolver.cpp:2804-2846).
+bool dijkstra(const Tree& t, int from, int to, std::vector<int>* path) {
+  std::vector<float> dist(t.n, INFINITY);
+  std::vector<int> parent(t.n, -1);
+  typedef std::pair<float, int> QE;
+  std::priority_queue<QE, std::vector<QE>, std::greater<QE>> pq;
+  dist[from] = 0.0f;
+  parent[from] = from;
+  pq.push({0.0f, from});
+  while (!pq.empty()) {
+    QE e = pq.top();
+    pq.pop();
+    if (e.first > dist[e.second]) continue;
+    if (e.second == to) {
+      path->assign(1, to);
+      for (int p = to; p != from;) {
+        p = parent[p];
+        path->push_back(p);
+      }
+      return true;
+    }
+    for (auto& nb : t.adj[e.second]) {
+      float nd = e.first + nb.second;
+      if (nd < dist[nb.first]) {
+        dist[nb.first] = nd;
+        parent[nb.first] = e.second;
+        pq.push({nd, nb.first});
+      }
+    }
+  }
+  return false;
+}
+
+}  // namespace
+}  // namespace gprm
+
+using namespace gprm;
+
+extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* qgoal, const gprm_prm_params* prm,
+                              gprm_prm_result* res, float* path, int path_cap) {
+  if (!ctx || !qstart || !qgoal || !prm || !res) return GPRM_E_INVALID;
+  std::memset(res, 0, sizeof(*res));
+  const int nd = gprm_dim(ctx);
+  if (prm->num < 2 || !(prm->D > 0.0f) || prm->max_nodes < 2 || prm->max_steps < 1 || prm->nbuf < 1)
+    return GPRM_E_INVALID;
+  const auto t0 = Clock::now();
+  hipStream_t st = nullptr;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return GPRM_E_HIP;
+  struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); } } sg{st};
+  Surrogate* model = static_cast<Surrogate*>(prm->surrogate);
+
+  Tree T[2];
+  for (int g = 0; g < 2; ++g) {
+    T[g].ndof = nd;
+    T[g].cap = prm->max_nodes;
+    if (!T[g].dq.ensure(sizeof(float) * nd * (size_t)T[g].cap)) return GPRM_E_HIP;
+  }
+  // start and goal must be free (prm/src/main.cpp:135-138)
+  int c0 = 0, c1 = 0;
+  int rc = gprm_indicator1(ctx, qstart, &c0);
+  if (rc == GPRM_OK) rc = gprm_indicator1(ctx, qgoal, &c1);
+  if (rc != GPRM_OK) return rc;
+  res->point_checks += 2;
+  if (c0 != 0 || c1 != 0) return GPRM_OK;  // found = 0
+  insert(T[0], qstart, st);
+  insert(T[1], qgoal, st);
+
+  Rng rng(prm->seed * 0x9E3779B97F4A7C15ULL + 12345);
+  const int num = prm->num;
+  const float D = prm->D, D2 = prm->D * prm->D;
+  const int batch = 4 * num;  // candidates drawn per exact point-check batch
+  std::vector<float> candT((size_t)nd * batch);
+  std::vector<int> cand_tree(batch), ccode(batch);
+  std::vector<float> qnew((size_t)nd * num);
+  std::vector<int> qtree(num);
+  std::vector<float> prob(batch);
+  DevArr d_cand, d_code, d_new, d_nq, d_nn, d_nd2, d_cnt, d_qs, d_qe, d_er, d_prob;
+  if (!d_cand.ensure(sizeof(float) * nd * batch) || !d_code.ensure(sizeof(int) * batch) ||
+      !d_new.ensure(sizeof(float) * nd * num) || !d_nq.ensure(sizeof(int) * prm->nbuf) ||
+      !d_nn.ensure(sizeof(int) * prm->nbuf) || !d_nd2.ensure(sizeof(float) * prm->nbuf) || !d_cnt.ensure(8) ||
+      !d_qs.ensure(sizeof(float) * nd * prm->nbuf) || !d_qe.ensure(sizeof(float) * nd * prm->nbuf) ||
+      !d_er.ensure(sizeof(int) * prm->nbuf) || !d_prob.ensure(sizeof(float) * batch))
+    return GPRM_E_HIP;
+  std::vector<int> hq, hn, her;
+  std::vector<float> hd2, hqs, hqe;
+
+  int conn_left = -1, conn_right = -1;
+  for (int step = 0; step < prm->max_steps && conn_left < 0; ++step) {
+    res->steps = step + 1;
+    // ---- 1. sampling ---------------------------------------------------------
+    auto ts = Clock::now();
+    int have[2] = {0, 0};
+    const int want[2] = {num / 2, num - num / 2};
+    int filled = 0;
+    int guard = 0;
+    while (filled < num) {
+      if (++guard > 10000) break;
+      for (int k = 0; k < batch; ++k) {
+        const int g = k & 1;  // alternate trees; surplus of one tree is discarded
+        const Tree& t = T[g];
+        int m;
+        for (;;) {  // base node with acceptance 1/(1+surrnum)^3
+          m = (int)(rng.uniform() * t.n);
+          if (m >= t.n) m = t.n - 1;
+          double x = 1.0 + t.surrnum[m];
+          if (rng.uniform() * x * x * x < 1.0) break;
+        }
+        cand_tree[k] = g;
+        for (int i = 0; i < nd; ++i)
+          candT[(size_t)batch * i + k] = t.q[(size_t)nd * m + i] - D + 2 * D * (float)rng.uniform();
+      }
+      // optional bias: half of the batch keeps only candidates the model
+      // predicts free (the exact check below still decides every one)
+      std::vector<char> skip(batch, 0);
+      if (model && prm->bias_sampling) {
+        if (hipMemcpyAsync(d_cand.p, candT.data(), sizeof(float) * nd * batch, hipMemcpyHostToDevice, st) != hipSuccess ||
+            surrogate_predict(model, d_cand.as<float>(), batch, batch, d_prob.as<float>(), st) != 0 ||
+            hipMemcpyAsync(prob.data(), d_prob.p, sizeof(float) * batch, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+          return GPRM_E_HIP;
+        for (int k = 0; k < batch; ++k)
+          if ((k & 2) && prob[k] > 0.5f) skip[k] = 1;
+        res->model_predictions += batch;
+      }
+      if (hipMemcpyAsync(d_cand.p, candT.data(), sizeof(float) * nd * batch, hipMemcpyHostToDevice, st) != hipSuccess)
+        return GPRM_E_HIP;
+      rc = gprm_indicator_dev(ctx, d_cand.as<float>(), d_code.as<int>(), batch, batch, st);
+      if (rc != GPRM_OK) return rc;
+      if (hipMemcpy(ccode.data(), d_code.p, sizeof(int) * batch, hipMemcpyDeviceToHost) != hipSuccess)
+        return GPRM_E_HIP;
+      res->point_checks += batch;
+      for (int k = 0; k < batch && filled < num; ++k) {
+        const int g = cand_tree[k];
+        if (skip[k] || ccode[k] != 0 || have[g] >= want[g]) continue;
+        for (int i = 0; i < nd; ++i) qnew[(size_t)nd * filled + i] = candT[(size_t)batch * i + k];
+        qtree[filled] = g;
+        ++have[g];
+        ++filled;
+      }
+    }
+    res->ms_sampling += ms_since(ts);
+    if (filled < num) break;
+
+    // ---- 2. radius neighbours in both trees ---------------------------------
+    auto tn = Clock::now();
+    std::vector<float> qnewT((size_t)nd * num);
+    for (int j = 0; j < num; ++j)
+      for (int i = 0; i < nd; ++i) qnewT[(size_t)num * i + j] = qnew[(size_t)nd * j + i];
+    if (hipMemcpyAsync(d_new.p, qnewT.data(), sizeof(float) * nd * num, hipMemcpyHostToDevice, st) != hipSuccess)
+      return GPRM_E_HIP;
+    // candidate edges, left tree then right tree per new node: (node j, tree g, stored index)
+    std::vector<int> eq, eg, en;
+    std::vector<float> ed;
+    for (int g = 0; g < 2; ++g) {
+      int zero[2] = {0, 0};
+      if (hipMemcpyAsync(d_cnt.p, zero, 8, hipMemcpyHostToDevice, st) != hipSuccess) return GPRM_E_HIP;
+      if (launch_near(nd, d_new.as<float>(), num, num, T[g].dq.as<float>(), T[g].n, T[g].cap, D2, d_nq.as<int>(),
+                      d_nn.as<int>(), d_nd2.as<float>(), d_cnt.as<int>(), prm->nbuf, st) != hipSuccess)
+        return GPRM_E_HIP;
+      int cnt = 0;
+      if (hipMemcpyAsync(&cnt, d_cnt.p, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        return GPRM_E_HIP;
+      cnt = std::min(cnt, prm->nbuf);
+      hq.resize(cnt); hn.resize(cnt); hd2.resize(cnt);
+      if (cnt > 0 && (hipMemcpy(hq.data(), d_nq.p, 4 * (size_t)cnt, hipMemcpyDeviceToHost) != hipSuccess ||
+                      hipMemcpy(hn.data(), d_nn.p, 4 * (size_t)cnt, hipMemcpyDeviceToHost) != hipSuccess ||
+                      hipMemcpy(hd2.data(), d_nd2.p, 4 * (size_t)cnt, hipMemcpyDeviceToHost) != hipSuccess))
+        return GPRM_E_HIP;
+      // deterministic order: by new node, then stored index
+      std::vector<int> order(cnt);
+      for (int i = 0; i < cnt; ++i) order[i] = i;
+      std::sort(order.begin(), order.end(), [&](int a, int b) { return hq[a] != hq[b] ? hq[a] < hq[b] : hn[a] < hn[b]; });
+      for (int i : order) {
+        if ((long long)eq.size() >= prm->nbuf) break;
+        eq.push_back(hq[i]);
+        eg.push_back(g);
+        en.push_back(hn[i]);
+        ed.push_back(std::sqrt(hd2[i]));
+      }
+    }
+    res->ms_near += ms_since(tn);
+
+    // ---- 3. exact edge checks ------------------------------------------------
+    auto te = Clock::now();
+    const int E = (int)eq.size();
+    her.assign(E, 1);
+    if (E > 0) {
+      hqs.resize((size_t)nd * E);
+      hqe.resize((size_t)nd * E);
+      for (int e = 0; e < E; ++e)
+        for (int i = 0; i < nd; ++i) {
+          hqs[(size_t)E * i + e] = qnew[(size_t)nd * eq[e] + i];
+          hqe[(size_t)E * i + e] = T[eg[e]].q[(size_t)nd * en[e] + i];
+        }
+      if (hipMemcpyAsync(d_qs.p, hqs.data(), sizeof(float) * nd * E, hipMemcpyHostToDevice, st) != hipSuccess ||
+          hipMemcpyAsync(d_qe.p, hqe.data(), sizeof(float) * nd * E, hipMemcpyHostToDevice, st) != hipSuccess)
+        return GPRM_E_HIP;
+      gprm_edge_stats es;
+      rc = gprm_indicator2_dev(ctx, d_qs.as<float>(), d_qe.as<float>(), d_er.as<int>(), E, E, st, &es);
+      if (rc != GPRM_OK) return rc;
+      if (hipMemcpy(her.data(), d_er.p, sizeof(int) * E, hipMemcpyDeviceToHost) != hipSuccess) return GPRM_E_HIP;
+      res->edges_checked += E;
+      res->edge_samples += es.samples_total;
+    }
+    res->ms_edges += ms_since(te);
+
+    // ---- 4. insert nodes with free edges; connection ------------------------
+    auto th = Clock::now();
+    size_t e = 0;
+    for (int j = 0; j < num && conn_left < 0; ++j) {
+      int pos[2] = {-1, -1};
+      size_t e0 = e;
+      while (e < (size_t)E && eq[e] < j) ++e;
+      e0 = e;
+      size_t e1 = e0;
+      while (e1 < (size_t)E && eq[e1] == j) ++e1;
+      for (int g = 0; g < 2; ++g) {
+        bool any = false;
+        for (size_t k = e0; k < e1; ++k)
+          if (eg[k] == g && her[k] == 0) { any = true; break; }
+        if (!any) continue;
+        pos[g] = insert(T[g], &qnew[(size_t)nd * j], st);
+        if (pos[g] < 0) break;
+        for (size_t k = e0; k < e1; ++k) {
+          if (eg[k] != g || her[k] != 0) continue;
+          const int other = en[k];
+          T[g].adj[pos[g]].push_back({other, ed[k]});
+          T[g].adj[other].push_back({pos[g], ed[k]});
+          ++T[g].surrnum[other];
+          ++T[g].surrnum[pos[g]];
+          ++res->edges_free;
+        }
+      }
+      e = e1;
+      if (pos[0] >= 0 && pos[1] >= 0) {
+        conn_left = pos[0];
+        conn_right = pos[1];
+      }
+      if ((pos[0] < 0 && T[0].n >= T[0].cap) || (pos[1] < 0 && T[1].n >= T[1].cap)) break;
+    }
+    res->ms_host += ms_since(th);
+    if (T[0].n >= T[0].cap || T[1].n >= T[1].cap) break;
+  }
+  res->nodes_left = T[0].n;
+  res->nodes_right = T[1].n;
+  if (conn_left >= 0) {
+    std::vector<int> pl, pr;
+    bool ok = dijkstra(T[0], 0, conn_left, &pl) && dijkstra(T[1], conn_right, 0, &pr);
+    if (ok) {
+      // start ... connection (left path is goal-first), then connection ... goal
+      // pl = [conn_left, ..., start], pr = [goal, ..., conn_right]; conn_left
+      // and conn_right hold the same configuration
+      std::vector<const float*> seq;
+      for (int i = (int)pl.size() - 1; i >= 0; --i) seq.push_back(&T[0].q[(size_t)nd * pl[i]]);
+      for (int i = (int)pr.size() - 2; i >= 0; --i) seq.push_back(&T[1].q[(size_t)nd * pr[i]]);
+      res->found = 1;
+      res->path_len = (int)seq.size();
+      if (path)
+        for (int i = 0; i < std::min(path_cap, res->path_len); ++i) std::memcpy(path + (size_t)nd * i, seq[i], 4 * nd);
+    }
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) return GPRM_E_HIP;
+  res->ms_total = ms_since(t0);
+  return GPRM_OK;
+}

@@ -7,7 +7,6 @@
 
 namespace gprm {
 
-constexpr int kSvaBlock = 256;      // threads per block of the separating-vector kernel
 constexpr int kFkBlock = 256;       // threads per block of the kinematics kernels
 constexpr int kMaxIter = 10;        // collision4.h:17-18 (max_for_loop, max_for_loop_whole)
 constexpr int kChunk = 64;          // items claimed per wave from the work counter
@@ -30,7 +29,7 @@ struct DevRobot {
 struct SceneImg {
   const int4* blob;        // device pointer to the image
   int bytes;               // image size, multiple of 16
-  int off_verts, off_pairs, off_vinfo, off_adj;
+  int off_verts, off_pairs, off_adj, off_spheres;
   int sumn, summ, M, N;
 };
 
@@ -39,6 +38,7 @@ struct SvaArgs {
   SceneImg img;
   int ndof;
   const float* frames;     // [item][ndof][12] link frames (columns 0..3, xyz)
+  float4* hist;            // [grid*block][kMaxIter] per-lane iteration history
   int nitems;              // item count, or -1: read *nitems_dev
   const int* nitems_dev;
   unsigned int* counter;   // work counter (zero before launch)
@@ -51,10 +51,11 @@ struct SvaArgs {
   int* edge_res;           // [edge] nonzero once any sample collides
   // optional counters, added once per block at exit (may be null):
   // [0] pair tests ending separated, [1] pair tests ending in a collision,
-  // [2] units dropped by early exit, [3] units completed, [4] iterations
+  // [2] pair tests skipped by early exit, [3] units completed, [4] iterations
+  // run, [5] pair tests decided separated by the bounding-sphere test
   unsigned long long* tally;
 };
-constexpr int kTallyCount = 5;
+constexpr int kTallyCount = 6;
 
 // Edge work description.
 struct EdgeArgs {
@@ -67,6 +68,7 @@ struct EdgeArgs {
 
 size_t sva_lds_bytes(const SceneImg& img);
 int sva_max_lds_bytes();
+int sva_block_for(const SceneImg& img);
 
 hipError_t launch_fk_points(const DevRobot& rb, const float* q, long long offset, int n, float* frames,
                             int* word, int check_bounds, hipStream_t st);
@@ -80,10 +82,16 @@ hipError_t launch_edge_compact(EdgeArgs ea, int e0, int e1, int lo, int hi, int*
 hipError_t launch_edge_fk(const DevRobot& rb, EdgeArgs ea, const int* item_edge, const int* item_pos,
                           const int* nitems, int cap, float* frames, hipStream_t st);
 hipError_t launch_fill_i32(int* p, int v, long long n, hipStream_t st);
+// Radius neighbours: every (query j, node l) with |node_l - query_j|^2 < D2,
+// appended to (out_q, out_node, out_d2) up to cap entries; *count receives the
+// total number of matches (may exceed cap).
+hipError_t launch_near(int ndof, const float* queries, int nq, long long qoff, const float* nodes, int nn,
+                       long long noff, float D2, int* out_q, int* out_node, float* out_d2, int* count, int cap,
+                       hipStream_t st);
 hipError_t launch_fill_u32(unsigned int* p, unsigned int v, long long n, hipStream_t st);
 
-// Number of resident blocks per CU for the separating-vector kernel with the
-// given LDS footprint, and the device's CU count.
-int sva_grid(size_t lds_bytes, int device);
+// Grid of the persistent separating-vector kernel: every CU filled as far as
+// LDS and the wave limit allow.
+int sva_grid(const SceneImg& img, int device);
 
 }  // namespace gprm

@@ -134,13 +134,60 @@ typedef struct {
   double fk_ms;
   long long pairs_separated;   /* separating-vector tests that ended separated (code 0) */
   long long pairs_colliding;   /* ... that ended with a nonzero code */
-  long long units_dropped;     /* work units (a configuration's run of up to 8 pairs)
-                                  skipped by early exit */
+  long long units_dropped;     /* pair tests skipped by early exit */
   long long units_done;        /* work units run to their end */
   long long iterations;        /* separating-vector iterations run */
+  long long bound_separated;   /* pair tests whose first iteration is decided
+                                  separated by a bounding-sphere margin test */
 } gprm_profile_stats;
 int gprm_profile_enable(gprm_ctx* ctx, int enable);
 int gprm_profile_read(gprm_ctx* ctx, gprm_profile_stats* out);
+
+/* Single-process bidirectional PRM on the device (the reference's
+ * PRMSolver<ndof>, prm/src/lib/prm/prmsolver.{h,cpp}, process 5 restated for
+ * one GPU: prmsolver.cpp:1973-2187).  Grows a tree from qstart and one from
+ * qgoal; every node and edge entering a tree is decided by the exact checks
+ * above.  On success res->found = 1 and path (AoS, ndof floats per node,
+ * up to path_cap nodes) runs from qstart to qgoal. */
+typedef struct {
+  int num;                  /* new nodes per step (the reference's --num) */
+  float D;                  /* connection radius and sampling half-width (--distD) */
+  int max_nodes;            /* capacity of each tree */
+  int max_steps;            /* steps before giving up (--maxsteps) */
+  unsigned long long seed;
+  int nbuf;                 /* candidate edges per step and tree */
+  void* surrogate;          /* optional model (gprm_surrogate_*), or NULL */
+  int bias_sampling;        /* with a model: half the candidates must be predicted free */
+  int order_edges;          /* with a model: check edges of likely-free nodes first */
+} gprm_prm_params;
+typedef struct {
+  int found, steps, path_len;
+  long long nodes_left, nodes_right;
+  long long point_checks;   /* candidate configurations checked exactly */
+  long long edges_checked, edges_free;
+  long long edge_samples;   /* sum of testnum over checked edges */
+  long long model_predictions;
+  double ms_total, ms_sampling, ms_near, ms_edges, ms_host;
+} gprm_prm_result;
+int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* qgoal, const gprm_prm_params* params,
+                   gprm_prm_result* res, float* path, int path_cap);
+
+/* Learned collision-probability model (no reference counterpart): an MLP
+ * 7 -> 256 -> 256 -> 256 -> 1 (SiLU, sigmoid output) on BF16 MFMA with FP32
+ * accumulation, trained by its own forward/backward kernels and Adam on
+ * labels of the exact checker (code != 0).  It only ranks configurations for
+ * gprm_prm_solve; it never decides one.  Device arrays are SoA with the given
+ * component offset; `batch` bounds n of a training call. */
+int gprm_surrogate_create(int ndof, int batch, unsigned long long seed, void** out);
+void gprm_surrogate_destroy(void* model);
+int gprm_surrogate_param_count(void);
+int gprm_surrogate_get_params(void* model, float* out);
+int gprm_surrogate_set_params(void* model, const float* in);
+int gprm_surrogate_gradients(void* model, const float* d_q, long long offset, const int* d_codes, int n,
+                             float* grads_out, float* loss_out);
+int gprm_surrogate_train_step(void* model, const float* d_q, long long offset, const int* d_codes, int n, float lr,
+                              float* loss_out);
+int gprm_surrogate_predict(void* model, const float* d_q, int n, long long offset, float* d_prob, void* stream);
 
 /* Last error message of the context (or of creation when ctx is NULL). */
 const char* gprm_last_error(gprm_ctx* ctx);

@@ -1,0 +1,177 @@
+"""C ABI of libgprm.so: every function declared in include/gprm.h is exported,
+and the error paths return GPRM_E_* codes instead of aborting (no device
+work: these run on the CPU)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "gprm.h")
+
+GPRM_E_INVALID, GPRM_E_SCENE, GPRM_E_HIP, GPRM_E_CAPACITY, GPRM_E_REQUEST, GPRM_E_STATE = -1, -2, -3, -4, -5, -6
+
+
+def declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(gprm_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_the_interface():
+    names = declared()
+    for n in ("gprm_create_from_dir", "gprm_create", "gprm_init", "gprm_destroy", "gprm_indicator1",
+              "gprm_indicator", "gprm_indicator2", "gprm_indicator2_async", "gprm_indicator2_wait",
+              "gprm_indicator2_compressed", "gprm_check_boundaries", "gprm_check_boundaries1", "gprm_min",
+              "gprm_max", "gprm_deltaq", "gprm_dim", "gprm_numthreads_all", "gprm_indicator_dev",
+              "gprm_indicator2_dev", "gprm_last_error", "gprm_scene_generate", "gprm_profile_enable",
+              "gprm_profile_read"):
+        assert n in names, n
+
+
+def test_library_exports_every_declared_symbol():
+    import gpuprm_amd
+    out = subprocess.run(["nm", "-D", "--defined-only", gpuprm_amd.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r"\sT\s(gprm_\w+)", out))
+    missing = [n for n in declared() if n not in exported]
+    assert not missing, missing
+    L = gpuprm_amd.lib()
+    for n in declared():
+        assert hasattr(L, n)
+
+
+def test_c_header_compiles_as_c(tmp_path):
+    src = tmp_path / "use.c"
+    src.write_text('#include "gprm.h"\nint main(void){ gprm_ctx* c = 0; (void)c; return GPRM_OK; }\n')
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), "-c", str(src),
+                    "-o", str(tmp_path / "use.o")], check=True)
+
+
+def _ctx(path):
+    import gpuprm_amd
+    L = gpuprm_amd.lib()
+    c = C.c_void_p()
+    rc = L.gprm_create_from_dir(path.encode(), 0.01, 64, C.byref(c))
+    return L, rc, c
+
+
+def test_missing_scene_is_an_error(tmp_path):
+    L, rc, c = _ctx(str(tmp_path / "nope"))
+    assert rc == GPRM_E_SCENE and not c.value
+    assert b"ndof.bin" in L.gprm_last_error(None)
+
+
+def test_corrupt_scene_is_an_error(workload_dir, tmp_path):
+    import shutil
+    d = str(tmp_path / "bad")
+    shutil.copytree(workload_dir(1), d)
+    # a vertex without edges (robotconfigspace.cpp:45-49 asserts this)
+    cnt = np.fromfile(d + "/polys/poly0/cnt.bin", np.int32)
+    cnt[0] = 0
+    cnt.tofile(d + "/polys/poly0/cnt.bin")
+    L, rc, c = _ctx(d)
+    assert rc == GPRM_E_SCENE
+    # truncated vertex file (the reference's read_file reads short silently)
+    shutil.rmtree(d)
+    shutil.copytree(workload_dir(1), d)
+    open(d + "/polys/poly3/vertices.bin", "wb").write(b"\0" * 8)
+    L, rc, c = _ctx(d)
+    assert rc == GPRM_E_SCENE
+
+
+def test_unsorted_pairs_are_rejected(workload_dir, tmp_path):
+    import shutil
+    d = str(tmp_path / "unsorted")
+    shutil.copytree(workload_dir(1), d)
+    f = np.fromfile(d + "/pairs/from.bin", np.int32)
+    t = np.fromfile(d + "/pairs/to.bin", np.int32)
+    f[[0, -1]] = f[[-1, 0]]
+    t[[0, -1]] = t[[-1, 0]]
+    f.tofile(d + "/pairs/from.bin")
+    t.tofile(d + "/pairs/to.bin")
+    L, rc, c = _ctx(d)
+    assert rc == GPRM_E_SCENE
+
+
+def test_calls_before_init_and_bad_arguments(workload_dir):
+    import gpuprm_amd
+    L, rc, c = _ctx(workload_dir(1))
+    assert rc == 0 and c.value
+    q = np.zeros(4, np.float32)
+    out = C.c_int(7)
+    FP = C.POINTER(C.c_float)
+    assert L.gprm_indicator1(c, q.ctypes.data_as(FP), C.byref(out)) == GPRM_E_STATE
+    assert L.gprm_indicator2_wait(c, 12345) == GPRM_E_STATE
+    assert L.gprm_init(c, 1, 1) == GPRM_E_INVALID  # rank must be < size
+    # host-side queries work without a device
+    assert L.gprm_dim(c) == 4
+    assert abs(L.gprm_deltaq(c) - 0.01) < 1e-9
+    assert L.gprm_min(c, 0) == np.float32(-np.pi) and L.gprm_max(c, 3) == np.float32(np.pi)
+    assert np.isnan(L.gprm_min(c, 9))
+    res = np.zeros(3, np.int32)
+    qb = np.array([0, 0, 4, 0, 0, 0, 0, 0, 0, -4, 0, 0], np.float32)  # items 1 and 0 out of bounds
+    assert L.gprm_check_boundaries(c, qb.ctypes.data_as(FP), res.ctypes.data_as(C.POINTER(C.c_int)), 3, 3) == 0
+    assert res.tolist() == [1, 0, 1]
+    assert L.gprm_check_boundaries(c, qb.ctypes.data_as(FP), res.ctypes.data_as(C.POINTER(C.c_int)), -1, 3) \
+        == GPRM_E_INVALID
+    L.gprm_destroy(c)
+
+
+def test_init_without_device_reports_hip_error(workload_dir):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    L, rc, c = _ctx(workload_dir(1))
+    assert L.gprm_init(c, 0, 1) == GPRM_E_HIP
+    L.gprm_destroy(c)
+
+
+def test_scene_generate_errors(tmp_path):
+    import gpuprm_amd
+    L = gpuprm_amd.lib()
+    assert L.gprm_scene_generate(4, 1, str(tmp_path / "x").encode()) == GPRM_E_INVALID
+    assert L.gprm_scene_generate(1, 1, None) == GPRM_E_INVALID
+
+
+def test_python_mirror_raises_on_errors(tmp_path):
+    import gpuprm_amd
+    with pytest.raises(gpuprm_amd.GprmError) as e:
+        gpuprm_amd.RobotConfigspace(str(tmp_path / "missing"))
+    assert e.value.code == GPRM_E_SCENE
+
+
+def test_cpp_dropin_header(workload_dir, tmp_path):
+    """RobotConfigspaceGPU<ndof> compiles against include/gprm_configspace.hpp,
+    links libgprm.so and answers the host-side queries of Configspace<ndof>."""
+    import gpuprm_amd
+    src = tmp_path / "dropin.cpp"
+    src.write_text(r'''
+#include "gprm_configspace.hpp"
+#include <cstdio>
+int run(Configspace<4>* space) {  // a planner only sees the interface
+  float q[4] = {0, 0, 5, 0};
+  return space->dim() * 100 + space->check_boundaries(q) * 10 + (space->min(0) < 0);
+}
+int main(int argc, char** argv) {
+  RobotConfigspaceGPU<4> cs(argv[1], 0.01f, 64);
+  int r = run(&cs);
+  std::printf("%d %d\n", r, cs.init(0, 1));
+  try { RobotConfigspaceGPU<6> wrong(argv[1], 0.01f, 64); } catch (const GprmException& e) { std::printf("wrong ndof %d\n", e.code); }
+  try { RobotConfigspaceGPU<4> missing("/nonexistent", 0.01f, 64); } catch (const GprmException& e) { std::printf("missing %d\n", e.code); }
+  return 0;
+}
+''')
+    exe = tmp_path / "dropin"
+    libdir = os.path.dirname(gpuprm_amd.LIB_PATH)
+    subprocess.run(["g++", "-std=c++11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe), "-L", libdir, "-lgprm", "-Wl,-rpath," + libdir], check=True)
+    out = subprocess.run([str(exe), workload_dir(1)], capture_output=True, text=True, check=True).stdout.split("\n")
+    r, init_rc = map(int, out[0].split())
+    assert r == 411
+    assert init_rc in (0, GPRM_E_HIP)
+    assert out[1] == "wrong ndof %d" % GPRM_E_SCENE
+    assert out[2] == "missing %d" % GPRM_E_SCENE
