@@ -175,18 +175,27 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
     const int want[2] = {num / 2, num - num / 2};
     int filled = 0;
     int guard = 0;
+    // Base nodes: the reference draws a node uniformly and keeps it with
+    // probability 1/(1+surrnum)^3, i.e. it samples nodes with weight
+    // 1/(1+surrnum)^3; drawn here directly from cumulative weights.
+    std::vector<double> cum[2];
+    for (int g = 0; g < 2; ++g) {
+      cum[g].resize(T[g].n);
+      double acc = 0.0;
+      for (int m = 0; m < T[g].n; ++m) {
+        double x = 1.0 + T[g].surrnum[m];
+        acc += 1.0 / (x * x * x);
+        cum[g][m] = acc;
+      }
+    }
     while (filled < num) {
-      if (++guard > 10000) break;
+      if (++guard > 1000) break;
       for (int k = 0; k < batch; ++k) {
         const int g = k & 1;  // alternate trees; surplus of one tree is discarded
         const Tree& t = T[g];
-        int m;
-        for (;;) {  // base node with acceptance 1/(1+surrnum)^3
-          m = (int)(rng.uniform() * t.n);
-          if (m >= t.n) m = t.n - 1;
-          double x = 1.0 + t.surrnum[m];
-          if (rng.uniform() * x * x * x < 1.0) break;
-        }
+        const double u = rng.uniform() * cum[g].back();
+        int m = (int)(std::upper_bound(cum[g].begin(), cum[g].end(), u) - cum[g].begin());
+        if (m >= t.n) m = t.n - 1;
         cand_tree[k] = g;
         for (int i = 0; i < nd; ++i)
           candT[(size_t)batch * i + k] = t.q[(size_t)nd * m + i] - D + 2 * D * (float)rng.uniform();
@@ -230,9 +239,11 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
       for (int i = 0; i < nd; ++i) qnewT[(size_t)num * i + j] = qnew[(size_t)nd * j + i];
     if (hipMemcpyAsync(d_new.p, qnewT.data(), sizeof(float) * nd * num, hipMemcpyHostToDevice, st) != hipSuccess)
       return GPRM_E_HIP;
-    // candidate edges, left tree then right tree per new node: (node j, tree g, stored index)
-    std::vector<int> eq, eg, en;
-    std::vector<float> ed;
+    // candidate edges (new node j, tree g, stored node, distance) in the
+    // reference's buffer order: by new node, left tree before right tree
+    // (find_neighbours), at most nbuf per step
+    struct Cand { int j, g, node; float d2; };
+    std::vector<Cand> cands;
     for (int g = 0; g < 2; ++g) {
       int zero[2] = {0, 0};
       if (hipMemcpyAsync(d_cnt.p, zero, 8, hipMemcpyHostToDevice, st) != hipSuccess) return GPRM_E_HIP;
@@ -249,17 +260,19 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
                       hipMemcpy(hn.data(), d_nn.p, 4 * (size_t)cnt, hipMemcpyDeviceToHost) != hipSuccess ||
                       hipMemcpy(hd2.data(), d_nd2.p, 4 * (size_t)cnt, hipMemcpyDeviceToHost) != hipSuccess))
         return GPRM_E_HIP;
-      // deterministic order: by new node, then stored index
-      std::vector<int> order(cnt);
-      for (int i = 0; i < cnt; ++i) order[i] = i;
-      std::sort(order.begin(), order.end(), [&](int a, int b) { return hq[a] != hq[b] ? hq[a] < hq[b] : hn[a] < hn[b]; });
-      for (int i : order) {
-        if ((long long)eq.size() >= prm->nbuf) break;
-        eq.push_back(hq[i]);
-        eg.push_back(g);
-        en.push_back(hn[i]);
-        ed.push_back(std::sqrt(hd2[i]));
-      }
+      for (int i = 0; i < cnt; ++i) cands.push_back({hq[i], g, hn[i], hd2[i]});
+    }
+    std::sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) {
+      return a.j != b.j ? a.j < b.j : (a.g != b.g ? a.g < b.g : a.node < b.node);
+    });
+    if ((long long)cands.size() > prm->nbuf) cands.resize(prm->nbuf);
+    std::vector<int> eq(cands.size()), eg(cands.size()), en(cands.size());
+    std::vector<float> ed(cands.size());
+    for (size_t i = 0; i < cands.size(); ++i) {
+      eq[i] = cands[i].j;
+      eg[i] = cands[i].g;
+      en[i] = cands[i].node;
+      ed[i] = std::sqrt(cands[i].d2);
     }
     res->ms_near += ms_since(tn);
 
@@ -292,9 +305,8 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
     size_t e = 0;
     for (int j = 0; j < num && conn_left < 0; ++j) {
       int pos[2] = {-1, -1};
-      size_t e0 = e;
       while (e < (size_t)E && eq[e] < j) ++e;
-      e0 = e;
+      const size_t e0 = e;
       size_t e1 = e0;
       while (e1 < (size_t)E && eq[e1] == j) ++e1;
       for (int g = 0; g < 2; ++g) {
