@@ -33,6 +33,10 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
 SEED = 2024
 NDOF = 6
 M_PAIRS = 0
+# HBM bytes per sva_kernel<1> launch of this workload (FETCH_SIZE + WRITE_SIZE,
+# each scaled by the same counter's reading for a known-size copy kernel),
+# from profiles/sva_kernel_edges_wl2_traffic.md; None until measured
+TRAFFIC_BYTES_PER_LAUNCH = None
 
 
 def log(msg):
@@ -148,6 +152,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--verify-edges", type=int, default=3000)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="before the timed steps, run a plain elementwise kernel that reads and writes "
+                         "MIB MiB once each (the known-size kernel that scales HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus %d)"
@@ -174,6 +181,16 @@ def main():
     qs = torch.from_numpy(QS).cuda()
     qe = torch.from_numpy(QE).cuda()
     res = torch.zeros(E, dtype=torch.int32, device="cuda")
+    if args.copy_calibration > 0:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        log("copy calibration: neg kernel over %d MiB read + %d MiB written"
+            % (args.copy_calibration, args.copy_calibration))
+        del src, dst
     log("%d edges; warmup %d" % (E, args.warmup))
     for _ in range(args.warmup):
         st = space.indicator2_dev(qs, qe, res)
@@ -212,12 +229,15 @@ def main():
     sva_s = prof.sva_ms / 1000.0
     achieved = flops / sva_s / 1e12 if sva_s > 0 else 0.0
     roofline = {
-        "bound": "valu",
+        # compute-bound on the FP32 pipe; the test is scalar geometry, so the
+        # FP32 vector ALU (same 157.3 TFLOP/s published peak) does the work
+        "bound": "mfma",
+        "pipe": "fp32 valu",
         "achieved": round(achieved, 3),
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-        "traffic": None,
+        "traffic": TRAFFIC_BYTES_PER_LAUNCH,
         "kernel": "sva_kernel<1> (separating-vector, edge mode)",
         "launches_per_step": prof.sva_launches / args.steps,
         "avg_launch_ms": prof.sva_ms / max(prof.sva_launches, 1),
