@@ -77,6 +77,7 @@ class PrmParams(C.Structure):
         ("surrogate", C.c_void_p),
         ("bias_sampling", C.c_int),
         ("order_edges", C.c_int),
+        ("edge_budget", C.c_int),
     ]
 
 
@@ -92,11 +93,14 @@ class PrmResult(C.Structure):
         ("edges_free", C.c_longlong),
         ("edge_samples", C.c_longlong),
         ("model_predictions", C.c_longlong),
+        ("candidates_skipped", C.c_longlong),
+        ("edges_skipped", C.c_longlong),
         ("ms_total", C.c_double),
         ("ms_sampling", C.c_double),
         ("ms_near", C.c_double),
         ("ms_edges", C.c_double),
         ("ms_host", C.c_double),
+        ("ms_model", C.c_double),
     ]
 
     def as_dict(self):
@@ -416,13 +420,17 @@ class RobotConfigspace:
 
     # -- single-process PRM ----------------------------------------------
     def prm_solve(self, qstart, qgoal, num=2048, D=0.5, max_nodes=1 << 18, max_steps=200, seed=1,
-                  nbuf=1 << 21, surrogate=None, bias_sampling=False, order_edges=False, path_cap=1 << 16):
+                  nbuf=1 << 21, surrogate=None, bias_sampling=False, order_edges=False, edge_budget=0,
+                  path_cap=1 << 16):
         """Bidirectional PRM from qstart to qgoal (the reference's PRMSolver
         process, prm/src/lib/prm/prmsolver.cpp:1973-2187, on one GPU).
-        Returns (PrmResult, path (n, ndof) or None)."""
+        edge_budget=0 checks every candidate edge, as the reference does; k > 0
+        checks at most k per (new node, tree), ranked by the surrogate when
+        order_edges is set, else shortest first.  Returns (PrmResult, path
+        (n, ndof) or None)."""
         p = PrmParams(int(num), float(D), int(max_nodes), int(max_steps), int(seed), int(nbuf),
                       C.c_void_p(surrogate.handle if surrogate is not None else None),
-                      int(bool(bias_sampling)), int(bool(order_edges)))
+                      int(bool(bias_sampling)), int(bool(order_edges)), int(edge_budget))
         r = PrmResult()
         path = np.zeros((path_cap, self.ndof), np.float32)
         _check(lib().gprm_prm_solve(self._ctx, _f32(qstart).ctypes.data_as(_FP), _f32(qgoal).ctypes.data_as(_FP),

@@ -317,6 +317,7 @@ int run_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n,
     a.img = c->img;
     a.ndof = nd;
     a.frames = sl.frames.as<float>();
+    a.fstride = cn;
     a.hist = sl.hist.as<float4>();
     a.nitems = cn;
     a.counter = sl.counters.as<unsigned>() + ci;
@@ -406,6 +407,7 @@ int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long 
     a.img = c->img;
     a.ndof = nd;
     a.frames = sl.frames.as<float>();
+    a.fstride = pcap;
     a.hist = sl.hist.as<float4>();
     a.nitems = -1;
     a.nitems_dev = cnt;

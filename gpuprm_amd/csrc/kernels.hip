@@ -111,9 +111,12 @@ __device__ __forceinline__ T12 t_lapply(const T12& self, const T12& T) {
   return r;
 }
 
-// Kinematics<ndof>::calculate (kinematics.h:27-37); frames 1..ndof to out.
+// Kinematics<ndof>::calculate (kinematics.h:27-37); frames 1..ndof of item
+// to the component-major frame array: entry c of link frame i at
+// out[(12 i + c) stride + item], so consecutive items are consecutive words.
 template <int NDOF>
-__device__ __forceinline__ void fk_store(const DevRobot& rb, const float* q, float* __restrict__ out) {
+__device__ __forceinline__ void fk_store(const DevRobot& rb, const float* q, float* __restrict__ out, int item,
+                                         long long stride) {
   const int nd = NDOF > 0 ? NDOF : rb.ndof;
   T12 T = t_set(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
@@ -124,20 +127,18 @@ __device__ __forceinline__ void fk_store(const DevRobot& rb, const float* q, flo
     else dset = q[i];
     T12 A = t_set(rb.a[i], rb.alpha[i], qset, dset);
     T = t_lapply(A, T);
-    float4* o = reinterpret_cast<float4*>(out + 12 * i);
-    o[0] = make_float4(T.m[0], T.m[1], T.m[2], T.m[3]);
-    o[1] = make_float4(T.m[4], T.m[5], T.m[6], T.m[7]);
-    o[2] = make_float4(T.m[8], T.m[9], T.m[10], T.m[11]);
+    float* o = out + (size_t)(12 * i) * stride + item;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) o[(size_t)c * stride] = T.m[c];
   }
 }
 
-__device__ __forceinline__ T12 load_frame(const float* __restrict__ frames, int item, int ndof, int sys) {
-  const float4* f = reinterpret_cast<const float4*>(frames + ((size_t)item * ndof + (sys - 1)) * 12);
-  float4 a = f[0], b = f[1], c = f[2];
+// Frame of link sys (1..ndof) of item from the component-major frame array.
+__device__ __forceinline__ T12 load_frame(const float* __restrict__ frames, int item, long long stride, int sys) {
+  const float* f = frames + (size_t)(12 * (sys - 1)) * stride + item;
   T12 t;
-  t.m[0] = a.x; t.m[1] = a.y; t.m[2] = a.z;  t.m[3] = a.w;
-  t.m[4] = b.x; t.m[5] = b.y; t.m[6] = b.z;  t.m[7] = b.w;
-  t.m[8] = c.x; t.m[9] = c.y; t.m[10] = c.z; t.m[11] = c.w;
+#pragma unroll
+  for (int c = 0; c < 12; ++c) t.m[c] = f[(size_t)c * stride];
   return t;
 }
 
@@ -158,7 +159,7 @@ constexpr int kWordOutOfBounds = 2;
 template <int NDOF>
 __global__ void __launch_bounds__(kFkBlock)
     fk_points_kernel(DevRobot rb, const float* __restrict__ q, long long offset, int n,
-                     float* __restrict__ frames, int* __restrict__ word, int check_bounds) {
+                     float* __restrict__ frames, long long fstride, int* __restrict__ word, int check_bounds) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int nd = NDOF > 0 ? NDOF : rb.ndof;
@@ -175,7 +176,7 @@ __global__ void __launch_bounds__(kFkBlock)
   } else {
     word[i] = kWordFree;
   }
-  fk_store<NDOF>(rb, qq, frames + (size_t)i * nd * 12);
+  fk_store<NDOF>(rb, qq, frames, i, fstride);
 }
 
 // codes[i] from the point mode result word.
@@ -191,7 +192,7 @@ template <int NDOF>
 __global__ void __launch_bounds__(kFkBlock)
     fk_edges_kernel(DevRobot rb, EdgeArgs ea, const int* __restrict__ item_edge,
                     const int* __restrict__ item_pos, const int* __restrict__ nitems_dev, int cap,
-                    float* __restrict__ frames) {
+                    float* __restrict__ frames, long long fstride) {
   int n = *nitems_dev;
   if (n > cap) n = cap;
   const int nd = NDOF > 0 ? NDOF : rb.ndof;
@@ -212,7 +213,7 @@ __global__ void __launch_bounds__(kFkBlock)
 #pragma unroll
     for (int j = 0; j < (NDOF > 0 ? NDOF : kMaxDofDev); ++j)
       if (NDOF > 0 || j < nd) qq[j] = c1 * ea.qs[e + ea.offset * j] + c2 * ea.qe[e + ea.offset * j];
-    fk_store<NDOF>(rb, qq, frames + (size_t)i * nd * 12);
+    fk_store<NDOF>(rb, qq, frames, i, fstride);
   }
 }
 
@@ -347,11 +348,10 @@ struct Lds {
   const unsigned short* adj;    // rows of 4k neighbour ids, padded with the row's own vertex
 };
 
-// Per-lane iteration history R[k] (x, y, z) and combs[k] (w: p | q << 16),
-// in global memory: only iterations that do not separate touch it.
-struct Hist {
-  float4* h;  // [kMaxIter] for this lane
-};
+// Per-lane iteration history R[k] (x, y, z) and combs[k] (w: p | q << 16) in
+// global memory, entry k of a lane at H[k * hs] (lanes adjacent, so the lanes
+// of a wave at the same k touch one contiguous span); only iterations that do
+// not separate touch it.
 
 __device__ __forceinline__ int info_of(const float4& r) { return __float_as_int(r.w); }
 
@@ -401,7 +401,8 @@ __device__ __forceinline__ V2 mult23(const Orth& o, const V3& x) {
 }
 
 // find_half_plane (collision4.h:136-223), R[0..k-1] in the lane's history.
-__device__ __forceinline__ bool find_half_plane(const float4* __restrict__ H, int k, V3& w, const V3& rk) {
+__device__ __forceinline__ bool find_half_plane(const float4* __restrict__ H, size_t hs, int k, V3& w,
+                                                const V3& rk) {
   float dp = dot3(w, rk);
   if (dp > 0) return true;
   Orth o;
@@ -413,14 +414,14 @@ __device__ __forceinline__ bool find_half_plane(const float4* __restrict__ H, in
     o.M3 = -y / d;    o.M4 = x / d;     o.M5 = 0.0f;
   }
   V2 ra = mult23(o, v3(H[0]));
-  V2 rb = mult23(o, v3(H[1]));
+  V2 rb = mult23(o, v3(H[hs]));
   if (cross2(ra, rb) < 0) {
     V2 t = ra;
     ra = rb;
     rb = t;
   }
   for (int i = 2; i < k; ++i) {
-    V2 T = mult23(o, v3(H[i]));
+    V2 T = mult23(o, v3(H[i * hs]));
     float rbcross = cross2(rb, T);
     if (cross2(ra, T) > 0) {
       if (rbcross > 0) rb = T;
@@ -442,11 +443,11 @@ __device__ __forceinline__ bool find_half_plane(const float4* __restrict__ H, in
 // Iteration that did not separate (collision4.h:301-323): repetition test,
 // w update, half plane, reflection.  Returns a nonzero code when the pair is
 // decided (l+1: no half plane), else 0.  Taken by few iterations.
-__device__ __forceinline__ int sva_slow(float4* __restrict__ H, int& k, int l, int pq, const V3& r, float dp,
-                                     V3& S, V3& w) {
-  H[k] = make_float4(r.x, r.y, r.z, __int_as_float(pq));
+__device__ __forceinline__ int sva_slow(float4* __restrict__ H, size_t hs, int& k, int l, int pq, const V3& r,
+                                        float dp, V3& S, V3& w) {
+  H[k * hs] = make_float4(r.x, r.y, r.z, __int_as_float(pq));
   for (int j = 0; j < k; ++j)
-    if (__float_as_int(H[j].w) == pq) {
+    if (__float_as_int(H[j * hs].w) == pq) {
       S = w;  // (p,q) seen before
       return 0;
     }
@@ -455,7 +456,7 @@ __device__ __forceinline__ int sva_slow(float4* __restrict__ H, int& k, int l, i
     w = V3{r0.x + r.x, r0.y + r.y, r0.z + r.z};
     normalize3(w);
   }
-  if (k >= 2 && !find_half_plane(H, k, w, r)) return l + 1;
+  if (k >= 2 && !find_half_plane(H, hs, k, w, r)) return l + 1;
   float b = -2.0f * dp;
   S.x = S.x + b * r.x;
   S.y = S.y + b * r.y;
@@ -482,7 +483,8 @@ __global__ void __launch_bounds__(BLOCK) sva_kernel(SvaArgs A) {
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
-  float4* __restrict__ H = A.hist + ((size_t)blockIdx.x * BLOCK + tid) * kMaxIter;
+  float4* __restrict__ H = A.hist + (size_t)blockIdx.x * BLOCK + tid;
+  const size_t hs = (size_t)gridDim.x * BLOCK;
   const int nitems = A.nitems >= 0 ? A.nitems : *A.nitems_dev;
   const int M = A.img.M;
   const int ndof = A.ndof;
@@ -572,8 +574,8 @@ __global__ void __launch_bounds__(BLOCK) sva_kernel(SvaArgs A) {
       const int sP = pd.y & 255, sQ = pd.y >> 8;
       sysP0 = sP == 0;
       sysQ0 = sQ == 0;
-      if (!sysP0 && sP != fsP) { tp = load_frame(A.frames, item, ndof, sP); fsP = sP; }
-      if (!sysQ0 && sQ != fsQ) { tq = load_frame(A.frames, item, ndof, sQ); fsQ = sQ; }
+      if (!sysP0 && sP != fsP) { tp = load_frame(A.frames, item, A.fstride, sP); fsP = sP; }
+      if (!sysQ0 && sQ != fsQ) { tq = load_frame(A.frames, item, A.fstride, sQ); fsQ = sQ; }
       V3 vq = v3(L.verts[vbQ]), vp = v3(L.verts[vbP]);
       V3 Pq = sysQ0 ? vq : t_apply(tq, vq);
       V3 Pp = sysP0 ? vp : t_apply(tp, vp);
@@ -643,7 +645,7 @@ __global__ void __launch_bounds__(BLOCK) sva_kernel(SvaArgs A) {
     if (dp >= -kEpsBelow) {
       fin = true;
     } else {
-      code = sva_slow(H, k, l, (p - vbP) | ((q - vbQ) << 16), r, dp, S, w);
+      code = sva_slow(H, hs, k, l, (p - vbP) | ((q - vbQ) << 16), r, dp, S, w);
       if (code != 0) {
         fin = true;
       } else if (++l >= kMaxIter) {
@@ -735,7 +737,8 @@ hipError_t launch_fk_points(const DevRobot& rb, const float* q, long long offset
                             int* word, int check_bounds, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   dim3 grid((n + kFkBlock - 1) / kFkBlock), block(kFkBlock);
-  GPRM_FK_DISPATCH(fk_points_kernel, grid, block, st, rb, q, offset, n, frames, word, check_bounds);
+  GPRM_FK_DISPATCH(fk_points_kernel, grid, block, st, rb, q, offset, n, frames, (long long)n, word,
+                   check_bounds);
   return hipGetLastError();
 }
 
@@ -751,7 +754,8 @@ hipError_t launch_edge_fk(const DevRobot& rb, EdgeArgs ea, const int* item_edge,
   int blocks = (cap + kFkBlock - 1) / kFkBlock;
   if (blocks > 4096) blocks = 4096;
   dim3 grid(blocks), block(kFkBlock);
-  GPRM_FK_DISPATCH(fk_edges_kernel, grid, block, st, rb, ea, item_edge, item_pos, nitems, cap, frames);
+  GPRM_FK_DISPATCH(fk_edges_kernel, grid, block, st, rb, ea, item_edge, item_pos, nitems, cap, frames,
+                   (long long)cap);
   return hipGetLastError();
 }
 

@@ -12,19 +12,25 @@
 //   2. list every stored node of either tree closer than D (L2) to each new
 //      node (find_neighbours / get_near_vertices, :2532-2574, :2701-2748) with
 //      a GPU radius search over all nodes instead of the q[0] block index;
-//   3. check all candidate edges with the exact edge check (indicator2);
+//   3. check the candidate edges with the exact edge check (indicator2): all
+//      of them, as the reference does, or at most edge_budget per (new node,
+//      tree), ranked by the collision-probability model when one is given and
+//      by length otherwise;
 //   4. insert each new node into every tree it has a free edge to, with all
-//      its free edges (calc_conn, part3a); a node joining both trees connects
-//      them, and Dijkstra on each tree (do_dijkstra, :2804-2846) gives the path.
-// An optional collision-probability model (surrogate.hip) biases sampling
-// and orders edge checks; the exact checks above still decide every node and
-// edge that enters a tree.
+//      its free checked edges (calc_conn, part3a); a node joining both trees
+//      connects them, and Dijkstra on each tree (do_dijkstra, :2804-2846)
+//      gives the path.
+// The optional model (surrogate.hip) ranks; it never decides.  Sampling bias
+// drops candidates before their exact check, edge ranking chooses which
+// edges are checked; every node and edge that enters a tree has been decided
+// free by the exact checks above.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <numeric>
 #include <queue>
 #include <vector>
 
@@ -40,6 +46,9 @@ using Clock = std::chrono::steady_clock;
 double ms_since(Clock::time_point t) {
   return std::chrono::duration<double, std::milli>(Clock::now() - t).count();
 }
+
+// Points scored along an edge by the model: interior points at t = s/(S+1).
+constexpr int kScorePoints = 8;
 
 struct DevArr {
   void* p = nullptr;
@@ -61,23 +70,38 @@ struct DevArr {
 // prmsolver.h:40-58, without the block index).
 struct Tree {
   int ndof = 0, cap = 0, n = 0;
+  int ndev = 0;                         // nodes [0, ndev) are on the device
   std::vector<float> q;                 // AoS ndof*n
   std::vector<int> surrnum;             // neighbours per node
   std::vector<std::vector<std::pair<int, float>>> adj;
   DevArr dq;                            // SoA ndof x cap on the device
+  std::vector<float> stage;             // host staging of one upload
 };
 
-
-int insert(Tree& t, const float* q, hipStream_t st) {
+int insert(Tree& t, const float* q) {
   if (t.n >= t.cap) return -1;
   int pos = t.n++;
   t.q.insert(t.q.end(), q, q + t.ndof);
   t.surrnum.push_back(0);
   t.adj.emplace_back();
-  for (int i = 0; i < t.ndof; ++i)
-    if (hipMemcpyAsync(t.dq.as<float>() + (size_t)t.cap * i + pos, q + i, 4, hipMemcpyHostToDevice, st) != hipSuccess)
-      return -1;
   return pos;
+}
+
+// Upload nodes [ndev, n) to the device copy, one copy per component.
+bool flush(Tree& t, hipStream_t st) {
+  const int cnt = t.n - t.ndev;
+  if (cnt <= 0) return true;
+  // the previous upload may still read the staging buffer
+  if (hipStreamSynchronize(st) != hipSuccess) return false;
+  t.stage.resize((size_t)t.ndof * cnt);
+  for (int m = 0; m < cnt; ++m)
+    for (int i = 0; i < t.ndof; ++i) t.stage[(size_t)cnt * i + m] = t.q[(size_t)t.ndof * (t.ndev + m) + i];
+  for (int i = 0; i < t.ndof; ++i)
+    if (hipMemcpyAsync(t.dq.as<float>() + (size_t)t.cap * i + t.ndev, t.stage.data() + (size_t)cnt * i,
+                       sizeof(float) * cnt, hipMemcpyHostToDevice, st) != hipSuccess)
+      return false;
+  t.ndev = t.n;
+  return true;
 }
 
 // Dijkstra on one tree (do_dijkstra, prmsolver.cpp:2804-2846).
@@ -113,6 +137,20 @@ bool dijkstra(const Tree& t, int from, int to, std::vector<int>* path) {
   return false;
 }
 
+// Model probabilities of n host configurations (SoA with stride n).
+int predict_host(Surrogate* model, const std::vector<float>& qT, int nd, int n, DevArr& d_q, DevArr& d_p,
+                 std::vector<float>* prob, hipStream_t st) {
+  prob->resize(n);
+  if (n == 0) return GPRM_OK;
+  if (!d_q.ensure(sizeof(float) * nd * (size_t)n) || !d_p.ensure(sizeof(float) * (size_t)n)) return GPRM_E_HIP;
+  if (hipMemcpyAsync(d_q.p, qT.data(), sizeof(float) * nd * (size_t)n, hipMemcpyHostToDevice, st) != hipSuccess ||
+      surrogate_predict(model, d_q.as<float>(), n, n, d_p.as<float>(), st) != 0 ||
+      hipMemcpyAsync(prob->data(), d_p.p, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return GPRM_E_HIP;
+  return GPRM_OK;
+}
+
 }  // namespace
 }  // namespace gprm
 
@@ -123,13 +161,16 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
   if (!ctx || !qstart || !qgoal || !prm || !res) return GPRM_E_INVALID;
   std::memset(res, 0, sizeof(*res));
   const int nd = gprm_dim(ctx);
-  if (prm->num < 2 || !(prm->D > 0.0f) || prm->max_nodes < 2 || prm->max_steps < 1 || prm->nbuf < 1)
+  if (prm->num < 2 || !(prm->D > 0.0f) || prm->max_nodes < 2 || prm->max_steps < 1 || prm->nbuf < 1 ||
+      prm->edge_budget < 0)
     return GPRM_E_INVALID;
   const auto t0 = Clock::now();
   hipStream_t st = nullptr;
   if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return GPRM_E_HIP;
   struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); } } sg{st};
   Surrogate* model = static_cast<Surrogate*>(prm->surrogate);
+  const bool bias = model && prm->bias_sampling;
+  const bool rank_by_model = model && prm->order_edges;
 
   Tree T[2];
   for (int g = 0; g < 2; ++g) {
@@ -144,27 +185,25 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
   if (rc != GPRM_OK) return rc;
   res->point_checks += 2;
   if (c0 != 0 || c1 != 0) return GPRM_OK;  // found = 0
-  insert(T[0], qstart, st);
-  insert(T[1], qgoal, st);
+  insert(T[0], qstart);
+  insert(T[1], qgoal);
+  if (!flush(T[0], st) || !flush(T[1], st)) return GPRM_E_HIP;
 
   Rng rng(prm->seed * 0x9E3779B97F4A7C15ULL + 12345);
   const int num = prm->num;
   const float D = prm->D, D2 = prm->D * prm->D;
   const int batch = 4 * num;  // candidates drawn per exact point-check batch
-  std::vector<float> candT((size_t)nd * batch);
-  std::vector<int> cand_tree(batch), ccode(batch);
+  std::vector<float> candT((size_t)nd * batch), keptT;
+  std::vector<int> cand_tree(batch), kept, ccode(batch);
   std::vector<float> qnew((size_t)nd * num);
-  std::vector<int> qtree(num);
-  std::vector<float> prob(batch);
-  DevArr d_cand, d_code, d_new, d_nq, d_nn, d_nd2, d_cnt, d_qs, d_qe, d_er, d_prob;
+  std::vector<float> prob;
+  DevArr d_cand, d_code, d_new, d_nq, d_nn, d_nd2, d_cnt, d_qs, d_qe, d_er, d_mq, d_mp;
   if (!d_cand.ensure(sizeof(float) * nd * batch) || !d_code.ensure(sizeof(int) * batch) ||
       !d_new.ensure(sizeof(float) * nd * num) || !d_nq.ensure(sizeof(int) * prm->nbuf) ||
-      !d_nn.ensure(sizeof(int) * prm->nbuf) || !d_nd2.ensure(sizeof(float) * prm->nbuf) || !d_cnt.ensure(8) ||
-      !d_qs.ensure(sizeof(float) * nd * prm->nbuf) || !d_qe.ensure(sizeof(float) * nd * prm->nbuf) ||
-      !d_er.ensure(sizeof(int) * prm->nbuf) || !d_prob.ensure(sizeof(float) * batch))
+      !d_nn.ensure(sizeof(int) * prm->nbuf) || !d_nd2.ensure(sizeof(float) * prm->nbuf) || !d_cnt.ensure(8))
     return GPRM_E_HIP;
   std::vector<int> hq, hn, her;
-  std::vector<float> hd2, hqs, hqe;
+  std::vector<float> hd2, hqs, hqe, scoreT, score;
 
   int conn_left = -1, conn_right = -1;
   for (int step = 0; step < prm->max_steps && conn_left < 0; ++step) {
@@ -200,31 +239,40 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
         for (int i = 0; i < nd; ++i)
           candT[(size_t)batch * i + k] = t.q[(size_t)nd * m + i] - D + 2 * D * (float)rng.uniform();
       }
-      // optional bias: half of the batch keeps only candidates the model
-      // predicts free (the exact check below still decides every one)
-      std::vector<char> skip(batch, 0);
-      if (model && prm->bias_sampling) {
-        if (hipMemcpyAsync(d_cand.p, candT.data(), sizeof(float) * nd * batch, hipMemcpyHostToDevice, st) != hipSuccess ||
-            surrogate_predict(model, d_cand.as<float>(), batch, batch, d_prob.as<float>(), st) != 0 ||
-            hipMemcpyAsync(prob.data(), d_prob.p, sizeof(float) * batch, hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipStreamSynchronize(st) != hipSuccess)
-          return GPRM_E_HIP;
-        for (int k = 0; k < batch; ++k)
-          if ((k & 2) && prob[k] > 0.5f) skip[k] = 1;
+      // Sampling bias: every second pair of candidates must be predicted
+      // free; candidates it drops are never checked.  The others, and all
+      // candidates without a model, go to the exact check.
+      kept.clear();
+      if (bias) {
+        auto tm = Clock::now();
+        rc = predict_host(model, candT, nd, batch, d_mq, d_mp, &prob, st);
+        if (rc != GPRM_OK) return rc;
         res->model_predictions += batch;
+        res->ms_model += ms_since(tm);
+        for (int k = 0; k < batch; ++k)
+          if (!((k & 2) && prob[k] > 0.5f)) kept.push_back(k);
+      } else {
+        kept.resize(batch);
+        std::iota(kept.begin(), kept.end(), 0);
       }
-      if (hipMemcpyAsync(d_cand.p, candT.data(), sizeof(float) * nd * batch, hipMemcpyHostToDevice, st) != hipSuccess)
+      const int nk = (int)kept.size();
+      res->candidates_skipped += batch - nk;
+      keptT.resize((size_t)nd * nk);
+      for (int i = 0; i < nd; ++i)
+        for (int a = 0; a < nk; ++a) keptT[(size_t)nk * i + a] = candT[(size_t)batch * i + kept[a]];
+      if (nk == 0) continue;
+      if (hipMemcpyAsync(d_cand.p, keptT.data(), sizeof(float) * nd * nk, hipMemcpyHostToDevice, st) != hipSuccess)
         return GPRM_E_HIP;
-      rc = gprm_indicator_dev(ctx, d_cand.as<float>(), d_code.as<int>(), batch, batch, st);
+      rc = gprm_indicator_dev(ctx, d_cand.as<float>(), d_code.as<int>(), nk, nk, st);
       if (rc != GPRM_OK) return rc;
-      if (hipMemcpy(ccode.data(), d_code.p, sizeof(int) * batch, hipMemcpyDeviceToHost) != hipSuccess)
+      if (hipMemcpyAsync(ccode.data(), d_code.p, sizeof(int) * nk, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
         return GPRM_E_HIP;
-      res->point_checks += batch;
-      for (int k = 0; k < batch && filled < num; ++k) {
-        const int g = cand_tree[k];
-        if (skip[k] || ccode[k] != 0 || have[g] >= want[g]) continue;
-        for (int i = 0; i < nd; ++i) qnew[(size_t)nd * filled + i] = candT[(size_t)batch * i + k];
-        qtree[filled] = g;
+      res->point_checks += nk;
+      for (int a = 0; a < nk && filled < num; ++a) {
+        const int g = cand_tree[kept[a]];
+        if (ccode[a] != 0 || have[g] >= want[g]) continue;
+        for (int i = 0; i < nd; ++i) qnew[(size_t)nd * filled + i] = keptT[(size_t)nk * i + a];
         ++have[g];
         ++filled;
       }
@@ -266,6 +314,64 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
       return a.j != b.j ? a.j < b.j : (a.g != b.g ? a.g < b.g : a.node < b.node);
     });
     if ((long long)cands.size() > prm->nbuf) cands.resize(prm->nbuf);
+    res->ms_near += ms_since(tn);
+
+    // ---- 2b. edge budget: rank the candidates of each (new node, tree) -------
+    if (prm->edge_budget > 0) {
+      const int B = prm->edge_budget;
+      // groups of equal (j, g) are contiguous
+      std::vector<size_t> gstart;
+      for (size_t i = 0; i < cands.size(); ++i)
+        if (i == 0 || cands[i].j != cands[i - 1].j || cands[i].g != cands[i - 1].g) gstart.push_back(i);
+      gstart.push_back(cands.size());
+      // the model scores only edges of groups with more than B candidates
+      score.assign(cands.size(), 0.0f);
+      if (rank_by_model) {
+        auto tm = Clock::now();
+        std::vector<size_t> need;
+        for (size_t a = 0; a + 1 < gstart.size(); ++a)
+          if (gstart[a + 1] - gstart[a] > (size_t)B)
+            for (size_t i = gstart[a]; i < gstart[a + 1]; ++i) need.push_back(i);
+        const int np = (int)(need.size() * kScorePoints);
+        scoreT.resize((size_t)nd * np);
+        for (size_t a = 0; a < need.size(); ++a) {
+          const Cand& c = cands[need[a]];
+          const float* qa = &qnew[(size_t)nd * c.j];
+          const float* qb = &T[c.g].q[(size_t)nd * c.node];
+          for (int s = 0; s < kScorePoints; ++s) {
+            const float t = (float)(s + 1) / (float)(kScorePoints + 1);
+            for (int i = 0; i < nd; ++i)
+              scoreT[(size_t)np * i + a * kScorePoints + s] = qa[i] + t * (qb[i] - qa[i]);
+          }
+        }
+        rc = predict_host(model, scoreT, nd, np, d_mq, d_mp, &prob, st);
+        if (rc != GPRM_OK) return rc;
+        res->model_predictions += np;
+        for (size_t a = 0; a < need.size(); ++a) {
+          float m = 0.0f;
+          for (int s = 0; s < kScorePoints; ++s) m = std::max(m, prob[a * kScorePoints + s]);
+          score[need[a]] = m;
+        }
+        res->ms_model += ms_since(tm);
+      }
+      std::vector<Cand> keep;
+      std::vector<size_t> idx;
+      for (size_t a = 0; a + 1 < gstart.size(); ++a) {
+        const size_t b0 = gstart[a], b1 = gstart[a + 1];
+        idx.resize(b1 - b0);
+        std::iota(idx.begin(), idx.end(), b0);
+        if (b1 - b0 > (size_t)B) {
+          std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) {
+            return score[x] != score[y] ? score[x] < score[y] : cands[x].d2 < cands[y].d2;
+          });
+          idx.resize(B);
+          std::sort(idx.begin(), idx.end());
+        }
+        for (size_t i : idx) keep.push_back(cands[i]);
+      }
+      res->edges_skipped += (long long)(cands.size() - keep.size());
+      cands.swap(keep);
+    }
     std::vector<int> eq(cands.size()), eg(cands.size()), en(cands.size());
     std::vector<float> ed(cands.size());
     for (size_t i = 0; i < cands.size(); ++i) {
@@ -274,13 +380,15 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
       en[i] = cands[i].node;
       ed[i] = std::sqrt(cands[i].d2);
     }
-    res->ms_near += ms_since(tn);
 
     // ---- 3. exact edge checks ------------------------------------------------
     auto te = Clock::now();
     const int E = (int)eq.size();
     her.assign(E, 1);
     if (E > 0) {
+      if (!d_qs.ensure(sizeof(float) * nd * (size_t)E) || !d_qe.ensure(sizeof(float) * nd * (size_t)E) ||
+          !d_er.ensure(sizeof(int) * (size_t)E))
+        return GPRM_E_HIP;
       hqs.resize((size_t)nd * E);
       hqe.resize((size_t)nd * E);
       for (int e = 0; e < E; ++e)
@@ -294,7 +402,9 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
       gprm_edge_stats es;
       rc = gprm_indicator2_dev(ctx, d_qs.as<float>(), d_qe.as<float>(), d_er.as<int>(), E, E, st, &es);
       if (rc != GPRM_OK) return rc;
-      if (hipMemcpy(her.data(), d_er.p, sizeof(int) * E, hipMemcpyDeviceToHost) != hipSuccess) return GPRM_E_HIP;
+      if (hipMemcpyAsync(her.data(), d_er.p, sizeof(int) * E, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        return GPRM_E_HIP;
       res->edges_checked += E;
       res->edge_samples += es.samples_total;
     }
@@ -314,7 +424,7 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
         for (size_t k = e0; k < e1; ++k)
           if (eg[k] == g && her[k] == 0) { any = true; break; }
         if (!any) continue;
-        pos[g] = insert(T[g], &qnew[(size_t)nd * j], st);
+        pos[g] = insert(T[g], &qnew[(size_t)nd * j]);
         if (pos[g] < 0) break;
         for (size_t k = e0; k < e1; ++k) {
           if (eg[k] != g || her[k] != 0) continue;
@@ -333,6 +443,7 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
       }
       if ((pos[0] < 0 && T[0].n >= T[0].cap) || (pos[1] < 0 && T[1].n >= T[1].cap)) break;
     }
+    if (!flush(T[0], st) || !flush(T[1], st)) return GPRM_E_HIP;
     res->ms_host += ms_since(th);
     if (T[0].n >= T[0].cap || T[1].n >= T[1].cap) break;
   }
@@ -342,7 +453,6 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
     std::vector<int> pl, pr;
     bool ok = dijkstra(T[0], 0, conn_left, &pl) && dijkstra(T[1], conn_right, 0, &pr);
     if (ok) {
-      // start ... connection (left path is goal-first), then connection ... goal
       // pl = [conn_left, ..., start], pr = [goal, ..., conn_right]; conn_left
       // and conn_right hold the same configuration
       std::vector<const float*> seq;

@@ -37,8 +37,10 @@ struct SceneImg {
 struct SvaArgs {
   SceneImg img;
   int ndof;
-  const float* frames;     // [item][ndof][12] link frames (columns 0..3, xyz)
-  float4* hist;            // [grid*block][kMaxIter] per-lane iteration history
+  const float* frames;     // link frames, component-major: entry c of link s (1..ndof)
+                           // of item i at frames[(12 (s-1) + c) fstride + i]
+  long long fstride;       // item stride of frames (n of the kinematics launch, or its cap)
+  float4* hist;            // [kMaxIter][grid*block] per-lane iteration history
   int nitems;              // item count, or -1: read *nitems_dev
   const int* nitems_dev;
   unsigned int* counter;   // work counter (zero before launch)
@@ -70,6 +72,7 @@ size_t sva_lds_bytes(const SceneImg& img);
 int sva_max_lds_bytes();
 int sva_block_for(const SceneImg& img);
 
+// Kinematics of n configurations; frames component-major with item stride n.
 hipError_t launch_fk_points(const DevRobot& rb, const float* q, long long offset, int n, float* frames,
                             int* word, int check_bounds, hipStream_t st);
 hipError_t launch_finalize_points(const int* word, int* codes, int n, hipStream_t st);
@@ -79,6 +82,8 @@ hipError_t launch_edge_prologue(const DevRobot& rb, float dq, EdgeArgs ea, int* 
                                 hipStream_t st);
 hipError_t launch_edge_compact(EdgeArgs ea, int e0, int e1, int lo, int hi, int* item_edge,
                                int* item_pos, int* nitems, hipStream_t st);
+// Kinematics of the *nitems (<= cap) compacted samples; frames component-major
+// with item stride cap.
 hipError_t launch_edge_fk(const DevRobot& rb, EdgeArgs ea, const int* item_edge, const int* item_pos,
                           const int* nitems, int cap, float* frames, hipStream_t st);
 hipError_t launch_fill_i32(int* p, int v, long long n, hipStream_t st);

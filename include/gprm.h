@@ -157,8 +157,13 @@ typedef struct {
   unsigned long long seed;
   int nbuf;                 /* candidate edges per step and tree */
   void* surrogate;          /* optional model (gprm_surrogate_*), or NULL */
-  int bias_sampling;        /* with a model: half the candidates must be predicted free */
-  int order_edges;          /* with a model: check edges of likely-free nodes first */
+  int bias_sampling;        /* with a model: every second pair of candidates must be
+                               predicted free (p < 0.5); the rest are dropped unchecked */
+  int order_edges;          /* with a model and edge_budget: rank a node's candidate edges
+                               by the model (max probability at 8 interior points) */
+  int edge_budget;          /* 0: check every candidate edge (the reference's rule);
+                               k > 0: at most k per (new node, tree), lowest rank first
+                               (by the model with order_edges, else shortest first) */
 } gprm_prm_params;
 typedef struct {
   int found, steps, path_len;
@@ -167,7 +172,10 @@ typedef struct {
   long long edges_checked, edges_free;
   long long edge_samples;   /* sum of testnum over checked edges */
   long long model_predictions;
+  long long candidates_skipped;  /* sampling candidates dropped unchecked by the bias */
+  long long edges_skipped;       /* candidate edges left unchecked by edge_budget */
   double ms_total, ms_sampling, ms_near, ms_edges, ms_host;
+  double ms_model;               /* model inference inside the steps */
 } gprm_prm_result;
 int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* qgoal, const gprm_prm_params* params,
                    gprm_prm_result* res, float* path, int path_cap);

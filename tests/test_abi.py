@@ -51,6 +51,30 @@ def test_c_header_compiles_as_c(tmp_path):
                     "-o", str(tmp_path / "use.o")], check=True)
 
 
+def test_python_structs_match_c_layout(tmp_path):
+    """Every ctypes mirror of a header struct has the C size and field offsets."""
+    import gpuprm_amd
+    pairs = [("gprm_edge_stats", gpuprm_amd.EdgeStats), ("gprm_profile_stats", gpuprm_amd.ProfileStats),
+             ("gprm_prm_params", gpuprm_amd.PrmParams), ("gprm_prm_result", gpuprm_amd.PrmResult)]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gprm.h"', 'int main(void){']
+    for cname, py in pairs:
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in py._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                  check=True).stdout.splitlines())
+    for cname, py in pairs:
+        assert int(got[cname]) == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got["%s.%s" % (cname, f)]) == getattr(py, f).offset, (cname, f)
+
+
 def _ctx(path):
     import gpuprm_amd
     L = gpuprm_amd.lib()
