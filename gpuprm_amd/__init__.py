@@ -18,7 +18,8 @@ from typing import Optional
 import numpy as np
 
 __all__ = ["RobotConfigspace", "Surrogate", "GprmError", "EdgeStats", "ProfileStats", "PrmParams", "PrmResult", "lib",
-           "generate_scene", "LIB_PATH"]
+           "generate_scene", "LIB_PATH", "LaunchShape",
+           "BucketInfo", "SceneBatch"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgprm.so")
 
@@ -78,6 +79,28 @@ class PrmParams(C.Structure):
         ("bias_sampling", C.c_int),
         ("order_edges", C.c_int),
         ("edge_budget", C.c_int),
+    ]
+
+
+class LaunchShape(C.Structure):
+    _fields_ = [
+        ("block", C.c_int),
+        ("waves", C.c_int),
+        ("pairs_per_unit", C.c_int),
+        ("grid", C.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BucketInfo(C.Structure):
+    _fields_ = [
+        ("min_vertices", C.c_int),
+        ("max_vertices", C.c_int),
+        ("scenes", C.c_int),
+        ("lds_bytes", C.c_int),
+        ("shape", LaunchShape),
     ]
 
 
@@ -171,6 +194,20 @@ def lib():
     L.gprm_surrogate_predict.argtypes = [vp, vp, C.c_int, C.c_longlong, vp, vp]
     L.gprm_profile_enable.argtypes = [vp, C.c_int]
     L.gprm_profile_read.argtypes = [vp, C.POINTER(ProfileStats)]
+    L.gprm_get_launch_shape.argtypes = [vp, C.POINTER(LaunchShape)]
+    L.gprm_set_launch_shape.argtypes = [vp, C.POINTER(LaunchShape)]
+    L.gprm_autotune.argtypes = [vp, vp, vp, C.c_int, C.c_longlong, C.c_int, C.POINTER(LaunchShape), _FP]
+    PP = C.POINTER(vp)
+    LL = C.POINTER(C.c_longlong)
+    L.gprm_batch_create.argtypes = [PP, C.c_int, PP]
+    L.gprm_batch_destroy.argtypes = [vp]
+    L.gprm_batch_destroy.restype = None
+    L.gprm_batch_bucket_info.argtypes = [vp, C.c_int, C.POINTER(BucketInfo)]
+    L.gprm_batch_set_bucket_shape.argtypes = [vp, C.c_int, C.POINTER(LaunchShape)]
+    L.gprm_batch_indicator_dev.argtypes = [vp, PP, LL, _IP, PP, C.c_int, _FP]
+    L.gprm_batch_autotune.argtypes = [vp, PP, LL, _IP, PP, C.c_int]
+    L.gprm_batch_last_error.argtypes = [vp]
+    L.gprm_batch_last_error.restype = C.c_char_p
     _lib = L
     return L
 
@@ -180,6 +217,92 @@ def _check(rc: int, ctx=None) -> int:
         msg = lib().gprm_last_error(ctx)
         raise GprmError(rc, msg.decode() if msg else "")
     return rc
+
+
+class SceneBatch:
+    """Point checks of many scenes as one launch sequence (gprm_batch_*):
+    one kinematics launch, one persistent separating-vector launch per
+    vertex-count bucket, one finalize launch; optionally a replayed hipGraph.
+    `spaces` are initialised RobotConfigspace objects on one device with one
+    ndof; they must outlive the batch."""
+
+    BUCKETS = 4
+
+    def __init__(self, spaces):
+        self.spaces = list(spaces)
+        arr = (C.c_void_p * len(self.spaces))(*[s._ctx.value for s in self.spaces])
+        self._h = C.c_void_p()
+        rc = lib().gprm_batch_create(arr, len(self.spaces), C.byref(self._h))
+        if rc < 0:
+            raise GprmError(rc, (lib().gprm_batch_last_error(None) or b"").decode())
+
+    def _ok(self, rc):
+        if rc < 0:
+            raise GprmError(rc, (lib().gprm_batch_last_error(self._h) or b"").decode())
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().gprm_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def bucket_info(self):
+        out = []
+        for i in range(self.BUCKETS):
+            b = BucketInfo()
+            self._ok(lib().gprm_batch_bucket_info(self._h, i, C.byref(b)))
+            out.append({"min_vertices": b.min_vertices, "max_vertices": b.max_vertices, "scenes": b.scenes,
+                        "lds_bytes": b.lds_bytes, "shape": b.shape.as_dict()})
+        return out
+
+    def set_bucket_shape(self, bucket: int, block: int, waves: int, pairs_per_unit: int) -> None:
+        sh = LaunchShape(int(block), int(waves), int(pairs_per_unit), 0)
+        self._ok(lib().gprm_batch_set_bucket_shape(self._h, int(bucket), C.byref(sh)))
+
+    def tuning(self):
+        """The per-bucket shapes, as a table that apply_tuning() accepts."""
+        return [{"min_vertices": b["min_vertices"], "max_vertices": b["max_vertices"],
+                 "block": b["shape"]["block"], "waves": b["shape"]["waves"],
+                 "pairs_per_unit": b["shape"]["pairs_per_unit"]} for b in self.bucket_info() if b["scenes"]]
+
+    def apply_tuning(self, table) -> None:
+        """Set the shape of every bucket in use whose vertex range a row of
+        `table` names (rows of tuning())."""
+        for i, b in enumerate(self.bucket_info()):
+            if not b["scenes"]:
+                continue
+            for row in table:
+                if (row["min_vertices"], row["max_vertices"]) == (b["min_vertices"], b["max_vertices"]):
+                    self.set_bucket_shape(i, row["block"], row["waves"], row["pairs_per_unit"])
+
+    def _arrays(self, qs, res):
+        S = len(self.spaces)
+        assert len(qs) == S and len(res) == S
+        qa = (C.c_void_p * S)(*[q.data_ptr() for q in qs])
+        ra = (C.c_void_p * S)(*[r.data_ptr() for r in res])
+        n = (C.c_int * S)(*[int(q.shape[1]) for q in qs])
+        off = (C.c_longlong * S)(*[max(1, int(q.shape[1])) for q in qs])
+        return qa, off, n, ra
+
+    def indicator_dev(self, qs, res, use_graph: bool = False) -> np.ndarray:
+        """qs[s]: float32 device tensor (ndof, n_s) contiguous; res[s]: int32
+        device tensor (n_s,).  Returns each scene's completion time in ms
+        after the separating-vector launch started (device clock)."""
+        qa, off, n, ra = self._arrays(qs, res)
+        done = np.zeros(len(self.spaces), np.float32)
+        self._ok(lib().gprm_batch_indicator_dev(self._h, qa, off, n, ra, int(bool(use_graph)),
+                                                done.ctypes.data_as(_FP)))
+        return done
+
+    def autotune(self, qs, res, reps: int = 2):
+        qa, off, n, ra = self._arrays(qs, res)
+        self._ok(lib().gprm_batch_autotune(self._h, qa, off, n, ra, int(reps)))
+        return self.tuning()
 
 
 def generate_scene(workload: int, seed: int, out_dir: str) -> str:
@@ -445,6 +568,29 @@ class RobotConfigspace:
         st = ProfileStats()
         _check(lib().gprm_profile_read(self._ctx, C.byref(st)), self._ctx)
         return st
+
+    # -- launch shape and autotuner (no reference counterpart) -----------
+    def get_launch_shape(self) -> LaunchShape:
+        sh = LaunchShape()
+        _check(lib().gprm_get_launch_shape(self._ctx, C.byref(sh)), self._ctx)
+        return sh
+
+    def set_launch_shape(self, block: int, waves: int, pairs_per_unit: int) -> LaunchShape:
+        sh = LaunchShape(int(block), int(waves), int(pairs_per_unit), 0)
+        _check(lib().gprm_set_launch_shape(self._ctx, C.byref(sh)), self._ctx)
+        return self.get_launch_shape()
+
+    def autotune(self, q, qe=None, reps: int = 3):
+        """Time every kernel variant x pairs-per-unit on device tensors q (and
+        qe for edges), (ndof, N) contiguous; keeps and returns the fastest
+        shape and its time in ms."""
+        N = int(q.shape[1])
+        sh = LaunchShape()
+        ms = C.c_float(0)
+        _check(lib().gprm_autotune(self._ctx, C.c_void_p(q.data_ptr()),
+                                   C.c_void_p(qe.data_ptr() if qe is not None else None), N, N, int(reps),
+                                   C.byref(sh), C.byref(ms)), self._ctx)
+        return sh, ms.value
 
     # -- helpers -----------------------------------------------------------
     def _check_soa(self, a, N, offset):

@@ -1,0 +1,407 @@
+// batch.cpp -- many scenes checked together on one device.
+//
+// No reference counterpart: the reference holds one scene per
+// RobotConfigspace object and launches one kernel per request
+// (robotconfigspace.cpp:718-723).  A batch takes initialised contexts (one
+// scene each, same device, same ndof) and runs the point checks of all of
+// them as one sequence: a fill of the work counters, ONE kinematics launch
+// over every scene's configurations, one persistent separating-vector launch
+// per vertex-count bucket (every scene of the bucket, blocks rotating over
+// the scenes, see sva_multi_kernel), and ONE finalize launch.  With use_graph
+// the sequence is captured once into a hipGraph and replayed for as long as
+// the call's arrays and sizes stay the same.
+//
+// Buckets group scenes by total vertex count; each bucket has its own launch
+// shape (block size and occupancy target, i.e. how many scene copies share
+// a CU's LDS, and pairs per work unit), set by the caller or picked by
+// gprm_batch_autotune from timed runs of the caller's workload.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gprm.h"
+#include "context.h"
+#include "runtime.h"
+
+namespace gprm {
+namespace {
+
+// Bucket edges in total scene vertices: [0,512), [512,1024), [1024,2048), [2048, inf).
+constexpr int kBucketEdges[] = {0, 512, 1024, 2048, 1 << 30};
+constexpr int kBuckets = 4;
+constexpr int kMaxBatchItems = 1 << 22;  // configurations per scene and call
+
+struct Bucket {
+  int lo = 0, hi = 0;          // vertex-count range
+  std::vector<int> scenes;     // indices into the batch
+  size_t lds = 0;              // largest image among its scenes
+  SvaShape shape;
+  DevBuf tasks;                // SceneTask[scenes.size()]
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct Signature {
+  std::vector<const void*> q, res;
+  std::vector<long long> off;
+  std::vector<int> n;
+  bool operator==(const Signature& o) const { return q == o.q && res == o.res && off == o.off && n == o.n; }
+};
+
+}  // namespace
+}  // namespace gprm
+
+using namespace gprm;
+
+struct gprm_batch {
+  std::vector<gprm_ctx*> ctx;
+  int device = 0, ndof = 0;
+  hipStream_t stream = nullptr;
+  Bucket buckets[kBuckets];
+  DevBuf frames, words, ctl, fk_tasks, hist;
+  int fk_blocks = 0;
+  long long total_items = 0;
+  Signature sig;
+  bool have_sig = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int clock_khz = 0;
+  std::string err;
+  ~gprm_batch() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto& b : buckets) {
+      if (b.ev0) (void)hipEventDestroy(b.ev0);
+      if (b.ev1) (void)hipEventDestroy(b.ev1);
+    }
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+thread_local std::string g_batch_error;
+
+int bfail(gprm_batch* b, int code, const std::string& msg) {
+  if (b) b->err = msg;
+  else g_batch_error = msg;
+  return code;
+}
+
+int bhip(gprm_batch* b, hipError_t e, const char* what) {
+  return bfail(b, GPRM_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define BHIP(b, expr, what)                              \
+  do {                                                   \
+    hipError_t e_ = (expr);                              \
+    if (e_ != hipSuccess) return bhip(b, e_, what);      \
+  } while (0)
+
+void drop_graph(gprm_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->have_sig = false;
+}
+
+// Control words: per scene (work counter, units done), then per scene the
+// completion clock (u64), then the start clock (u64).
+size_t ctl_words(int S) { return 2 * (size_t)S + 2 * (size_t)S + 2; }
+unsigned* ctl_counter(gprm_batch* b, int s) { return b->ctl.as<unsigned>() + 2 * s; }
+unsigned long long* ctl_tdone(gprm_batch* b, int s) {
+  return reinterpret_cast<unsigned long long*>(b->ctl.as<unsigned>() + 2 * b->ctx.size()) + s;
+}
+unsigned long long* ctl_tstart(gprm_batch* b) {
+  return reinterpret_cast<unsigned long long*>(b->ctl.as<unsigned>() + 4 * b->ctx.size());
+}
+
+// Task tables and buffers for one call signature (uploaded outside any capture).
+int prepare(gprm_batch* b, const Signature& sg) {
+  const int S = (int)b->ctx.size();
+  const int nd = b->ndof;
+  std::vector<long long> base(S + 1, 0);
+  for (int s = 0; s < S; ++s) base[s + 1] = base[s] + sg.n[s];
+  b->total_items = base[S];
+  BHIP(b, b->frames.ensure(sizeof(float) * 12 * nd * (size_t)std::max(1LL, base[S])), "hipMalloc frames");
+  BHIP(b, b->words.ensure(sizeof(int) * (size_t)std::max(1LL, base[S])), "hipMalloc words");
+  BHIP(b, b->ctl.ensure(sizeof(unsigned) * ctl_words(S)), "hipMalloc control words");
+  size_t hist = 0;
+  for (auto& bk : b->buckets)
+    if (!bk.scenes.empty()) hist = std::max(hist, (size_t)kMaxIter * bk.shape.grid * bk.shape.block);
+  BHIP(b, b->hist.ensure(sizeof(float) * 4 * std::max<size_t>(hist, 1)), "hipMalloc history");
+  std::vector<FkTask> fk(S);
+  int blocks = 0;
+  for (int s = 0; s < S; ++s) {
+    FkTask& t = fk[s];
+    t.rb = b->ctx[s]->rb;
+    t.q = static_cast<const float*>(sg.q[s]);
+    t.offset = sg.off[s];
+    t.n = sg.n[s];
+    t.block0 = blocks;
+    t.frames = b->frames.as<float>() + (size_t)12 * nd * base[s];
+    t.word = b->words.as<int>() + base[s];
+    t.codes = static_cast<int*>(const_cast<void*>(sg.res[s]));
+    blocks += (sg.n[s] + kFkBlock - 1) / kFkBlock;
+  }
+  b->fk_blocks = blocks;
+  BHIP(b, b->fk_tasks.ensure(sizeof(FkTask) * (size_t)S), "hipMalloc tasks");
+  BHIP(b, hipMemcpy(b->fk_tasks.p, fk.data(), sizeof(FkTask) * (size_t)S, hipMemcpyHostToDevice), "copy tasks");
+  for (auto& bk : b->buckets) {
+    if (bk.scenes.empty()) continue;
+    std::vector<SceneTask> st(bk.scenes.size());
+    for (size_t k = 0; k < bk.scenes.size(); ++k) {
+      const int s = bk.scenes[k];
+      const gprm_ctx* c = b->ctx[s];
+      SceneTask& t = st[k];
+      t.a = SvaArgs{};
+      t.a.img = c->img;
+      t.a.ndof = nd;
+      t.a.frames = fk[s].frames;
+      t.a.fstride = sg.n[s];
+      t.a.hist = b->hist.as<float4>();
+      t.a.nitems = sg.n[s];
+      t.a.counter = ctl_counter(b, s);
+      t.a.pairs_per_unit = bk.shape.ppu;
+      t.a.units_per_item = (c->img.M + bk.shape.ppu - 1) / bk.shape.ppu;
+      if ((long long)sg.n[s] * t.a.units_per_item >= (1LL << 31))
+        return bfail(b, GPRM_E_CAPACITY, "too many work units for scene " + std::to_string(s));
+      t.a.word = fk[s].word;
+      t.done = ctl_counter(b, s) + 1;
+      t.t_done = ctl_tdone(b, s);
+    }
+    BHIP(b, bk.tasks.ensure(sizeof(SceneTask) * st.size()), "hipMalloc tasks");
+    BHIP(b, hipMemcpy(bk.tasks.p, st.data(), sizeof(SceneTask) * st.size(), hipMemcpyHostToDevice), "copy tasks");
+  }
+  return GPRM_OK;
+}
+
+// The call's launch sequence on b->stream (capturable: kernels only).
+int enqueue(gprm_batch* b, bool events) {
+  const int S = (int)b->ctx.size();
+  hipStream_t st = b->stream;
+  BHIP(b, launch_fill_u32(b->ctl.as<unsigned>(), 0u, 4LL * S, st), "fill");
+  BHIP(b, launch_fill_u32(b->ctl.as<unsigned>() + 4 * S, 0xffffffffu, 2, st), "fill");
+  BHIP(b, launch_fk_points_multi(b->fk_tasks.as<FkTask>(), S, b->fk_blocks, b->ndof, st), "kinematics launch");
+  for (auto& bk : b->buckets) {
+    if (bk.scenes.empty()) continue;
+    if (events) BHIP(b, hipEventRecord(bk.ev0, st), "event");
+    BHIP(b, launch_sva_multi(bk.tasks.as<SceneTask>(), (int)bk.scenes.size(), 0, bk.shape, bk.lds,
+                             ctl_tstart(b), nullptr, st),
+         "separating-vector launch");
+    if (events) BHIP(b, hipEventRecord(bk.ev1, st), "event");
+  }
+  BHIP(b, launch_finalize_points_multi(b->fk_tasks.as<FkTask>(), S, b->fk_blocks, st), "finalize launch");
+  return GPRM_OK;
+}
+
+int run(gprm_batch* b, const Signature& sg, bool use_graph, bool events, float* done_ms) {
+  BHIP(b, hipSetDevice(b->device), "hipSetDevice");
+  const bool same = b->have_sig && b->sig == sg;
+  if (!same) {
+    drop_graph(b);
+    int rc = prepare(b, sg);
+    if (rc != GPRM_OK) return rc;
+    b->sig = sg;
+    b->have_sig = true;
+  }
+  if (use_graph && !events) {
+    if (!b->exec) {
+      BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+      int rc = enqueue(b, false);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(b->stream, &g);
+      if (rc != GPRM_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return bhip(b, e, "end capture");
+      b->graph = g;
+      BHIP(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "instantiate graph");
+    }
+    BHIP(b, hipGraphLaunch(b->exec, b->stream), "graph launch");
+  } else {
+    int rc = enqueue(b, events);
+    if (rc != GPRM_OK) return rc;
+  }
+  BHIP(b, hipStreamSynchronize(b->stream), "sync");
+  if (done_ms) {
+    const int S = (int)b->ctx.size();
+    std::vector<unsigned long long> td(S + 1);
+    BHIP(b, hipMemcpy(td.data(), ctl_tdone(b, 0), sizeof(unsigned long long) * (S + 1), hipMemcpyDeviceToHost),
+         "copy clocks");
+    const unsigned long long t0 = td[S];
+    for (int s = 0; s < S; ++s)
+      done_ms[s] = (sg.n[s] == 0 || td[s] == 0 || td[s] < t0) ? 0.0f
+                                                               : (float)((double)(td[s] - t0) / b->clock_khz);
+  }
+  return GPRM_OK;
+}
+
+int signature(gprm_batch* b, const float* const* d_q, const long long* offset, const int* n, int* const* d_res,
+              Signature* sg) {
+  const int S = (int)b->ctx.size();
+  if (!d_q || !offset || !n || !d_res) return bfail(b, GPRM_E_INVALID, "null argument");
+  sg->q.resize(S);
+  sg->res.resize(S);
+  sg->off.resize(S);
+  sg->n.resize(S);
+  for (int s = 0; s < S; ++s) {
+    if (n[s] < 0 || n[s] > kMaxBatchItems || offset[s] < n[s] || (n[s] > 0 && (!d_q[s] || !d_res[s])))
+      return bfail(b, GPRM_E_INVALID, "bad arguments for scene " + std::to_string(s));
+    sg->q[s] = d_q[s];
+    sg->res[s] = d_res[s];
+    sg->off[s] = offset[s];
+    sg->n[s] = n[s];
+  }
+  return GPRM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gprm_batch_create(gprm_ctx* const* ctxs, int nscenes, gprm_batch** out) {
+  if (!ctxs || nscenes < 1 || !out) return bfail(nullptr, GPRM_E_INVALID, "bad arguments");
+  *out = nullptr;
+  for (int s = 0; s < nscenes; ++s)
+    if (!ctxs[s] || !ctxs[s]->inited) return bfail(nullptr, GPRM_E_STATE, "scene context not initialised");
+  const int dev = ctxs[0]->device, nd = ctxs[0]->scene.ndof;
+  for (int s = 1; s < nscenes; ++s)
+    if (ctxs[s]->device != dev || ctxs[s]->scene.ndof != nd)
+      return bfail(nullptr, GPRM_E_INVALID, "scenes of a batch need the same device and ndof");
+  auto* b = new gprm_batch();
+  b->ctx.assign(ctxs, ctxs + nscenes);
+  b->device = dev;
+  b->ndof = nd;
+  hipError_t e = hipSetDevice(dev);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&b->clock_khz, hipDeviceAttributeWallClockRate, dev);
+  for (int i = 0; i < kBuckets && e == hipSuccess; ++i) {
+    Bucket& bk = b->buckets[i];
+    bk.lo = kBucketEdges[i];
+    bk.hi = kBucketEdges[i + 1];
+    e = hipEventCreate(&bk.ev0);
+    if (e == hipSuccess) e = hipEventCreate(&bk.ev1);
+  }
+  if (e != hipSuccess || b->clock_khz <= 0) {
+    int rc = bhip(nullptr, e != hipSuccess ? e : hipErrorInvalidValue, "batch set-up");
+    delete b;
+    return rc;
+  }
+  for (int s = 0; s < nscenes; ++s) {
+    const int nv = ctxs[s]->img.sumn;
+    for (auto& bk : b->buckets)
+      if (nv >= bk.lo && nv < bk.hi) {
+        bk.scenes.push_back(s);
+        bk.lds = std::max(bk.lds, sva_lds_bytes(ctxs[s]->img));
+      }
+  }
+  for (auto& bk : b->buckets) {
+    if (bk.scenes.empty()) continue;
+    // default: as many scene copies per CU as LDS holds (5, 4 or 1)
+    const size_t cap = (size_t)sva_max_lds_bytes() - 16;
+    if (5 * bk.lds <= cap) { bk.shape.block = 256; bk.shape.waves = 5; }
+    else if (4 * bk.lds <= cap) { bk.shape.block = 256; bk.shape.waves = 4; }
+    else { bk.shape.block = 1024; bk.shape.waves = 4; }
+    bk.shape.ppu = 32;
+    e = sva_multi_shape_finish(bk.lds, &bk.shape, dev);
+    if (e != hipSuccess) {
+      int rc = bhip(nullptr, e, "scene too large for the multi-scene kernel");
+      delete b;
+      return rc;
+    }
+  }
+  *out = b;
+  return GPRM_OK;
+}
+
+void gprm_batch_destroy(gprm_batch* b) {
+  if (b) (void)hipSetDevice(b->device);
+  delete b;
+}
+
+int gprm_batch_bucket_info(gprm_batch* b, int i, gprm_bucket_info* out) {
+  if (!b || !out || i < 0 || i >= kBuckets) return bfail(b, GPRM_E_INVALID, "bad arguments");
+  const Bucket& bk = b->buckets[i];
+  out->min_vertices = bk.lo;
+  out->max_vertices = bk.hi;
+  out->scenes = (int)bk.scenes.size();
+  out->lds_bytes = (int)bk.lds;
+  out->shape = gprm_launch_shape{bk.shape.block, bk.shape.waves, bk.shape.ppu, bk.shape.grid};
+  return GPRM_OK;
+}
+
+int gprm_batch_set_bucket_shape(gprm_batch* b, int i, const gprm_launch_shape* sh) {
+  if (!b || !sh || i < 0 || i >= kBuckets) return bfail(b, GPRM_E_INVALID, "bad arguments");
+  Bucket& bk = b->buckets[i];
+  if (bk.scenes.empty()) return GPRM_OK;
+  SvaShape s;
+  s.block = sh->block;
+  s.waves = sh->waves;
+  s.ppu = sh->pairs_per_unit;
+  BHIP(b, hipSetDevice(b->device), "hipSetDevice");
+  hipError_t e = sva_multi_shape_finish(bk.lds, &s, b->device);
+  if (e == hipErrorInvalidValue || e == hipErrorInvalidConfiguration)
+    return bfail(b, GPRM_E_INVALID, "launch shape not available for this bucket");
+  if (e != hipSuccess) return bhip(b, e, "launch shape");
+  bk.shape = s;
+  drop_graph(b);  // task tables and the graph depend on the shape
+  return GPRM_OK;
+}
+
+int gprm_batch_indicator_dev(gprm_batch* b, const float* const* d_q, const long long* offset, const int* n,
+                             int* const* d_res, int use_graph, float* done_ms) {
+  if (!b) return bfail(nullptr, GPRM_E_INVALID, "null batch");
+  Signature sg;
+  int rc = signature(b, d_q, offset, n, d_res, &sg);
+  if (rc != GPRM_OK) return rc;
+  return run(b, sg, use_graph != 0, false, done_ms);
+}
+
+int gprm_batch_autotune(gprm_batch* b, const float* const* d_q, const long long* offset, const int* n,
+                        int* const* d_res, int reps) {
+  if (!b) return bfail(nullptr, GPRM_E_INVALID, "null batch");
+  if (reps < 1) return bfail(b, GPRM_E_INVALID, "reps < 1");
+  Signature sg;
+  int rc = signature(b, d_q, offset, n, d_res, &sg);
+  if (rc != GPRM_OK) return rc;
+  const gprm_launch_shape variants[] = {{256, 5, 0, 0}, {256, 4, 0, 0}, {1024, 4, 0, 0}};
+  const int ppus[] = {16, 32, 64};
+  for (int i = 0; i < kBuckets; ++i) {
+    Bucket& bk = b->buckets[i];
+    if (bk.scenes.empty()) continue;
+    gprm_launch_shape best{bk.shape.block, bk.shape.waves, bk.shape.ppu, bk.shape.grid};
+    float best_ms = INFINITY;
+    for (const auto& v : variants)
+      for (int ppu : ppus) {
+        gprm_launch_shape sh = v;
+        sh.pairs_per_unit = ppu;
+        if (gprm_batch_set_bucket_shape(b, i, &sh) != GPRM_OK) continue;  // does not fit this bucket
+        float ms = INFINITY;
+        for (int r = 0; r <= reps; ++r) {  // the first run warms up
+          rc = run(b, sg, false, true, nullptr);
+          if (rc != GPRM_OK) return rc;
+          float t = 0.0f;
+          BHIP(b, hipEventElapsedTime(&t, bk.ev0, bk.ev1), "event time");
+          if (r > 0) ms = std::min(ms, t);
+        }
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = gprm_launch_shape{bk.shape.block, bk.shape.waves, bk.shape.ppu, bk.shape.grid};
+        }
+      }
+    rc = gprm_batch_set_bucket_shape(b, i, &best);
+    if (rc != GPRM_OK) return rc;
+  }
+  return GPRM_OK;
+}
+
+const char* gprm_batch_last_error(gprm_batch* b) { return b ? b->err.c_str() : g_batch_error.c_str(); }
+
+}  // extern "C"

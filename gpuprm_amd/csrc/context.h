@@ -1,0 +1,73 @@
+// context.h -- the state behind a gprm_ctx (one scene on one device), shared
+// by the translation units of the host runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gprm.h"
+#include "runtime.h"
+#include "scene.h"
+
+namespace gprm {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  hipError_t ensure(size_t b) {
+    if (b <= bytes) return hipSuccess;
+    if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+    hipError_t e = hipMalloc(&p, b);
+    if (e == hipSuccess) bytes = b;
+    return e;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Buffers of one in-flight request (the reference keeps one buffer set and
+// one stream per outstanding request, robotconfigspace.cpp:627-650).
+struct Slot {
+  hipStream_t stream = nullptr;
+  DevBuf frames, status, item_edge, item_pos, counters, qs, qe, res, testnum, stride, misc, tally, hist;
+  int* host_res = nullptr;
+  int N = 0;
+  bool busy = false;
+  // profiling: (start, end) event pairs of this call's launches; kind 0 = SVA, 1 = kinematics
+  std::vector<hipEvent_t> ev_start, ev_end;
+  std::vector<int> ev_kind;
+  size_t ev_used = 0;
+  bool tally_pending = false;
+  ~Slot() {
+    for (auto e : ev_start) (void)hipEventDestroy(e);
+    for (auto e : ev_end) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+}  // namespace gprm
+
+struct gprm_ctx {
+  gprm::Scene scene;
+  gprm::DevRobot rb{};
+  float dq = 0.01f;
+  int nbuf = 0;
+  int device = -1;
+  bool inited = false;
+  gprm::DevBuf img_buf;
+  gprm::SceneImg img{};
+  gprm::SvaShape shape;
+  std::vector<std::unique_ptr<gprm::Slot>> slots;
+  std::map<int, int> requests;  // request id -> slot index
+  int next_request = 0;
+  long long numthreads_all = 0;
+  bool profile = false;
+  gprm_profile_stats prof{};
+  std::string err;
+  std::mutex mu;
+};

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -19,6 +20,7 @@
 #include <vector>
 
 #include "../../include/gprm.h"
+#include "context.h"
 #include "runtime.h"
 #include "scene.h"
 
@@ -27,66 +29,12 @@ namespace {
 
 constexpr int kItemBudget = 1 << 23;   // configurations per kinematics/SVA chunk
 constexpr int kFirstWindow = 2;        // sample positions in the first edge round
-constexpr int kPairsPerUnit = 32;      // pairs a lane claims at once
 
 thread_local std::string g_create_error;
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  ~DevBuf() { if (p) (void)hipFree(p); }
-  hipError_t ensure(size_t b) {
-    if (b <= bytes) return hipSuccess;
-    if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
-    hipError_t e = hipMalloc(&p, b);
-    if (e == hipSuccess) bytes = b;
-    return e;
-  }
-  template <class T> T* as() const { return static_cast<T*>(p); }
-};
-
-// Buffers of one in-flight request (the reference keeps one buffer set and
-// one stream per outstanding request, robotconfigspace.cpp:627-650).
-struct Slot {
-  hipStream_t stream = nullptr;
-  DevBuf frames, status, item_edge, item_pos, counters, qs, qe, res, testnum, stride, misc, tally, hist;
-  int* host_res = nullptr;
-  int N = 0;
-  bool busy = false;
-  // profiling: (start, end) event pairs of this call's launches; kind 0 = SVA, 1 = kinematics
-  std::vector<hipEvent_t> ev_start, ev_end;
-  std::vector<int> ev_kind;
-  size_t ev_used = 0;
-  bool tally_pending = false;
-  ~Slot() {
-    for (auto e : ev_start) (void)hipEventDestroy(e);
-    for (auto e : ev_end) (void)hipEventDestroy(e);
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-};
 
 }  // namespace
 }  // namespace gprm
 
-struct gprm_ctx {
-  gprm::Scene scene;
-  gprm::DevRobot rb{};
-  float dq = 0.01f;
-  int nbuf = 0;
-  int device = -1;
-  bool inited = false;
-  gprm::DevBuf img_buf;
-  gprm::SceneImg img{};
-  int grid = 0;
-  std::vector<std::unique_ptr<gprm::Slot>> slots;
-  std::map<int, int> requests;  // request id -> slot index
-  int next_request = 0;
-  long long numthreads_all = 0;
-  bool profile = false;
-  gprm_profile_stats prof{};
-  std::string err;
-  std::mutex mu;
-};
 
 namespace gprm {
 namespace {
@@ -294,16 +242,24 @@ int prof_harvest(gprm_ctx* c, Slot& sl) {
   return GPRM_OK;
 }
 
+int units_per_item(const gprm_ctx* c) { return (c->img.M + c->shape.ppu - 1) / c->shape.ppu; }
+
+// Items of one launch: at most kItemBudget, and few enough that the unit
+// count items * units_per_item stays below 2^31.
+int item_budget(const gprm_ctx* c) {
+  return std::max(1, std::min(kItemBudget, 0x7fffffff / std::max(1, units_per_item(c))));
+}
+
 // Point checks of n items on the device (q SoA with offset, res[n]).
 int run_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n, int* d_res,
                hipStream_t st) {
   if (n <= 0) return GPRM_OK;
   const int nd = c->scene.ndof;
-  const int chunk = std::min(n, kItemBudget);
+  const int chunk = std::min(n, item_budget(c));
   GPRM_HIP(sl.frames.ensure(sizeof(float) * 12 * nd * (size_t)chunk), "hipMalloc frames");
   GPRM_HIP(sl.status.ensure(sizeof(int) * (size_t)chunk), "hipMalloc words");
   const int nchunks = (n + chunk - 1) / chunk;
-  GPRM_HIP(sl.hist.ensure(sizeof(float) * 4 * kMaxIter * (size_t)c->grid * sva_block_for(c->img)), "hipMalloc history");
+  GPRM_HIP(sl.hist.ensure(sizeof(float) * 4 * kMaxIter * (size_t)c->shape.grid * c->shape.block), "hipMalloc history");
   GPRM_HIP(sl.counters.ensure(sizeof(unsigned) * (size_t)nchunks), "hipMalloc counters");
   GPRM_HIP(launch_fill_u32(sl.counters.as<unsigned>(), 0u, nchunks, st), "fill");
   for (int ci = 0; ci < nchunks; ++ci) {
@@ -321,12 +277,12 @@ int run_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n,
     a.hist = sl.hist.as<float4>();
     a.nitems = cn;
     a.counter = sl.counters.as<unsigned>() + ci;
-    a.pairs_per_unit = kPairsPerUnit;
-    a.units_per_item = (c->img.M + kPairsPerUnit - 1) / kPairsPerUnit;
+    a.pairs_per_unit = c->shape.ppu;
+    a.units_per_item = units_per_item(c);
     a.word = sl.status.as<int>();
     a.tally = tally_ptr(c, sl);
     GPRM_HIP(prof_begin(c, sl, 0, st, &ev), "event");
-    GPRM_HIP(launch_sva(a, 0, c->grid, st), "separating-vector launch");
+    GPRM_HIP(launch_sva(a, 0, c->shape, st), "separating-vector launch");
     GPRM_HIP(prof_end(c, sl, ev, st), "event");
     GPRM_HIP(launch_finalize_points(sl.status.as<int>(), d_res + c0, cn, st), "finalize launch");
   }
@@ -379,7 +335,7 @@ int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long 
   std::vector<Launch> plan;
   for (auto& w : win) {
     int width = w.second - w.first;
-    int per = std::max(1, kItemBudget / width);
+    int per = std::max(1, item_budget(c) / width);
     for (int e0 = 0; e0 < N; e0 += per) plan.push_back({e0, std::min(N, e0 + per), w.first, w.second});
   }
   int cap = 0;
@@ -387,7 +343,7 @@ int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long 
   GPRM_HIP(sl.frames.ensure(sizeof(float) * 12 * nd * (size_t)std::max(cap, 1)), "hipMalloc frames");
   GPRM_HIP(sl.item_edge.ensure(sizeof(int) * (size_t)std::max(cap, 1)), "hipMalloc items");
   GPRM_HIP(sl.item_pos.ensure(sizeof(int) * (size_t)std::max(cap, 1)), "hipMalloc items");
-  GPRM_HIP(sl.hist.ensure(sizeof(float) * 4 * kMaxIter * (size_t)c->grid * sva_block_for(c->img)), "hipMalloc history");
+  GPRM_HIP(sl.hist.ensure(sizeof(float) * 4 * kMaxIter * (size_t)c->shape.grid * c->shape.block), "hipMalloc history");
   // per launch: [0] item count, [1] work counter
   GPRM_HIP(sl.counters.ensure(sizeof(int) * 2 * std::max<size_t>(plan.size(), 1)), "hipMalloc counters");
   GPRM_HIP(launch_fill_u32(sl.counters.as<unsigned>(), 0u, 2 * (long long)plan.size(), st), "fill");
@@ -414,11 +370,11 @@ int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long 
     a.counter = reinterpret_cast<unsigned*>(cnt + 1);
     a.item_edge = sl.item_edge.as<int>();
     a.edge_res = d_res;
-    a.pairs_per_unit = kPairsPerUnit;
-    a.units_per_item = (c->img.M + kPairsPerUnit - 1) / kPairsPerUnit;
+    a.pairs_per_unit = c->shape.ppu;
+    a.units_per_item = units_per_item(c);
     a.tally = tally_ptr(c, sl);
     GPRM_HIP(prof_begin(c, sl, 0, st, &ev), "event");
-    GPRM_HIP(launch_sva(a, 1, c->grid, st), "separating-vector launch");
+    GPRM_HIP(launch_sva(a, 1, c->shape, st), "separating-vector launch");
     GPRM_HIP(prof_end(c, sl, ev, st), "event");
   }
   if (stats) {
@@ -544,7 +500,8 @@ int gprm_init(gprm_ctx* c, int rank, int size) {
   GPRM_HIP(c->img_buf.ensure(image.size()), "hipMalloc scene");
   GPRM_HIP(hipMemcpy(c->img_buf.p, image.data(), image.size(), hipMemcpyHostToDevice), "copy scene");
   c->img.blob = c->img_buf.as<int4>();
-  c->grid = sva_grid(c->img, c->device);
+  c->shape = sva_default_shape(c->img);
+  GPRM_HIP(sva_shape_finish(c->img, &c->shape, c->device), "separating-vector launch shape");
   c->inited = true;
   return GPRM_OK;
 }
@@ -741,6 +698,74 @@ int gprm_profile_read(gprm_ctx* c, gprm_profile_stats* out) {
   if (!c || !out) return GPRM_E_INVALID;
   std::lock_guard<std::mutex> lk(c->mu);
   *out = c->prof;
+  return GPRM_OK;
+}
+
+int gprm_get_launch_shape(gprm_ctx* c, gprm_launch_shape* out) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!out) return fail(c, GPRM_E_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  *out = gprm_launch_shape{c->shape.block, c->shape.waves, c->shape.ppu, c->shape.grid};
+  return GPRM_OK;
+}
+
+int gprm_set_launch_shape(gprm_ctx* c, const gprm_launch_shape* in) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!in) return fail(c, GPRM_E_INVALID, "null argument");
+  SvaShape sh;
+  sh.block = in->block;
+  sh.waves = in->waves;
+  sh.ppu = in->pairs_per_unit;
+  hipError_t e = sva_shape_finish(c->img, &sh, c->device);
+  if (e == hipErrorInvalidValue || e == hipErrorInvalidConfiguration)
+    return fail(c, GPRM_E_INVALID, "launch shape not available for this scene");
+  if (e != hipSuccess) return hip_fail(c, e, "launch shape");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->shape = sh;
+  return GPRM_OK;
+}
+
+int gprm_autotune(gprm_ctx* c, const float* d_q, const float* d_qe, int n, long long offset, int reps,
+                  gprm_launch_shape* best, float* ms_best) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!d_q || n < 1 || offset < n || reps < 1 || !best) return fail(c, GPRM_E_INVALID, "bad arguments");
+  int* d_res = nullptr;
+  GPRM_HIP(hipMalloc(&d_res, sizeof(int) * (size_t)n), "hipMalloc results");
+  struct Free { int* p; ~Free() { (void)hipFree(p); } } fr{d_res};
+  const gprm_launch_shape variants[] = {{256, 5, 0, 0}, {256, 4, 0, 0}, {1024, 4, 0, 0}};
+  const int ppus[] = {16, 32, 64};
+  gprm_launch_shape keep{};
+  if ((rc = gprm_get_launch_shape(c, &keep)) != GPRM_OK) return rc;
+  double best_ms = 1e300;
+  gprm_launch_shape found = keep;
+  for (const auto& v : variants)
+    for (int ppu : ppus) {
+      gprm_launch_shape sh = v;
+      sh.pairs_per_unit = ppu;
+      if (gprm_set_launch_shape(c, &sh) != GPRM_OK) continue;  // not available for this scene
+      double ms = 1e300;
+      for (int r = 0; r <= reps; ++r) {  // the first run warms up
+        const auto t0 = std::chrono::steady_clock::now();
+        rc = d_qe ? gprm_indicator2_dev(c, d_q, d_qe, d_res, n, offset, nullptr, nullptr)
+                  : gprm_indicator_dev(c, d_q, d_res, n, offset, nullptr);
+        const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (rc != GPRM_OK) {
+          (void)gprm_set_launch_shape(c, &keep);
+          return rc;
+        }
+        if (r > 0) ms = std::min(ms, t);
+      }
+      if (ms < best_ms) {
+        best_ms = ms;
+        (void)gprm_get_launch_shape(c, &found);
+      }
+    }
+  if ((rc = gprm_set_launch_shape(c, &found)) != GPRM_OK) return rc;
+  *best = found;
+  if (ms_best) *ms_best = (float)best_ms;
   return GPRM_OK;
 }
 

@@ -465,29 +465,37 @@ __device__ __forceinline__ int sva_slow(float4* __restrict__ H, size_t hs, int& 
   return 0;
 }
 
-template <int EDGE, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) sva_kernel(SvaArgs A) {
-  extern __shared__ int4 smem[];
-  {
-    const int n16 = A.img.bytes >> 4;
-    for (int i = threadIdx.x; i < n16; i += BLOCK) smem[i] = A.img.blob[i];
-  }
+// Work counters of one lane of the separating-vector loop.
+struct SvaCounts {
+  unsigned sep = 0, col = 0, dropped = 0, units = 0, iter = 0, bound = 0;
+  unsigned claimed = 0;  // units this lane claimed (all finished when the loop returns)
+};
+
+// Copy a scene image into LDS (every thread of the block takes part; the
+// caller synchronises before reading it).
+template <int BLOCK>
+__device__ __forceinline__ Lds stage_scene(const SceneImg& img, int4* smem) {
+  const int n16 = img.bytes >> 4;
+  for (int i = threadIdx.x; i < n16; i += BLOCK) smem[i] = img.blob[i];
   const char* base = reinterpret_cast<const char*>(smem);
   Lds L;
-  L.verts = reinterpret_cast<const float4*>(base + A.img.off_verts);
-  L.pairs = reinterpret_cast<const int4*>(base + A.img.off_pairs);
-  L.adj = reinterpret_cast<const unsigned short*>(base + A.img.off_adj);
-  L.spheres = reinterpret_cast<const float4*>(base + A.img.off_spheres);
-  __syncthreads();
+  L.verts = reinterpret_cast<const float4*>(base + img.off_verts);
+  L.pairs = reinterpret_cast<const int4*>(base + img.off_pairs);
+  L.adj = reinterpret_cast<const unsigned short*>(base + img.off_adj);
+  L.spheres = reinterpret_cast<const float4*>(base + img.off_spheres);
+  return L;
+}
 
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
+// Units of one scene (its image in L) until its queue is exhausted and
+// every lane of the wave is idle.  H: this lane's iteration history (stride
+// hs).  Waves run independently: there is no block barrier inside.
+template <int EDGE>
+__device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, float4* __restrict__ H, size_t hs,
+                                          SvaCounts& cnt) {
+  const int lane = threadIdx.x & 63;
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
-  float4* __restrict__ H = A.hist + (size_t)blockIdx.x * BLOCK + tid;
-  const size_t hs = (size_t)gridDim.x * BLOCK;
   const int nitems = A.nitems >= 0 ? A.nitems : *A.nitems_dev;
   const int M = A.img.M;
-  const int ndof = A.ndof;
   const int PC = A.pairs_per_unit;
   const int nch = A.units_per_item;
   const unsigned nunits = (unsigned)nitems * (unsigned)nch;
@@ -507,7 +515,7 @@ __global__ void __launch_bounds__(BLOCK) sva_kernel(SvaArgs A) {
 
   unsigned q_base = 0, q_left = 0;
   bool exhausted = false;
-  unsigned n_sep = 0, n_col = 0, n_dropped = 0, n_units = 0, n_iter = 0, n_bound = 0;
+  unsigned n_sep = 0, n_col = 0, n_dropped = 0, n_units = 0, n_iter = 0, n_bound = 0, n_claimed = 0;
 
   for (;;) {
     // ---- refill idle lanes from the wave's queue of units -----------------
@@ -539,6 +547,7 @@ __global__ void __launch_bounds__(BLOCK) sva_kernel(SvaArgs A) {
         pend = min(M, pair + PC);
         need = false;
         need_init = true;
+        ++n_claimed;
         fsP = fsQ = ckQ = -1;
         // early exit: in edge mode once any sample of the edge is blocked; in
         // point mode for pairs after (in CRS order) a pair known to collide,
@@ -676,21 +685,135 @@ __global__ void __launch_bounds__(BLOCK) sva_kernel(SvaArgs A) {
     }
   }
 
-  if (A.tally != nullptr) {
-    // per-block sums through LDS (the scene is no longer read), one atomic per counter
+  cnt.sep += n_sep;
+  cnt.col += n_col;
+  cnt.dropped += n_dropped;
+  cnt.units += n_units;
+  cnt.iter += n_iter;
+  cnt.bound += n_bound;
+  cnt.claimed += n_claimed;
+}
+
+// Per-block sums of the lane counters through LDS, one atomic per counter.
+// Every thread of the block calls it; the LDS scene is not read afterwards.
+template <int BLOCK>
+__device__ __forceinline__ void add_tally(const SvaCounts& c, int4* smem, unsigned long long* tally) {
+  __syncthreads();
+  unsigned* t = reinterpret_cast<unsigned*>(smem);
+  const int tid = threadIdx.x;
+  if (tid < kTallyCount) t[tid] = 0u;
+  __syncthreads();
+  atomicAdd(&t[0], c.sep);
+  atomicAdd(&t[1], c.col);
+  atomicAdd(&t[2], c.dropped);
+  atomicAdd(&t[3], c.units);
+  atomicAdd(&t[4], c.iter);
+  atomicAdd(&t[5], c.bound);
+  __syncthreads();
+  if (tid < kTallyCount) atomicAdd(&tally[tid], (unsigned long long)t[tid]);
+}
+
+// WAVES: occupancy target in waves per SIMD; the compiler keeps the register
+// budget at 512/WAVES VGPRs (5 waves: 96 VGPRs, a few spilled dwords).
+template <int EDGE, int BLOCK, int WAVES>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) sva_kernel(SvaArgs A) {
+  extern __shared__ int4 smem[];
+  const Lds L = stage_scene<BLOCK>(A.img, smem);
+  __syncthreads();
+  float4* __restrict__ H = A.hist + (size_t)blockIdx.x * BLOCK + threadIdx.x;
+  const size_t hs = (size_t)gridDim.x * BLOCK;
+  SvaCounts c;
+  sva_units<EDGE>(A, L, H, hs, c);
+  if (A.tally != nullptr) add_tally<BLOCK>(c, smem, A.tally);
+}
+
+// Sum over the 64 lanes of a wave (every lane receives it).
+__device__ __forceinline__ unsigned wave_sum(unsigned v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Many scenes in one persistent launch.  Blocks start on different scenes
+// (block b on scene b mod ntasks) and visit every scene once in rotation:
+// a block whose next scene still has unclaimed units stages that scene's
+// image into LDS and runs its units; one that finds the queue drained skips
+// it.  When a wave leaves a scene, every unit it claimed there is finished;
+// the wave whose count completes the scene's total stamps the device clock.
+template <int EDGE, int BLOCK, int WAVES>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, 8)))
+    sva_multi_kernel(const SceneTask* __restrict__ tasks, int ntasks, unsigned long long* t_start,
+                     unsigned long long* tally) {
+  extern __shared__ int4 smem[];
+  __shared__ int s_live;
+  if (threadIdx.x == 0) atomicMin(t_start, (unsigned long long)wall_clock64());
+  float4* __restrict__ H = tasks[0].a.hist + (size_t)blockIdx.x * BLOCK + threadIdx.x;
+  const size_t hs = (size_t)gridDim.x * BLOCK;
+  SvaCounts c;
+  for (int r = 0; r < ntasks; ++r) {
+    const int t = (int)((blockIdx.x + (unsigned)r) % (unsigned)ntasks);
+    const SvaArgs A = tasks[t].a;
+    const int ni = A.nitems >= 0 ? A.nitems : *A.nitems_dev;
+    const unsigned nunits = (unsigned)ni * (unsigned)A.units_per_item;
+    if (threadIdx.x == 0)
+      s_live = __hip_atomic_load(A.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < nunits ? 1 : 0;
     __syncthreads();
-    unsigned* t = reinterpret_cast<unsigned*>(smem);
-    if (tid < kTallyCount) t[tid] = 0u;
-    __syncthreads();
-    atomicAdd(&t[0], n_sep);
-    atomicAdd(&t[1], n_col);
-    atomicAdd(&t[2], n_dropped);
-    atomicAdd(&t[3], n_units);
-    atomicAdd(&t[4], n_iter);
-    atomicAdd(&t[5], n_bound);
-    __syncthreads();
-    if (tid < kTallyCount) atomicAdd(&A.tally[tid], (unsigned long long)t[tid]);
+    if (s_live) {
+      const Lds L = stage_scene<BLOCK>(A.img, smem);
+      __syncthreads();
+      const unsigned before = c.claimed;
+      sva_units<EDGE>(A, L, H, hs, c);
+      const unsigned mine = wave_sum(c.claimed - before);
+      if ((threadIdx.x & 63) == 0 && mine > 0) {
+        const unsigned old = atomicAdd(tasks[t].done, mine);
+        if (old + mine == nunits) tasks[t].t_done[0] = (unsigned long long)wall_clock64();
+      }
+    }
+    __syncthreads();  // s_live and the LDS image are reused by the next scene
   }
+  if (tally != nullptr) add_tally<BLOCK>(c, smem, tally);
+}
+
+// Kinematics of the configurations of many scenes in one launch; block b
+// belongs to the task with the largest block0 <= b.
+template <int NDOF>
+__global__ void __launch_bounds__(kFkBlock) fk_points_multi_kernel(const FkTask* __restrict__ tasks, int ntasks) {
+  int lo = 0, hi = ntasks - 1;
+  const int b = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].block0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const FkTask& T = tasks[lo];
+  const int i = (b - T.block0) * kFkBlock + threadIdx.x;
+  if (i >= T.n) return;
+  const int nd = NDOF > 0 ? NDOF : T.rb.ndof;
+  float qq[kMaxDofDev];
+#pragma unroll
+  for (int j = 0; j < (NDOF > 0 ? NDOF : kMaxDofDev); ++j)
+    if (NDOF > 0 || j < nd) qq[j] = T.q[i + T.offset * j];
+  int st = 0;
+  for (int j = 0; j < nd; ++j)
+    if (qq[j] < T.rb.mins[j] || qq[j] > T.rb.maxs[j]) { st = 1; break; }
+  T.word[i] = st ? kWordOutOfBounds : kWordFree;
+  fk_store<NDOF>(T.rb, qq, T.frames, i, (long long)T.n);
+}
+
+// codes of every task from its result words (same block map).
+__global__ void finalize_points_multi_kernel(const FkTask* __restrict__ tasks, int ntasks) {
+  int lo = 0, hi = ntasks - 1;
+  const int b = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].block0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const FkTask& T = tasks[lo];
+  const int i = (b - T.block0) * kFkBlock + threadIdx.x;
+  if (i >= T.n) return;
+  const int w = T.word[i];
+  T.codes[i] = w == kWordFree ? 0 : (w & 15) - 1;
 }
 
 }  // namespace
@@ -702,24 +825,6 @@ __global__ void __launch_bounds__(BLOCK) sva_kernel(SvaArgs A) {
 size_t sva_lds_bytes(const SceneImg& img) { return (size_t)img.bytes; }
 
 int sva_max_lds_bytes() { return 160 * 1024; }
-
-int sva_block_for(const SceneImg& img) {
-  // four 256-thread blocks per CU when four scene copies fit in LDS, else one
-  // 1024-thread block sharing a single copy
-  return 4 * sva_lds_bytes(img) <= (size_t)sva_max_lds_bytes() ? 256 : 1024;
-}
-
-int sva_grid(const SceneImg& img, int device) {
-  hipDeviceProp_t prop;
-  int cus = 256;
-  if (hipGetDeviceProperties(&prop, device) == hipSuccess) cus = prop.multiProcessorCount;
-  const int block = sva_block_for(img);
-  int per_cu = (int)(sva_max_lds_bytes() / (sva_lds_bytes(img) ? sva_lds_bytes(img) : 1));
-  int by_waves = 32 / (block / 64);  // at most 8 waves per SIMD
-  if (per_cu > by_waves) per_cu = by_waves;
-  if (per_cu < 1) per_cu = 1;
-  return cus * per_cu;
-}
 
 #define GPRM_FK_DISPATCH(KERNEL, GRID, BLOCK, ST, ...)                                      \
   switch (rb.ndof) {                                                                        \
@@ -759,24 +864,121 @@ hipError_t launch_edge_fk(const DevRobot& rb, EdgeArgs ea, const int* item_edge,
   return hipGetLastError();
 }
 
-template <int EDGE, int BLOCK>
-static hipError_t launch_sva_t(const SvaArgs& a, int grid, size_t lds, hipStream_t st) {
-  if (lds > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&sva_kernel<EDGE, BLOCK>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
+template <int EDGE, int BLOCK, int WAVES>
+static const void* sva_fn() {
+  return reinterpret_cast<const void*>(&sva_kernel<EDGE, BLOCK, WAVES>);
+}
+
+// The kernel variants a shape may name: (block, waves per SIMD).
+static const void* sva_variant(int edge_mode, int block, int waves) {
+  if (block == 256 && waves == 5) return edge_mode ? sva_fn<1, 256, 5>() : sva_fn<0, 256, 5>();
+  if (block == 256 && waves == 4) return edge_mode ? sva_fn<1, 256, 4>() : sva_fn<0, 256, 4>();
+  if (block == 1024 && waves == 4) return edge_mode ? sva_fn<1, 1024, 4>() : sva_fn<0, 1024, 4>();
+  return nullptr;
+}
+
+template <int EDGE, int BLOCK, int WAVES>
+static const void* sva_multi_fn() {
+  return reinterpret_cast<const void*>(&sva_multi_kernel<EDGE, BLOCK, WAVES>);
+}
+
+static const void* sva_multi_variant(int edge_mode, int block, int waves) {
+  if (block == 256 && waves == 5) return edge_mode ? sva_multi_fn<1, 256, 5>() : sva_multi_fn<0, 256, 5>();
+  if (block == 256 && waves == 4) return edge_mode ? sva_multi_fn<1, 256, 4>() : sva_multi_fn<0, 256, 4>();
+  if (block == 1024 && waves == 4) return edge_mode ? sva_multi_fn<1, 1024, 4>() : sva_multi_fn<0, 1024, 4>();
+  return nullptr;
+}
+
+// Resident blocks over all CUs of a kernel variant with lds dynamic bytes.
+static hipError_t resident_grid(const void* const fns[2], int block, size_t lds, int device, int* grid) {
+  int cus = 0, blocks = 0;
+  hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  if (e != hipSuccess) return e;
+  // the occupancy query sees the dynamic LDS only once the attribute allows it
+  for (int em = 0; em < 2; ++em) {
+    if (!fns[em]) return hipErrorInvalidValue;
+    if (lds > 64 * 1024 &&
+        (e = hipFuncSetAttribute(fns[em], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)) != hipSuccess)
+      return e;
   }
-  hipLaunchKernelGGL((sva_kernel<EDGE, BLOCK>), dim3(grid), dim3(BLOCK), lds, st, a);
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fns[1], block, lds);
+  if (e != hipSuccess) return e;
+  if (blocks < 1) return hipErrorInvalidConfiguration;
+  *grid = cus * blocks;
+  return hipSuccess;
+}
+
+hipError_t sva_multi_shape_finish(size_t lds, SvaShape* sh, int device) {
+  if (lds > (size_t)sva_max_lds_bytes() - 16) return hipErrorInvalidValue;
+  if (sh->ppu < 1 || sh->ppu > 256) return hipErrorInvalidValue;
+  const void* fns[2] = {sva_multi_variant(0, sh->block, sh->waves), sva_multi_variant(1, sh->block, sh->waves)};
+  return resident_grid(fns, sh->block, lds, device, &sh->grid);
+}
+
+hipError_t sva_shape_finish(const SceneImg& img, SvaShape* sh, int device) {
+  const size_t lds = sva_lds_bytes(img);
+  if (lds > (size_t)sva_max_lds_bytes()) return hipErrorInvalidValue;
+  if (sh->ppu < 1 || sh->ppu > 256) return hipErrorInvalidValue;
+  const void* fns[2] = {sva_variant(0, sh->block, sh->waves), sva_variant(1, sh->block, sh->waves)};
+  return resident_grid(fns, sh->block, lds, device, &sh->grid);
+}
+
+SvaShape sva_default_shape(const SceneImg& img) {
+  SvaShape sh;
+  // five 256-thread blocks per CU when five scene copies fit in LDS, else
+  // one 1024-thread block sharing a single copy
+  if (5 * sva_lds_bytes(img) <= (size_t)sva_max_lds_bytes()) {
+    sh.block = 256;
+    sh.waves = 5;
+  } else if (4 * sva_lds_bytes(img) <= (size_t)sva_max_lds_bytes()) {
+    sh.block = 256;
+    sh.waves = 4;
+  } else {
+    sh.block = 1024;
+    sh.waves = 4;
+  }
+  sh.ppu = 32;
+  return sh;
+}
+
+hipError_t launch_sva(const SvaArgs& a, int edge_mode, const SvaShape& sh, hipStream_t st) {
+  const size_t lds = sva_lds_bytes(a.img);
+  if (lds > (size_t)sva_max_lds_bytes() || a.hist == nullptr || sh.grid < 1) return hipErrorInvalidValue;
+  const void* fn = sva_variant(edge_mode, sh.block, sh.waves);
+  if (!fn) return hipErrorInvalidValue;
+  void* args[] = {const_cast<SvaArgs*>(&a)};
+  return hipLaunchKernel(fn, dim3(sh.grid), dim3(sh.block), args, lds, st);
+}
+
+hipError_t launch_sva_multi(const SceneTask* d_tasks, int ntasks, int edge_mode, const SvaShape& sh, size_t lds,
+                            unsigned long long* t_start, unsigned long long* tally, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  if (lds > (size_t)sva_max_lds_bytes() - 16 || sh.grid < 1) return hipErrorInvalidValue;
+  const void* fn = sva_multi_variant(edge_mode, sh.block, sh.waves);
+  if (!fn) return hipErrorInvalidValue;
+  void* args[] = {const_cast<SceneTask**>(&d_tasks), &ntasks, &t_start, &tally};
+  return hipLaunchKernel(fn, dim3(sh.grid), dim3(sh.block), args, lds, st);
+}
+
+hipError_t launch_fk_points_multi(const FkTask* d_tasks, int ntasks, int nblocks, int ndof, hipStream_t st) {
+  if (ntasks <= 0 || nblocks <= 0) return hipSuccess;
+  dim3 grid(nblocks), block(kFkBlock);
+  switch (ndof) {
+    case 2: hipLaunchKernelGGL(fk_points_multi_kernel<2>, grid, block, 0, st, d_tasks, ntasks); break;
+    case 3: hipLaunchKernelGGL(fk_points_multi_kernel<3>, grid, block, 0, st, d_tasks, ntasks); break;
+    case 4: hipLaunchKernelGGL(fk_points_multi_kernel<4>, grid, block, 0, st, d_tasks, ntasks); break;
+    case 5: hipLaunchKernelGGL(fk_points_multi_kernel<5>, grid, block, 0, st, d_tasks, ntasks); break;
+    case 6: hipLaunchKernelGGL(fk_points_multi_kernel<6>, grid, block, 0, st, d_tasks, ntasks); break;
+    case 7: hipLaunchKernelGGL(fk_points_multi_kernel<7>, grid, block, 0, st, d_tasks, ntasks); break;
+    default: hipLaunchKernelGGL(fk_points_multi_kernel<0>, grid, block, 0, st, d_tasks, ntasks); break;
+  }
   return hipGetLastError();
 }
 
-hipError_t launch_sva(const SvaArgs& a, int edge_mode, int grid, hipStream_t st) {
-  size_t lds = sva_lds_bytes(a.img);
-  if (lds > (size_t)sva_max_lds_bytes() || a.hist == nullptr) return hipErrorInvalidValue;
-  const int block = sva_block_for(a.img);
-  if (block == 256)
-    return edge_mode ? launch_sva_t<1, 256>(a, grid, lds, st) : launch_sva_t<0, 256>(a, grid, lds, st);
-  return edge_mode ? launch_sva_t<1, 1024>(a, grid, lds, st) : launch_sva_t<0, 1024>(a, grid, lds, st);
+hipError_t launch_finalize_points_multi(const FkTask* d_tasks, int ntasks, int nblocks, hipStream_t st) {
+  if (ntasks <= 0 || nblocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(finalize_points_multi_kernel, dim3(nblocks), dim3(kFkBlock), 0, st, d_tasks, ntasks);
+  return hipGetLastError();
 }
 
 hipError_t launch_edge_prologue(const DevRobot& rb, float dq, EdgeArgs ea, int* testnum_out,

@@ -59,6 +59,25 @@ struct SvaArgs {
 };
 constexpr int kTallyCount = 6;
 
+// One scene of a multi-scene separating-vector launch (device memory).
+struct SceneTask {
+  SvaArgs a;                    // the scene's arguments; hist is shared by all tasks
+  unsigned* done;               // units finished (zero before launch)
+  unsigned long long* t_done;   // device clock (wall_clock64) when its last unit finished
+};
+
+// One scene of a multi-scene kinematics / finalize launch (device memory).
+struct FkTask {
+  DevRobot rb;
+  const float* q;               // SoA, component j of item i at q[i + offset*j]
+  long long offset;
+  int n;
+  int block0;                   // first block of this task (tasks sorted by block0)
+  float* frames;                // component-major, item stride n
+  int* word;                    // [n] result words
+  int* codes;                   // [n] codes written by the finalize kernel
+};
+
 // Edge work description.
 struct EdgeArgs {
   const float* qs; const float* qe; long long offset;  // SoA, component i of edge k at [k + offset*i]
@@ -70,13 +89,19 @@ struct EdgeArgs {
 
 size_t sva_lds_bytes(const SceneImg& img);
 int sva_max_lds_bytes();
-int sva_block_for(const SceneImg& img);
 
 // Kinematics of n configurations; frames component-major with item stride n.
 hipError_t launch_fk_points(const DevRobot& rb, const float* q, long long offset, int n, float* frames,
                             int* word, int check_bounds, hipStream_t st);
 hipError_t launch_finalize_points(const int* word, int* codes, int n, hipStream_t st);
-hipError_t launch_sva(const SvaArgs& a, int edge_mode, int grid, hipStream_t st);
+struct SvaShape;
+hipError_t launch_sva(const SvaArgs& a, int edge_mode, const SvaShape& sh, hipStream_t st);
+// Multi-scene launches (d_tasks in device memory).  The kinematics and
+// finalize grids have nblocks = sum over tasks of ceil(n / kFkBlock) blocks.
+hipError_t launch_fk_points_multi(const FkTask* d_tasks, int ntasks, int nblocks, int ndof, hipStream_t st);
+hipError_t launch_finalize_points_multi(const FkTask* d_tasks, int ntasks, int nblocks, hipStream_t st);
+hipError_t launch_sva_multi(const SceneTask* d_tasks, int ntasks, int edge_mode, const SvaShape& sh, size_t lds,
+                            unsigned long long* t_start, unsigned long long* tally, hipStream_t st);
 hipError_t launch_edge_prologue(const DevRobot& rb, float dq, EdgeArgs ea, int* testnum_out,
                                 int* stride_out, int* max_testnum, unsigned long long* total,
                                 hipStream_t st);
@@ -95,8 +120,22 @@ hipError_t launch_near(int ndof, const float* queries, int nq, long long qoff, c
                        hipStream_t st);
 hipError_t launch_fill_u32(unsigned int* p, unsigned int v, long long n, hipStream_t st);
 
-// Grid of the persistent separating-vector kernel: every CU filled as far as
-// LDS and the wave limit allow.
-int sva_grid(const SceneImg& img, int device);
+// Launch shape of the persistent separating-vector kernel: a kernel variant
+// (block size, occupancy target in waves per SIMD), the pairs of one work
+// unit, and the grid: every CU filled with as many blocks as are resident at
+// once (the runtime's occupancy for the variant and the scene's LDS).
+struct SvaShape {
+  int block = 256;
+  int waves = 5;
+  int ppu = 32;
+  int grid = 0;
+};
+// Variants: (256, 5), (256, 4), (1024, 4).  Fills in grid; an error for an
+// unknown variant, a scene beyond LDS or ppu outside [1, 256].
+hipError_t sva_shape_finish(const SceneImg& img, SvaShape* sh, int device);
+// The same for the multi-scene kernel with lds bytes of scene image.
+hipError_t sva_multi_shape_finish(size_t lds, SvaShape* sh, int device);
+// The shape used until a caller (or the autotuner) sets another.
+SvaShape sva_default_shape(const SceneImg& img);
 
 }  // namespace gprm

@@ -197,6 +197,61 @@ int gprm_surrogate_train_step(void* model, const float* d_q, long long offset, c
                               float* loss_out);
 int gprm_surrogate_predict(void* model, const float* d_q, int n, long long offset, float* d_prob, void* stream);
 
+/* Launch shape of the persistent separating-vector kernel (no reference
+ * counterpart: the reference launches one thread per configuration,
+ * robotconfigspace.cpp:718-723).  Variants (block, waves): (256, 5), (256, 4),
+ * (1024, 4); a variant is available when its blocks fit the scene in LDS. */
+typedef struct {
+  int block;            /* threads per block */
+  int waves;            /* occupancy target, waves per SIMD */
+  int pairs_per_unit;   /* pairs a lane claims at once, 1..256 */
+  int grid;             /* set by the library: resident blocks over all CUs */
+} gprm_launch_shape;
+int gprm_get_launch_shape(gprm_ctx* ctx, gprm_launch_shape* out);
+int gprm_set_launch_shape(gprm_ctx* ctx, const gprm_launch_shape* in);
+/* Autotuner: runs the given device workload (point checks of n
+ * configurations when d_qe is NULL, else n edges; SoA with offset >= n) with
+ * every available variant x pairs_per_unit in {16, 32, 64}, `reps` timed runs
+ * each after one warm-up, keeps the fastest shape and returns it with its best
+ * wall time in ms. */
+int gprm_autotune(gprm_ctx* ctx, const float* d_q, const float* d_qe, int n, long long offset, int reps,
+                  gprm_launch_shape* best, float* ms_best);
+
+/* Batches of scenes (no reference counterpart: the reference checks one scene
+ * per RobotConfigspace object and launch).  A batch runs the point checks of
+ * many initialised contexts (same device, same ndof; they must outlive the
+ * batch) as one sequence: one kinematics launch over all scenes, one
+ * persistent separating-vector launch per vertex-count bucket, one finalize
+ * launch.  Buckets: scenes with [0,512), [512,1024), [1024,2048) and >= 2048
+ * vertices in total, each with its own launch shape. */
+typedef struct gprm_batch gprm_batch;
+typedef struct {
+  int min_vertices, max_vertices;  /* the bucket's range of scene vertex counts */
+  int scenes;                      /* scenes of the batch in it */
+  int lds_bytes;                   /* largest scene image among them */
+  gprm_launch_shape shape;
+} gprm_bucket_info;
+int gprm_batch_create(gprm_ctx* const* ctxs, int nscenes, gprm_batch** out);
+void gprm_batch_destroy(gprm_batch* batch);
+int gprm_batch_bucket_info(gprm_batch* batch, int bucket, gprm_bucket_info* out);   /* bucket 0..3 */
+int gprm_batch_set_bucket_shape(gprm_batch* batch, int bucket, const gprm_launch_shape* shape);
+/* Scene s: n[s] (<= 2^22) configurations at d_q[s] (device, SoA with
+ * offset[s] >= n[s]) -> codes in d_res[s] (device), as gprm_indicator_dev
+ * computes them.  use_graph: the sequence is captured into a hipGraph the
+ * first time these arrays and sizes are seen and replayed while they stay the
+ * same.  done_ms (host, nscenes entries, may be NULL): when each scene's last
+ * check finished, in ms after the first separating-vector block started
+ * (device clock).  Returns when the results are complete. */
+int gprm_batch_indicator_dev(gprm_batch* batch, const float* const* d_q, const long long* offset, const int* n,
+                             int* const* d_res, int use_graph, float* done_ms);
+/* Autotuner: for every bucket in use, times its separating-vector launch
+ * (HIP events; reps runs after one warm-up, best kept) on the given workload
+ * with every available variant x pairs_per_unit in {16, 32, 64} and keeps
+ * the fastest shape. */
+int gprm_batch_autotune(gprm_batch* batch, const float* const* d_q, const long long* offset, const int* n,
+                        int* const* d_res, int reps);
+const char* gprm_batch_last_error(gprm_batch* batch);  /* NULL: error of the last create */
+
 /* Last error message of the context (or of creation when ctx is NULL). */
 const char* gprm_last_error(gprm_ctx* ctx);
 

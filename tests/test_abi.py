@@ -55,7 +55,8 @@ def test_python_structs_match_c_layout(tmp_path):
     """Every ctypes mirror of a header struct has the C size and field offsets."""
     import gpuprm_amd
     pairs = [("gprm_edge_stats", gpuprm_amd.EdgeStats), ("gprm_profile_stats", gpuprm_amd.ProfileStats),
-             ("gprm_prm_params", gpuprm_amd.PrmParams), ("gprm_prm_result", gpuprm_amd.PrmResult)]
+             ("gprm_prm_params", gpuprm_amd.PrmParams), ("gprm_prm_result", gpuprm_amd.PrmResult),
+             ("gprm_launch_shape", gpuprm_amd.LaunchShape), ("gprm_bucket_info", gpuprm_amd.BucketInfo)]
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gprm.h"', 'int main(void){']
     for cname, py in pairs:
         lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))

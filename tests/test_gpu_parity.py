@@ -175,3 +175,51 @@ def test_device_variants(workload_dir):
     assert st.samples_total == tot
     assert 0 < st.samples_scheduled <= tot
     assert np.array_equal(re.cpu().numpy() != 0, want != 0)
+
+
+@pytest.mark.parametrize("block,waves", [(256, 5), (256, 4), (1024, 4)])
+def test_launch_shapes_agree(workload_dir, block, waves):
+    """Every kernel variant and work-unit size gives the oracle's codes and
+    decisions (units of 1 and 7 pairs split the pair list at odd places)."""
+    import torch
+    d = workload_dir(2)
+    o = OracleScene(d)
+    s = _space(d)
+    rng = np.random.default_rng(block + waves)
+    Q = rng.uniform(-np.pi, np.pi, size=(6, 3000)).astype(np.float32)
+    want = o.indicator_batch(Q, nthreads=8)
+    V = rng.uniform(-np.pi, np.pi, size=(6, 200)).astype(np.float32)
+    QS, QE = _knn_edges(V, 10, rng, 500)
+    wante, _ = o.indicator2(QS, QE, nthreads=8)
+    q = torch.from_numpy(Q).cuda()
+    qs, qe = torch.from_numpy(QS).cuda(), torch.from_numpy(QE).cuda()
+    for ppu in (1, 7, 16, 64):
+        sh = s.set_launch_shape(block, waves, ppu)
+        assert (sh.block, sh.waves, sh.pairs_per_unit) == (block, waves, ppu) and sh.grid >= 256
+        r = torch.zeros(Q.shape[1], dtype=torch.int32, device="cuda")
+        s.indicator_dev(q, r)
+        assert np.array_equal(r.cpu().numpy(), want), ppu
+        re = torch.zeros(QS.shape[1], dtype=torch.int32, device="cuda")
+        s.indicator2_dev(qs, qe, re)
+        assert np.array_equal(re.cpu().numpy() != 0, wante != 0), ppu
+
+
+def test_autotune_keeps_results(workload_dir):
+    import gpuprm_amd
+    import torch
+    d = workload_dir(2)
+    o = OracleScene(d)
+    s = _space(d)
+    rng = np.random.default_rng(5)
+    Q = rng.uniform(-np.pi, np.pi, size=(6, 20000)).astype(np.float32)
+    q = torch.from_numpy(Q).cuda()
+    sh, ms = s.autotune(q, reps=2)
+    assert sh.pairs_per_unit in (16, 32, 64) and (sh.block, sh.waves) in ((256, 5), (256, 4), (1024, 4))
+    assert ms > 0 and s.get_launch_shape().as_dict() == sh.as_dict()
+    r = torch.zeros(Q.shape[1], dtype=torch.int32, device="cuda")
+    s.indicator_dev(q, r)
+    assert np.array_equal(r.cpu().numpy(), o.indicator_batch(Q, nthreads=8))
+    with pytest.raises(gpuprm_amd.GprmError):
+        s.set_launch_shape(512, 4, 32)
+    with pytest.raises(gpuprm_amd.GprmError):
+        s.set_launch_shape(256, 5, 0)
