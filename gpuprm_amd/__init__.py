@@ -19,7 +19,7 @@ import numpy as np
 
 __all__ = ["RobotConfigspace", "Surrogate", "GprmError", "EdgeStats", "ProfileStats", "PrmParams", "PrmResult", "lib",
            "generate_scene", "LIB_PATH", "LaunchShape",
-           "BucketInfo", "SceneBatch"]
+           "BucketInfo", "SceneBatch", "radius_neighbours"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgprm.so")
 
@@ -197,6 +197,8 @@ def lib():
     L.gprm_get_launch_shape.argtypes = [vp, C.POINTER(LaunchShape)]
     L.gprm_set_launch_shape.argtypes = [vp, C.POINTER(LaunchShape)]
     L.gprm_autotune.argtypes = [vp, vp, vp, C.c_int, C.c_longlong, C.c_int, C.POINTER(LaunchShape), _FP]
+    L.gprm_radius_neighbours.argtypes = [C.c_int, vp, C.c_int, C.c_longlong, vp, C.c_int, C.c_longlong, _FP, _FP,
+                                         C.c_float, vp, vp, vp, C.c_int, C.POINTER(C.c_longlong), vp]
     PP = C.POINTER(vp)
     LL = C.POINTER(C.c_longlong)
     L.gprm_batch_create.argtypes = [PP, C.c_int, PP]
@@ -217,6 +219,31 @@ def _check(rc: int, ctx=None) -> int:
         msg = lib().gprm_last_error(ctx)
         raise GprmError(rc, msg.decode() if msg else "")
     return rc
+
+
+def radius_neighbours(nodes, queries, mins, maxs, D, cap=None):
+    """GPU radius search (gprm_radius_neighbours): nodes (ndof, n) and queries
+    (ndof, nq) float32 device tensors, contiguous.  Returns (query, node, d2)
+    numpy arrays of every pair closer than D, sorted by (query, node)."""
+    import torch
+    nd, n = int(nodes.shape[0]), int(nodes.shape[1])
+    nq = int(queries.shape[1])
+    cap = int(cap if cap is not None else max(1, n * 8))
+    oq = torch.empty(cap, dtype=torch.int32, device=nodes.device)
+    on = torch.empty(cap, dtype=torch.int32, device=nodes.device)
+    od = torch.empty(cap, dtype=torch.float32, device=nodes.device)
+    tot = C.c_longlong(0)
+    rc = lib().gprm_radius_neighbours(nd, C.c_void_p(nodes.data_ptr()), n, max(1, n), C.c_void_p(queries.data_ptr()),
+                                      nq, max(1, nq), _f32(mins).ctypes.data_as(_FP), _f32(maxs).ctypes.data_as(_FP),
+                                      float(D), C.c_void_p(oq.data_ptr()), C.c_void_p(on.data_ptr()),
+                                      C.c_void_p(od.data_ptr()), cap, C.byref(tot), None)
+    _check(rc)
+    k = min(cap, tot.value)
+    if tot.value > cap:
+        raise GprmError(-4, "%d pairs exceed cap %d" % (tot.value, cap))
+    q, nn, d = oq[:k].cpu().numpy(), on[:k].cpu().numpy(), od[:k].cpu().numpy()
+    o = np.lexsort((nn, q))
+    return q[o], nn[o], d[o]
 
 
 class SceneBatch:

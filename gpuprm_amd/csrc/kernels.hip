@@ -16,6 +16,8 @@
 // direction so NaN behaves as it does there.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include "glibc_trig.h"
 #include "runtime.h"
 
@@ -321,6 +323,136 @@ __global__ void __launch_bounds__(256)
       if (o < cap) {
         out_q[o] = j;
         out_node[o] = l;
+        out_d2[o] = normsq;
+      }
+    }
+  }
+}
+
+// ---- cell index for the radius search -------------------------------------
+// Nodes are binned by their first g.dims coordinates into cells at least D
+// wide, so every node within D of a query lies in the 3^dims cells around the
+// query's own cell.  The index is a counting sort: per-cell counts, their
+// exclusive scan, and a scatter of the coordinates (SoA, stride n) and node
+// ids into cell order.
+
+__device__ __forceinline__ int grid_coord(const CellGrid& g, int d, float x) {
+  int c = (int)floorf((x - g.lo[d]) * g.inv[d]);
+  return c < 0 ? 0 : (c >= g.cells[d] ? g.cells[d] - 1 : c);
+}
+
+__global__ void grid_count_kernel(CellGrid g, const float* __restrict__ nodes, int n, long long noff,
+                                  int* __restrict__ cell_id, int* __restrict__ counts) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n) return;
+  int c = 0;
+  for (int d = g.dims - 1; d >= 0; --d) c = c * g.cells[d] + grid_coord(g, d, nodes[l + noff * d]);
+  cell_id[l] = c;
+  atomicAdd(&counts[c], 1);
+}
+
+// Exclusive scan of counts[0, ncells) into starts and cursor (one block of 1024).
+__global__ void __launch_bounds__(1024) grid_scan_kernel(const int* __restrict__ counts, int* __restrict__ starts,
+                                                         int* __restrict__ cursor, int ncells) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int per = (ncells + 1023) / 1024;
+  const int c0 = min(ncells, t * per), c1 = min(ncells, c0 + per);
+  int sum = 0;
+  for (int c = c0; c < c1; ++c) sum += counts[c];
+  part[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan of the partial sums
+    const int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - sum;
+  for (int c = c0; c < c1; ++c) {
+    starts[c] = run;
+    cursor[c] = run;
+    run += counts[c];
+  }
+}
+
+__global__ void grid_scatter_kernel(int ndof, const float* __restrict__ nodes, int n, long long noff,
+                                    const int* __restrict__ cell_id, int* __restrict__ cursor,
+                                    float* __restrict__ sorted, int* __restrict__ sid) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n) return;
+  const int pos = atomicAdd(&cursor[cell_id[l]], 1);
+  for (int i = 0; i < ndof; ++i) sorted[pos + (size_t)n * i] = nodes[l + noff * i];
+  sid[pos] = l;
+}
+
+// Radius search over the cell index: one block per query; the nodes of the
+// (up to 27) neighbouring cells form one index range that the block's
+// threads stride through.  Same squared distance (same summation order) and
+// same append as near_kernel.
+__global__ void __launch_bounds__(256)
+    near_grid_kernel(int ndof, const float* __restrict__ queries, int nq, long long qoff, CellGrid g,
+                     const int* __restrict__ starts, const int* __restrict__ counts, const float* __restrict__ sorted,
+                     const int* __restrict__ sid, int n, float D2, int* __restrict__ out_q,
+                     int* __restrict__ out_node, float* __restrict__ out_d2, int* __restrict__ count, int cap) {
+  __shared__ int s_start[27], s_pre[28];
+  const int j = blockIdx.x;
+  if (j >= nq) return;
+  float qr[kMaxDofDev];
+  for (int i = 0; i < ndof; ++i) qr[i] = queries[j + qoff * i];
+  if (threadIdx.x < 27) {
+    // neighbour k: offset digit[d] - 1 in grid dimension d; a dimension
+    // without grid (ndof < 3) takes only offset 0
+    const int k = threadIdx.x;
+    const int digit[3] = {k % 3, (k / 3) % 3, k / 9};
+    int c = 0;
+    bool ok = true;
+    for (int d = 2; d >= 0; --d) {
+      if (d >= g.dims) {
+        if (digit[d] != 1) ok = false;
+        continue;
+      }
+      const int x = grid_coord(g, d, qr[d]) + digit[d] - 1;
+      if (x < 0 || x >= g.cells[d]) ok = false;
+      c = c * g.cells[d] + x;
+    }
+    s_start[threadIdx.x] = ok ? starts[c] : 0;
+    s_pre[threadIdx.x + 1] = ok ? counts[c] : 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    s_pre[0] = 0;
+    for (int k = 0; k < 27; ++k) s_pre[k + 1] += s_pre[k];
+  }
+  __syncthreads();
+  const int total = s_pre[27];
+  const int lane = threadIdx.x & 63;
+  for (int base = 0; base < total; base += blockDim.x) {
+    const int idx = base + threadIdx.x;
+    float normsq = 0.0f;
+    bool hit = false;
+    int node = 0;
+    if (idx < total) {
+      int k = 0;
+      while (s_pre[k + 1] <= idx) ++k;
+      const int pos = s_start[k] + (idx - s_pre[k]);
+      for (int i = 0; i < ndof; ++i) {
+        float diff = sorted[pos + (size_t)n * i] - qr[i];
+        normsq += diff * diff;
+      }
+      hit = normsq < D2;
+      node = sid[pos];
+    }
+    unsigned long long m = __ballot(hit);
+    if (m == 0ull) continue;
+    int b = 0;
+    if (lane == 0) b = atomicAdd(count, __popcll(m));
+    b = __shfl(b, 0, 64);
+    if (hit) {
+      int o = b + __popcll(m & ((1ull << lane) - 1ull));
+      if (o < cap) {
+        out_q[o] = j;
+        out_node[o] = node;
         out_d2[o] = normsq;
       }
     }
@@ -1005,6 +1137,51 @@ hipError_t launch_near(int ndof, const float* queries, int nq, long long qoff, c
   if (nq <= 0 || nn <= 0) return hipSuccess;
   hipLaunchKernelGGL(near_kernel, dim3(nq), dim3(256), 0, st, ndof, queries, nq, qoff, nodes, nn, noff, D2, out_q,
                      out_node, out_d2, count, cap);
+  return hipGetLastError();
+}
+
+CellGrid make_cell_grid(int ndof, const float* mins, const float* maxs, float D) {
+  CellGrid g{};
+  g.dims = ndof < 3 ? ndof : 3;
+  g.ncells = 1;
+  for (int d = 0; d < 3; ++d) {
+    g.cells[d] = 1;
+    g.lo[d] = 0.0f;
+    g.inv[d] = 0.0f;
+    if (d >= g.dims) continue;
+    const float span = maxs[d] - mins[d];
+    int c = (span > 0.0f && D > 0.0f) ? (int)std::floor(span / D) : 1;
+    c = c < 1 ? 1 : (c > 64 ? 64 : c);
+    g.cells[d] = c;
+    g.lo[d] = mins[d];
+    g.inv[d] = span > 0.0f ? (float)c / span : 0.0f;
+    // a cell must stay at least D wide after rounding of inv
+    while (g.cells[d] > 1 && (float)(1.0 / g.inv[d]) < D) {
+      g.cells[d] -= 1;
+      g.inv[d] = (float)g.cells[d] / span;
+    }
+    g.ncells *= g.cells[d];
+  }
+  return g;
+}
+
+hipError_t launch_grid_build(const CellGrid& g, int ndof, const float* nodes, int n, long long noff, int* cell_id,
+                             int* counts, int* starts, int* cursor, float* sorted, int* sid, hipStream_t st) {
+  hipError_t e = launch_fill_i32(counts, 0, g.ncells, st);
+  if (e != hipSuccess || n <= 0) return e;
+  const dim3 grid((n + 255) / 256), block(256);
+  hipLaunchKernelGGL(grid_count_kernel, grid, block, 0, st, g, nodes, n, noff, cell_id, counts);
+  hipLaunchKernelGGL(grid_scan_kernel, dim3(1), dim3(1024), 0, st, counts, starts, cursor, g.ncells);
+  hipLaunchKernelGGL(grid_scatter_kernel, grid, block, 0, st, ndof, nodes, n, noff, cell_id, cursor, sorted, sid);
+  return hipGetLastError();
+}
+
+hipError_t launch_near_grid(int ndof, const float* queries, int nq, long long qoff, const CellGrid& g,
+                            const int* starts, const int* counts, const float* sorted, const int* sid, int n, float D2,
+                            int* out_q, int* out_node, float* out_d2, int* count, int cap, hipStream_t st) {
+  if (nq <= 0 || n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(near_grid_kernel, dim3(nq), dim3(256), 0, st, ndof, queries, nq, qoff, g, starts, counts, sorted,
+                     sid, n, D2, out_q, out_node, out_d2, count, cap);
   return hipGetLastError();
 }
 

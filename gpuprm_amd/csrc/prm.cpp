@@ -11,7 +11,8 @@
 //      of one host call each;
 //   2. list every stored node of either tree closer than D (L2) to each new
 //      node (find_neighbours / get_near_vertices, :2532-2574, :2701-2748) with
-//      a GPU radius search over all nodes instead of the q[0] block index;
+//      a GPU radius search over a cell index of the first three joint
+//      coordinates, in place of the reference's block index over q[0];
 //   3. check the candidate edges with the exact edge check (indicator2): all
 //      of them, as the reference does, or at most edge_budget per (new node,
 //      tree), ranked by the collision-probability model when one is given and
@@ -76,6 +77,8 @@ struct Tree {
   std::vector<std::vector<std::pair<int, float>>> adj;
   DevArr dq;                            // SoA ndof x cap on the device
   std::vector<float> stage;             // host staging of one upload
+  // cell index of the device nodes for the radius search (rebuilt on upload)
+  DevArr cell_id, counts, starts, cursor, sorted, sid;
 };
 
 int insert(Tree& t, const float* q) {
@@ -102,6 +105,17 @@ bool flush(Tree& t, hipStream_t st) {
       return false;
   t.ndev = t.n;
   return true;
+}
+
+// Rebuild the tree's cell index over its nodes [0, ndev).
+bool index_nodes(Tree& t, const CellGrid& g, hipStream_t st) {
+  if (!t.counts.ensure(sizeof(int) * (size_t)g.ncells) || !t.starts.ensure(sizeof(int) * (size_t)g.ncells) ||
+      !t.cursor.ensure(sizeof(int) * (size_t)g.ncells) || !t.cell_id.ensure(sizeof(int) * (size_t)t.cap) ||
+      !t.sid.ensure(sizeof(int) * (size_t)t.cap) || !t.sorted.ensure(sizeof(float) * t.ndof * (size_t)t.cap))
+    return false;
+  return launch_grid_build(g, t.ndof, t.dq.as<float>(), t.ndev, t.cap, t.cell_id.as<int>(), t.counts.as<int>(),
+                           t.starts.as<int>(), t.cursor.as<int>(), t.sorted.as<float>(), t.sid.as<int>(),
+                           st) == hipSuccess;
 }
 
 // Dijkstra on one tree (do_dijkstra, prmsolver.cpp:2804-2846).
@@ -187,7 +201,14 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
   if (c0 != 0 || c1 != 0) return GPRM_OK;  // found = 0
   insert(T[0], qstart);
   insert(T[1], qgoal);
-  if (!flush(T[0], st) || !flush(T[1], st)) return GPRM_E_HIP;
+  std::vector<float> lo(nd), hi(nd);
+  for (int i = 0; i < nd; ++i) {
+    lo[i] = gprm_min(ctx, i);
+    hi[i] = gprm_max(ctx, i);
+  }
+  const CellGrid grid = make_cell_grid(nd, lo.data(), hi.data(), prm->D);
+  for (int g = 0; g < 2; ++g)
+    if (!flush(T[g], st) || !index_nodes(T[g], grid, st)) return GPRM_E_HIP;
 
   Rng rng(prm->seed * 0x9E3779B97F4A7C15ULL + 12345);
   const int num = prm->num;
@@ -295,8 +316,9 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
     for (int g = 0; g < 2; ++g) {
       int zero[2] = {0, 0};
       if (hipMemcpyAsync(d_cnt.p, zero, 8, hipMemcpyHostToDevice, st) != hipSuccess) return GPRM_E_HIP;
-      if (launch_near(nd, d_new.as<float>(), num, num, T[g].dq.as<float>(), T[g].n, T[g].cap, D2, d_nq.as<int>(),
-                      d_nn.as<int>(), d_nd2.as<float>(), d_cnt.as<int>(), prm->nbuf, st) != hipSuccess)
+      if (launch_near_grid(nd, d_new.as<float>(), num, num, grid, T[g].starts.as<int>(), T[g].counts.as<int>(),
+                           T[g].sorted.as<float>(), T[g].sid.as<int>(), T[g].ndev, D2, d_nq.as<int>(), d_nn.as<int>(),
+                           d_nd2.as<float>(), d_cnt.as<int>(), prm->nbuf, st) != hipSuccess)
         return GPRM_E_HIP;
       int cnt = 0;
       if (hipMemcpyAsync(&cnt, d_cnt.p, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -443,7 +465,8 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
       }
       if ((pos[0] < 0 && T[0].n >= T[0].cap) || (pos[1] < 0 && T[1].n >= T[1].cap)) break;
     }
-    if (!flush(T[0], st) || !flush(T[1], st)) return GPRM_E_HIP;
+    for (int g = 0; g < 2; ++g)
+      if (!flush(T[g], st) || !index_nodes(T[g], grid, st)) return GPRM_E_HIP;
     res->ms_host += ms_since(th);
     if (T[0].n >= T[0].cap || T[1].n >= T[1].cap) break;
   }
@@ -466,5 +489,40 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
   }
   if (hipStreamSynchronize(st) != hipSuccess) return GPRM_E_HIP;
   res->ms_total = ms_since(t0);
+  return GPRM_OK;
+}
+
+// Radius neighbours on the device (get_near_vertices, prmsolver.cpp:2532-2574,
+// over all nodes instead of the q[0] blocks): every (query j, node l) with
+// |node_l - query_j|^2 < D^2, summed over the coordinates in order.
+extern "C" int gprm_radius_neighbours(int ndof, const float* d_nodes, int n, long long noff, const float* d_queries,
+                                      int nq, long long qoff, const float* mins, const float* maxs, float D,
+                                      int* d_out_q, int* d_out_node, float* d_out_d2, int cap, long long* total,
+                                      void* stream) {
+  if (ndof < 1 || ndof > kMaxDofDev || n < 0 || nq < 0 || cap < 0 || !(D > 0.0f) || !mins || !maxs || !total ||
+      (n > 0 && (!d_nodes || noff < n)) || (nq > 0 && (!d_queries || qoff < nq)) ||
+      (cap > 0 && (!d_out_q || !d_out_node || !d_out_d2)))
+    return GPRM_E_INVALID;
+  for (int i = 0; i < ndof; ++i)
+    if (!(maxs[i] >= mins[i])) return GPRM_E_INVALID;
+  *total = 0;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const CellGrid g = make_cell_grid(ndof, mins, maxs, D);
+  DevArr cell_id, counts, starts, cursor, sorted, sid, cnt;
+  if (!counts.ensure(sizeof(int) * (size_t)g.ncells) || !starts.ensure(sizeof(int) * (size_t)g.ncells) ||
+      !cursor.ensure(sizeof(int) * (size_t)g.ncells) || !cell_id.ensure(sizeof(int) * (size_t)std::max(n, 1)) ||
+      !sid.ensure(sizeof(int) * (size_t)std::max(n, 1)) ||
+      !sorted.ensure(sizeof(float) * ndof * (size_t)std::max(n, 1)) || !cnt.ensure(sizeof(int)))
+    return GPRM_E_HIP;
+  int h = 0;
+  if (launch_fill_i32(cnt.as<int>(), 0, 1, st) != hipSuccess ||
+      launch_grid_build(g, ndof, d_nodes, n, noff, cell_id.as<int>(), counts.as<int>(), starts.as<int>(),
+                        cursor.as<int>(), sorted.as<float>(), sid.as<int>(), st) != hipSuccess ||
+      launch_near_grid(ndof, d_queries, nq, qoff, g, starts.as<int>(), counts.as<int>(), sorted.as<float>(),
+                       sid.as<int>(), n, D * D, d_out_q, d_out_node, d_out_d2, cnt.as<int>(), cap, st) != hipSuccess ||
+      hipMemcpyAsync(&h, cnt.p, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return GPRM_E_HIP;
+  *total = h;
   return GPRM_OK;
 }

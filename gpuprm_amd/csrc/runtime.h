@@ -78,6 +78,15 @@ struct FkTask {
   int* codes;                   // [n] codes written by the finalize kernel
 };
 
+// Cell index of the radius search: nodes binned by their first `dims`
+// coordinates (cells at least D wide, at most 64 per dimension).
+struct CellGrid {
+  int dims;
+  int cells[3];
+  float lo[3], inv[3];   // cell = clamp(floor((x - lo) * inv), 0, cells - 1)
+  int ncells;
+};
+
 // Edge work description.
 struct EdgeArgs {
   const float* qs; const float* qe; long long offset;  // SoA, component i of edge k at [k + offset*i]
@@ -112,6 +121,16 @@ hipError_t launch_edge_compact(EdgeArgs ea, int e0, int e1, int lo, int hi, int*
 hipError_t launch_edge_fk(const DevRobot& rb, EdgeArgs ea, const int* item_edge, const int* item_pos,
                           const int* nitems, int cap, float* frames, hipStream_t st);
 hipError_t launch_fill_i32(int* p, int v, long long n, hipStream_t st);
+CellGrid make_cell_grid(int ndof, const float* mins, const float* maxs, float D);
+// Counting sort of n nodes (SoA, stride noff) into the cell index: counts and
+// starts per cell (g.ncells), cursor scratch (g.ncells), cell_id scratch (n),
+// sorted coordinates (SoA, stride n) and node ids (n).
+hipError_t launch_grid_build(const CellGrid& g, int ndof, const float* nodes, int n, long long noff, int* cell_id,
+                             int* counts, int* starts, int* cursor, float* sorted, int* sid, hipStream_t st);
+// launch_near over the cell index (same pairs; append order may differ).
+hipError_t launch_near_grid(int ndof, const float* queries, int nq, long long qoff, const CellGrid& g,
+                            const int* starts, const int* counts, const float* sorted, const int* sid, int n, float D2,
+                            int* out_q, int* out_node, float* out_d2, int* count, int cap, hipStream_t st);
 // Radius neighbours: every (query j, node l) with |node_l - query_j|^2 < D2,
 // appended to (out_q, out_node, out_d2) up to cap entries; *count receives the
 // total number of matches (may exceed cap).

@@ -180,6 +180,18 @@ typedef struct {
 int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* qgoal, const gprm_prm_params* params,
                    gprm_prm_result* res, float* path, int path_cap);
 
+/* Radius neighbours on the device (the planner's get_near_vertices,
+ * prm/src/lib/prm/prmsolver.cpp:2532-2574): every pair (query j, node l) with
+ * sum_i (node_l[i] - query_j[i])^2 < D^2, the sum taken in float over i in
+ * order.  Nodes and queries are SoA device arrays (component i of item k at
+ * [k + off*i], off >= count); mins/maxs (host, ndof) bound the nodes.  Up to
+ * cap pairs go to the device arrays d_out_* in no particular order; *total
+ * receives the number of pairs found (may exceed cap).  Through a cell index
+ * over the first three coordinates instead of the reference's q[0] blocks. */
+int gprm_radius_neighbours(int ndof, const float* d_nodes, int n, long long noff, const float* d_queries, int nq,
+                           long long qoff, const float* mins, const float* maxs, float D, int* d_out_q,
+                           int* d_out_node, float* d_out_d2, int cap, long long* total, void* stream);
+
 /* Learned collision-probability model (no reference counterpart): an MLP
  * 7 -> 256 -> 256 -> 256 -> 1 (SiLU, sigmoid output) on BF16 MFMA with FP32
  * accumulation, trained by its own forward/backward kernels and Adam on
