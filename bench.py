@@ -144,6 +144,214 @@ def cpu_baseline(scene_dir, QS, QE, testnum, target_s, rng):
     }, sub, res
 
 
+def _cpu_pool(scene_dir, nthreads):
+    from oracle import oracle as orc
+
+    if orc.ref_available():
+        return orc.RefPool(scene_dir, nthreads), "reference"
+    o = orc.OracleScene(scene_dir)
+
+    class _Port:
+        def indicator_batch(self, Q):
+            return o.indicator_batch(Q, nthreads=nthreads)
+
+    return _Port(), "port"
+
+
+def leg_points(steps):
+    """Point checks: 4-DOF arm, 8 boxes, 65,536 configurations (one
+    gprm_indicator_dev call per step), with the CPU path on a bounded sample."""
+    import torch
+
+    import gpuprm_amd
+    from oracle.oracle import OracleScene
+
+    d = tempfile.mkdtemp(prefix="gprm_pts_")
+    gpuprm_amd.generate_scene(1, SEED, d)
+    sp = gpuprm_amd.RobotConfigspace(d, 0.01, 1 << 16)
+    sp.init(0, 1)
+    rng = np.random.default_rng(SEED + 1)
+    Q = rng.uniform(-np.pi, np.pi, size=(4, 65536)).astype(np.float32)
+    q = torch.from_numpy(Q).cuda()
+    r = torch.zeros(65536, dtype=torch.int32, device="cuda")
+    for _ in range(3):
+        sp.indicator_dev(q, r)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sp.indicator_dev(q, r)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    got = r.cpu().numpy()
+    sub = rng.choice(65536, 8192, replace=False)
+    mism = int((got[sub] != OracleScene(d).indicator_batch(Q[:, sub].copy(), nthreads=min(16, os.cpu_count() or 1))).sum())
+    nthreads = max(1, min(16, os.cpu_count() or 1))
+    pool, kind = _cpu_pool(d, nthreads)
+    want = pool.indicator_batch(Q)
+    mism += int((got != want).sum())
+    # the CPU path repeats the same 65536 configurations for about 2 s
+    reps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < 2.0:
+        pool.indicator_batch(Q)
+        reps += 1
+    cdt = (time.perf_counter() - t0) / reps
+    return {"workload": "4-DOF arm, 8 obstacle boxes, 65536 configurations", "checks_per_s": 65536 / dt,
+            "ms_per_call": 1000 * dt, "code_mismatches": mism,
+            "cpu": {"value": 65536 / cdt, "unit": "checks/s", "cores": nthreads, "kind": kind,
+                    "sample": "the 65536 configurations %d times, %.2f s each" % (reps, cdt)},
+            "speedup_vs_cpu": cdt / dt}
+
+
+def leg_multi_scene(reps):
+    """64 scenes (7-DOF arm, 8..128 hulls of 8..24 points), 1k-64k uniform
+    configurations each, all checked by one graph-replayed batch (one
+    kinematics launch, one persistent launch per vertex-count bucket), shapes
+    from the autotuner (run first, untimed)."""
+    import torch
+
+    import gpuprm_amd
+
+    rng = np.random.default_rng(SEED + 5)
+    spaces, dirs = [], []
+    for s in range(64):
+        d = tempfile.mkdtemp(prefix="gprm_ms_")
+        gpuprm_amd.generate_scene(5, 1000 + s, d)
+        sp = gpuprm_amd.RobotConfigspace(d, 0.01, 1024)
+        sp.init(0, 1)
+        spaces.append(sp)
+        dirs.append(d)
+    sizes = [int(x) for x in rng.integers(1000, 65537, 64)]
+    qs = [torch.from_numpy(rng.uniform(-np.pi, np.pi, size=(7, n)).astype(np.float32)).cuda() for n in sizes]
+    rs = [torch.zeros(n, dtype=torch.int32, device="cuda") for n in sizes]
+    b = gpuprm_amd.SceneBatch(spaces)
+    table = b.autotune(qs, rs, reps=1)
+    b.indicator_dev(qs, rs, use_graph=True)  # capture
+    torch.cuda.synchronize()
+    times, lat = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        done = b.indicator_dev(qs, rs, use_graph=True)
+        times.append(time.perf_counter() - t0)
+        lat.append(done)
+    # exactness of the batch against the single-scene path on four scenes
+    mism = 0
+    for s in (0, 21, 42, 63):
+        r1 = torch.zeros(sizes[s], dtype=torch.int32, device="cuda")
+        spaces[s].indicator_dev(qs[s], r1)
+        mism += int((r1 != rs[s]).sum().item())
+    dt = float(np.median(times))
+    L = np.median(np.stack(lat), axis=0)
+    info = b.bucket_info()
+    return {"workload": "64 scenes: 7-DOF arm, 8..128 hulls of 8..24 points, 1k-64k configurations each",
+            "configs": int(sum(sizes)), "checks_per_s": sum(sizes) / dt, "ms_per_batch": 1000 * dt,
+            "scene_done_ms_p50": float(np.percentile(L, 50)), "scene_done_ms_p99": float(np.percentile(L, 99)),
+            "buckets": [{"vertices": [x["min_vertices"], x["max_vertices"]], "scenes": x["scenes"],
+                         "shape": x["shape"]} for x in info if x["scenes"]],
+            "tuning": table, "mismatches_vs_single_scene": mism}
+
+
+def _tasks(o, nd, n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < n:
+        a = rng.uniform(-np.pi, np.pi, nd).astype(np.float32)
+        b = rng.uniform(-np.pi, np.pi, nd).astype(np.float32)
+        if o.indicator1(a) == 0 and o.indicator1(b) == 0 and np.linalg.norm(a - b) > 4.0:
+            out.append((a, b))
+    return out
+
+
+def _path_ok(o, path):
+    if path is None:
+        return False
+    res, _ = o.indicator2(path[:-1].T.copy(), path[1:].T.copy(), nthreads=min(16, os.cpu_count() or 1))
+    return bool(np.all(res == 0) and np.all(o.indicator_batch(path.T) == 0))
+
+
+def leg_prm():
+    """Time-to-path of the single-GPU planner: the 7-DOF scene (32 hulls of
+    16 points, radius 0.5), and plain vs surrogate on the 6-DOF edge-checking
+    scene with a model trained on 4M exact labels.  Every path is re-verified
+    by the oracle."""
+    import torch
+
+    import gpuprm_amd
+    from oracle import mlp
+    from oracle.oracle import OracleScene
+
+    out = {}
+    d3 = tempfile.mkdtemp(prefix="gprm_prm3_")
+    gpuprm_amd.generate_scene(3, SEED, d3)
+    sp3 = gpuprm_amd.RobotConfigspace(d3, 0.01, 1 << 20)
+    sp3.init(0, 1)
+    o3 = OracleScene(d3)
+    runs = []
+    for i, (a, b) in enumerate(_tasks(o3, 7, 3, SEED + 7)):
+        r, path = sp3.prm_solve(a, b, num=8192, D=0.5, max_nodes=1 << 20, max_steps=400, seed=i)
+        runs.append({"found": r.found, "valid": _path_ok(o3, path), "ms": r.ms_total, "steps": r.steps,
+                     "nodes": r.nodes_left + r.nodes_right, "edge_samples": r.edge_samples})
+    out["7dof_time_to_path"] = {"workload": "7-DOF arm, 32 hulls of 16 points, D=0.5, 8192 new nodes/step",
+                                "ms_median": float(np.median([x["ms"] for x in runs])), "runs": runs}
+
+    d2 = tempfile.mkdtemp(prefix="gprm_prm2_")
+    gpuprm_amd.generate_scene(2, SEED, d2)
+    sp2 = gpuprm_amd.RobotConfigspace(d2, 0.01, 1 << 20)
+    sp2.init(0, 1)
+    o2 = OracleScene(d2)
+    rng = np.random.default_rng(SEED + 11)
+    N, ntest, B = 1 << 22, 1 << 16, 8192
+    Q = torch.from_numpy(rng.uniform(-np.pi, np.pi, size=(6, N)).astype(np.float32)).cuda()
+    codes = torch.zeros(N, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sp2.indicator_dev(Q, codes)
+    torch.cuda.synchronize()
+    t_label = time.perf_counter() - t0
+    model = gpuprm_amd.Surrogate(ndof=6, batch=B, seed=3)
+    ntr = N - ntest
+    t0 = time.perf_counter()
+    for ep in range(2):
+        for it in range(ntr // B):
+            model.train_step(Q[:, it * B:(it + 1) * B], codes[it * B:(it + 1) * B], lr=2e-3 if ep == 0 else 7e-4,
+                             offset=N)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    pt = torch.zeros(ntest, dtype=torch.float32, device="cuda")
+    model.predict(Q[:, ntr:].contiguous(), pt)
+    auc = mlp.auc(pt.cpu().numpy(), codes[ntr:].cpu().numpy() != 0)
+    P = torch.zeros(N, dtype=torch.float32, device="cuda")
+    model.predict(Q, P)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    model.predict(Q, P)
+    torch.cuda.synchronize()
+    t_pred = time.perf_counter() - t0
+    flop = 2 * (16 * 256 + 2 * 256 * 256 + 256)
+    modes = {"plain": {}, "surrogate": {"order_edges": True, "bias_sampling": True, "edge_budget": 4}}
+    res = {k: [] for k in modes}
+    for i, (a, b) in enumerate(_tasks(o2, 6, 4, SEED + 13)):
+        for k, kw in modes.items():
+            r, path = sp2.prm_solve(a, b, num=1024, D=1.0, max_nodes=1 << 18, max_steps=300, seed=i,
+                                    surrogate=model if kw else None, **kw)
+            res[k].append({"found": r.found, "valid": _path_ok(o2, path), "ms": r.ms_total, "steps": r.steps,
+                           "edge_samples": r.edge_samples, "point_checks": r.point_checks,
+                           "ms_model": r.ms_model})
+    med = {k: float(np.median([x["ms"] for x in v])) for k, v in res.items()}
+    out["surrogate_vs_plain"] = {
+        "workload": "6-DOF arm, 16 hulls of 12 points (edge-checking scene), D=1.0, 1024 new nodes/step, 4 tasks",
+        "labels": N, "label_ms": 1000 * t_label, "train_epochs": 2, "train_s": t_train, "auc": auc,
+        "predict_configs_per_s": N / t_pred, "predict_tflops": N * flop / t_pred / 1e12,
+        "predict_mfma_frac": N * flop / t_pred / 2.5e15,
+        "ms_median": med, "speedup": med["plain"] / med["surrogate"],
+        "edge_samples_saved": 1 - sum(x["edge_samples"] for x in res["surrogate"]) /
+        max(1, sum(x["edge_samples"] for x in res["plain"])),
+        "point_checks_saved": 1 - sum(x["point_checks"] for x in res["surrogate"]) /
+        max(1, sum(x["point_checks"] for x in res["plain"])),
+        "all_paths_valid": all(x["valid"] for v in res.values() for x in v), "runs": res}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +360,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--verify-edges", type=int, default=3000)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the point-check, multi-scene and planner legs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the timed steps, run a plain elementwise kernel that reads and writes "
                          "MIB MiB once each (the known-size kernel that scales HBM byte counters)")
@@ -287,6 +497,13 @@ def main():
     }
     if cpu:
         line["speedup_vs_cpu"] = value / cpu["value"]
+    if not args.no_extras:
+        log("point checks")
+        line["points"] = leg_points(20)
+        log("multi-scene batch")
+        line["multi_scene"] = leg_multi_scene(10)
+        log("planner")
+        line["prm"] = leg_prm()
     print(json.dumps(line), flush=True)
 
 

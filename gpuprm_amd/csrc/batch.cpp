@@ -38,7 +38,7 @@ constexpr int kMaxBatchItems = 1 << 22;  // configurations per scene and call
 struct Bucket {
   int lo = 0, hi = 0;          // vertex-count range
   std::vector<int> scenes;     // indices into the batch
-  size_t lds = 0;              // largest image among its scenes
+  size_t lds = 0;              // largest scene image among its scenes (bytes)
   SvaShape shape;
   DevBuf tasks;                // SceneTask[scenes.size()]
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -299,15 +299,15 @@ int gprm_batch_create(gprm_ctx* const* ctxs, int nscenes, gprm_batch** out) {
     for (auto& bk : b->buckets)
       if (nv >= bk.lo && nv < bk.hi) {
         bk.scenes.push_back(s);
-        bk.lds = std::max(bk.lds, sva_lds_bytes(ctxs[s]->img));
+        bk.lds = std::max(bk.lds, (size_t)ctxs[s]->img.bytes);
       }
   }
   for (auto& bk : b->buckets) {
     if (bk.scenes.empty()) continue;
     // default: as many scene copies per CU as LDS holds (5, 4 or 1)
     const size_t cap = (size_t)sva_max_lds_bytes() - 16;
-    if (5 * bk.lds <= cap) { bk.shape.block = 256; bk.shape.waves = 5; }
-    else if (4 * bk.lds <= cap) { bk.shape.block = 256; bk.shape.waves = 4; }
+    if (5 * (bk.lds + 16 * 256) <= cap) { bk.shape.block = 256; bk.shape.waves = 5; }
+    else if (4 * (bk.lds + 16 * 256) <= cap) { bk.shape.block = 256; bk.shape.waves = 4; }
     else { bk.shape.block = 1024; bk.shape.waves = 4; }
     bk.shape.ppu = 32;
     e = sva_multi_shape_finish(bk.lds, &bk.shape, dev);

@@ -114,8 +114,9 @@ __device__ __forceinline__ T12 t_lapply(const T12& self, const T12& T) {
 }
 
 // Kinematics<ndof>::calculate (kinematics.h:27-37); frames 1..ndof of item
-// to the component-major frame array: entry c of link frame i at
-// out[(12 i + c) stride + item], so consecutive items are consecutive words.
+// to the link-major frame array: the 12 floats of link frame i of item at
+// out[(i stride + item) 12], so one link's frames of consecutive items are
+// contiguous 48-byte records.
 template <int NDOF>
 __device__ __forceinline__ void fk_store(const DevRobot& rb, const float* q, float* __restrict__ out, int item,
                                          long long stride) {
@@ -129,18 +130,21 @@ __device__ __forceinline__ void fk_store(const DevRobot& rb, const float* q, flo
     else dset = q[i];
     T12 A = t_set(rb.a[i], rb.alpha[i], qset, dset);
     T = t_lapply(A, T);
-    float* o = out + (size_t)(12 * i) * stride + item;
-#pragma unroll
-    for (int c = 0; c < 12; ++c) o[(size_t)c * stride] = T.m[c];
+    float4* o = reinterpret_cast<float4*>(out + ((size_t)i * stride + item) * 12);
+    o[0] = make_float4(T.m[0], T.m[1], T.m[2], T.m[3]);
+    o[1] = make_float4(T.m[4], T.m[5], T.m[6], T.m[7]);
+    o[2] = make_float4(T.m[8], T.m[9], T.m[10], T.m[11]);
   }
 }
 
-// Frame of link sys (1..ndof) of item from the component-major frame array.
+// Frame of link sys (1..ndof) of item from the link-major frame array.
 __device__ __forceinline__ T12 load_frame(const float* __restrict__ frames, int item, long long stride, int sys) {
-  const float* f = frames + (size_t)(12 * (sys - 1)) * stride + item;
+  const float4* f = reinterpret_cast<const float4*>(frames + ((size_t)(sys - 1) * stride + item) * 12);
+  const float4 a = f[0], b = f[1], c = f[2];
   T12 t;
-#pragma unroll
-  for (int c = 0; c < 12; ++c) t.m[c] = f[(size_t)c * stride];
+  t.m[0] = a.x; t.m[1] = a.y; t.m[2] = a.z;  t.m[3] = a.w;
+  t.m[4] = b.x; t.m[5] = b.y; t.m[6] = b.z;  t.m[7] = b.w;
+  t.m[8] = c.x; t.m[9] = c.y; t.m[10] = c.z; t.m[11] = c.w;
   return t;
 }
 
@@ -618,12 +622,24 @@ __device__ __forceinline__ Lds stage_scene(const SceneImg& img, int4* smem) {
   return L;
 }
 
-// Units of one scene (its image in L) until its queue is exhausted and
-// every lane of the wave is idle.  H: this lane's iteration history (stride
-// hs).  Waves run independently: there is no block barrier inside.
+// Deferred pair tests of one wave: (item, pair) entries in LDS, a ring of
+// kDeferCap slots.  A wave fills it while fewer than 64 entries wait, so it
+// never holds more than 127.
+constexpr int kDeferCap = 128;
+
+// Units of one scene (its image in L) until its queue is exhausted and every
+// lane of the wave is idle.  Work alternates between two wave-wide phases so
+// that the lanes of a wave run the same code:
+//   A. every lane starts the next pair of its unit (gate, frames, S0) and
+//      applies the bounding-sphere test, which decides most pairs; a pair it
+//      does not decide is appended to the wave's deferral ring;
+//   B. once 64 pairs wait (or no unit is left), every lane takes a deferred
+//      pair and runs the full iteration until the ring is empty.
+// H: this lane's iteration history (stride hs); dq: the wave's ring.  Waves
+// run independently: there is no block barrier inside.
 template <int EDGE>
-__device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, float4* __restrict__ H, size_t hs,
-                                          SvaCounts& cnt) {
+__device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, int2* __restrict__ dq,
+                                          float4* __restrict__ H, size_t hs, SvaCounts& cnt) {
   const int lane = threadIdx.x & 63;
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
   const int nitems = A.nitems >= 0 ? A.nitems : *A.nitems_dev;
@@ -632,189 +648,238 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, float4
   const int nch = A.units_per_item;
   const unsigned nunits = (unsigned)nitems * (unsigned)nch;
 
-  // lane state: the current unit (item, pairs [pair, pend) of the stored
-  // order) and the separating-vector state of its current pair
+  // phase A state: the lane's unit (item, pairs [pair, pend) of the stored
+  // order), its gate and the frames cached for it
   int item = -1, edge = 0, pair = 0, pstart = 0, pend = 0, gate = 0x7fffffff;
-  bool retired = false, need_init = false;
-  int k = 0, l = 0, p = 0, q = 0, vbP = 0, vbQ = 0, crs = 0, fsP = -1, fsQ = -1, ckQ = -1;
+  bool retired = false;
+  int fsP = -1, fsQ = -1, ckQ = -1;
   V3 cQw{0, 0, 0};
   float rQ = 0.0f;
-  bool sysP0 = true, sysQ0 = true;
-  V3 S{0, 0, 0}, w{0, 0, 0};
   T12 tp, tq;
 #pragma unroll
   for (int i = 0; i < 12; ++i) tp.m[i] = tq.m[i] = 0.0f;
 
   unsigned q_base = 0, q_left = 0;
   bool exhausted = false;
+  unsigned d_head = 0, d_tail = 0;  // the ring's cursors (the same in every lane)
   unsigned n_sep = 0, n_col = 0, n_dropped = 0, n_units = 0, n_iter = 0, n_bound = 0, n_claimed = 0;
 
   for (;;) {
-    // ---- refill idle lanes from the wave's queue of units -----------------
-    bool need = (item < 0) && !retired;
-    unsigned long long m = __ballot(need);
-    while (m != 0ull && !exhausted) {
-      if (q_left == 0) {
-        unsigned b = 0;
-        if (lane == 0) b = atomicAdd(A.counter, (unsigned)kChunk);
-        b = __shfl(b, 0, 64);
-        if (b >= nunits) {
-          exhausted = true;
-          break;
+    // ================= phase A: pair starts and bounding-sphere tests ======
+    bool any_unit = true;
+    while (d_tail - d_head < 64) {
+      // ---- refill idle lanes from the wave's queue of units ---------------
+      bool need = (item < 0) && !retired;
+      unsigned long long m = __ballot(need);
+      while (m != 0ull && !exhausted) {
+        if (q_left == 0) {
+          unsigned b = 0;
+          if (lane == 0) b = atomicAdd(A.counter, (unsigned)kChunk);
+          b = __shfl(b, 0, 64);
+          if (b >= nunits) {
+            exhausted = true;
+            break;
+          }
+          q_base = b;
+          q_left = nunits - b < (unsigned)kChunk ? nunits - b : (unsigned)kChunk;
         }
-        q_base = b;
-        q_left = nunits - b < (unsigned)kChunk ? nunits - b : (unsigned)kChunk;
-      }
-      unsigned rank = (unsigned)__popcll(m & lt_mask);
-      unsigned avail = (unsigned)__popcll(m);
-      unsigned take = avail < q_left ? avail : q_left;
-      if (need && rank < take) {
-        // chunk-major order: the lanes of a wave run the same pairs on
-        // consecutive items, so they walk the same polytopes in step
-        unsigned u = q_base + rank;
-        int ch = (int)(u / (unsigned)nitems);
-        item = (int)(u - (unsigned)ch * (unsigned)nitems);
-        pair = ch * PC;
-        pstart = pair;
-        pend = min(M, pair + PC);
-        need = false;
-        need_init = true;
-        ++n_claimed;
-        fsP = fsQ = ckQ = -1;
-        // early exit: in edge mode once any sample of the edge is blocked; in
-        // point mode for pairs after (in CRS order) a pair known to collide,
-        // since the reference returns the first colliding pair's code
-        if (EDGE) {
-          edge = A.item_edge[item];
-          gate = __hip_atomic_load(&A.edge_res[edge], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ? 0 : 0x7fffffff;
-        } else {
-          gate = __hip_atomic_load(&A.word[item], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 4;
+        unsigned rank = (unsigned)__popcll(m & lt_mask);
+        unsigned avail = (unsigned)__popcll(m);
+        unsigned take = avail < q_left ? avail : q_left;
+        if (need && rank < take) {
+          // chunk-major order: the lanes of a wave run the same pairs on
+          // consecutive items, so they walk the same polytopes in step
+          unsigned u = q_base + rank;
+          int ch = (int)(u / (unsigned)nitems);
+          item = (int)(u - (unsigned)ch * (unsigned)nitems);
+          pair = ch * PC;
+          pstart = pair;
+          pend = min(M, pair + PC);
+          need = false;
+          ++n_claimed;
+          fsP = fsQ = ckQ = -1;
+          // early exit: in edge mode once any sample of the edge is blocked;
+          // in point mode for pairs after (in CRS order) a pair known to
+          // collide, since the reference returns the first colliding pair's code
+          if (EDGE) {
+            edge = A.item_edge[item];
+            gate = __hip_atomic_load(&A.edge_res[edge], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ? 0
+                                                                                                        : 0x7fffffff;
+          } else {
+            gate = __hip_atomic_load(&A.word[item], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 4;
+          }
         }
+        q_base += take;
+        q_left -= take;
+        m = __ballot(need);
       }
-      q_base += take;
-      q_left -= take;
-      m = __ballot(need);
-    }
-    if (need && exhausted) retired = true;
-    if (__ballot(item >= 0) == 0ull) break;
-    if (item < 0) continue;
+      if (need && exhausted) retired = true;
+      any_unit = __ballot(item >= 0) != 0ull;
+      if (!any_unit) break;
 
-    // ---- start a pair: gate, frames, S0 (collision4.h:228-244) -------------
-    if (need_init) {
-      int4 pd = L.pairs[pair];
-      crs = pd.z;
-      // the gate value loaded at the unit's start is consulted from its second
-      // pair on, so its memory latency overlaps the first pair's work
-      if (pair != pstart && (EDGE ? gate == 0 : gate <= crs)) {
-        ++n_dropped;
-        if (++pair == pend) item = -1;
+      bool defer = false;
+      int ditem = 0, dpair = 0;
+      if (item >= 0) {
+        const int4 pd = L.pairs[pair];
+        const int crs = pd.z;
+        // the gate loaded at the unit's start is consulted from its second
+        // pair on, so its memory latency overlaps the first pair's work
+        if (pair != pstart && (EDGE ? gate == 0 : gate <= crs)) {
+          ++n_dropped;
+        } else {
+          // start of the pair (collision4.h:228-244) and the bounding-sphere
+          // decision of iteration 0: iteration 0 returns 0 whenever
+          // S.normalize(q0 - p0) >= -1e-6 for the support vertices p0, q0 the
+          // climbs find.  If the bounding spheres show S.(q - p) > 0 with a
+          // wide margin for every vertex p of P and q of Q, that holds
+          // whichever vertices the climbs would pick, so the pair test ends
+          // here exactly as the reference's does (code 0, one iteration).  The
+          // margin, 2^-10 of the magnitudes involved, exceeds the rounding
+          // error of every quantity in the test and in the reference's
+          // iteration by orders of magnitude; a NaN direction never passes.
+          const int vbP = pd.x & 0xffff, vbQ = pd.x >> 16;
+          const int sP = pd.y & 255, sQ = pd.y >> 8;
+          const bool sysP0 = sP == 0, sysQ0 = sQ == 0;
+          if (!sysP0 && sP != fsP) { tp = load_frame(A.frames, item, A.fstride, sP); fsP = sP; }
+          if (!sysQ0 && sQ != fsQ) { tq = load_frame(A.frames, item, A.fstride, sQ); fsQ = sQ; ckQ = -1; }
+          const V3 vq = v3(L.verts[vbQ]), vp = v3(L.verts[vbP]);
+          const V3 Pq = sysQ0 ? vq : t_apply(tq, vq);
+          const V3 Pp = sysP0 ? vp : t_apply(tp, vp);
+          V3 S{Pq.x - Pp.x, Pq.y - Pp.y, Pq.z - Pp.z};
+          const float norm = normalize3(S);
+          if (norm <= kEpsBelow) S.x = 1.0f;
+          const int kP = pd.w & 0xffff, kQ = pd.w >> 16;
+          if (kQ != ckQ) {
+            const float4 sq = L.spheres[kQ];
+            cQw = sysQ0 ? v3(sq) : t_apply(tq, v3(sq));
+            rQ = sq.w;
+            ckQ = kQ;
+          }
+          const float4 sp = L.spheres[kP];
+          const V3 cPw = sysP0 ? v3(sp) : t_apply(tp, v3(sp));
+          const float rad = sp.w + rQ;
+          const float sn1 = fabsf(S.x) + fabsf(S.y) + fabsf(S.z);
+          const float gap = dot3(S, V3{cQw.x - cPw.x, cQw.y - cPw.y, cQw.z - cPw.z}) - sn1 * rad;
+          const float mag = fabsf(cQw.x) + fabsf(cQw.y) + fabsf(cQw.z) + fabsf(cPw.x) + fabsf(cPw.y) +
+                            fabsf(cPw.z) + rad;
+          if (gap > 0x1p-10f * mag * sn1) {
+            ++n_bound;
+            ++n_sep;
+          } else {
+            defer = true;
+            ditem = item;
+            dpair = pair;
+          }
+        }
+        if (++pair == pend) {
+          item = -1;
+          ++n_units;
+        }
+      }
+      // append the undecided pairs to the ring
+      const unsigned long long dm = __ballot(defer);
+      if (defer) dq[(d_tail + (unsigned)__popcll(dm & lt_mask)) & (kDeferCap - 1)] = make_int2(ditem, dpair);
+      d_tail += (unsigned)__popcll(dm);
+    }
+    if (!any_unit && d_tail == d_head) break;
+    // the ring's entries were written by other lanes of this wave
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+
+    // ================= phase B: full iterations of the deferred pairs ======
+    bool busy = false;
+    int bitem = 0, bedge = 0, crs = 0, k = 0, l = 0, p = 0, q = 0, vbP = 0, vbQ = 0;
+    bool sysP0 = true, sysQ0 = true;
+    V3 S{0, 0, 0}, w{0, 0, 0};
+    for (;;) {
+      const unsigned long long wm = __ballot(!busy);
+      const unsigned avail = d_tail - d_head;
+      const unsigned take = (unsigned)__popcll(wm) < avail ? (unsigned)__popcll(wm) : avail;
+      if (!busy && (unsigned)__popcll(wm & lt_mask) < take) {
+        const int2 e = dq[(d_head + (unsigned)__popcll(wm & lt_mask)) & (kDeferCap - 1)];
+        bitem = e.x;
+        const int4 pd = L.pairs[e.y];
+        crs = pd.z;
+        bool drop;
+        if (EDGE) {
+          bedge = A.item_edge[bitem];
+          drop = __hip_atomic_load(&A.edge_res[bedge], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+        } else {
+          drop = (__hip_atomic_load(&A.word[bitem], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 4) <= crs;
+        }
+        if (drop) {
+          ++n_dropped;
+        } else {
+          vbP = pd.x & 0xffff;
+          vbQ = pd.x >> 16;
+          const int sP = pd.y & 255, sQ = pd.y >> 8;
+          sysP0 = sP == 0;
+          sysQ0 = sQ == 0;
+          if (!sysP0) tp = load_frame(A.frames, bitem, A.fstride, sP);
+          if (!sysQ0) tq = load_frame(A.frames, bitem, A.fstride, sQ);
+          const V3 vq = v3(L.verts[vbQ]), vp = v3(L.verts[vbP]);
+          const V3 Pq = sysQ0 ? vq : t_apply(tq, vq);
+          const V3 Pp = sysP0 ? vp : t_apply(tp, vp);
+          S = V3{Pq.x - Pp.x, Pq.y - Pp.y, Pq.z - Pp.z};
+          const float norm = normalize3(S);
+          if (norm <= kEpsBelow) S.x = 1.0f;
+          p = vbP;
+          q = vbQ;
+          k = 0;
+          l = 0;
+          w = V3{0.0f, 0.0f, 0.0f};
+          busy = true;
+        }
+      }
+      d_head += take;
+      if (__ballot(busy) == 0ull) {
+        if (d_head == d_tail) break;
         continue;
       }
-      vbP = pd.x & 0xffff;
-      vbQ = pd.x >> 16;
-      const int sP = pd.y & 255, sQ = pd.y >> 8;
-      sysP0 = sP == 0;
-      sysQ0 = sQ == 0;
-      if (!sysP0 && sP != fsP) { tp = load_frame(A.frames, item, A.fstride, sP); fsP = sP; }
-      if (!sysQ0 && sQ != fsQ) { tq = load_frame(A.frames, item, A.fstride, sQ); fsQ = sQ; }
-      V3 vq = v3(L.verts[vbQ]), vp = v3(L.verts[vbP]);
-      V3 Pq = sysQ0 ? vq : t_apply(tq, vq);
-      V3 Pp = sysP0 ? vp : t_apply(tp, vp);
-      S = V3{Pq.x - Pp.x, Pq.y - Pp.y, Pq.z - Pp.z};
-      float norm = normalize3(S);
-      if (norm <= kEpsBelow) S.x = 1.0f;
-      p = vbP;
-      q = vbQ;
-      k = 0;
-      l = 0;
-      w = V3{0.0f, 0.0f, 0.0f};
-      need_init = false;
+      if (!busy) continue;
+      ++n_iter;
 
-      // Iteration 0 returns 0 whenever S.normalize(q0 - p0) >= -1e-6 for the
-      // support vertices p0, q0 the climbs find.  If the bounding spheres show
-      // S.(q - p) > 0 with a wide margin for every vertex p of P and q of Q,
-      // that holds whichever vertices the climbs would pick, so the pair test
-      // ends here exactly as the reference's does (code 0, one iteration).
-      // The margin, 2^-10 of the magnitudes involved, exceeds the rounding
-      // error of every quantity in the test and in the reference's iteration
-      // by orders of magnitude; a NaN direction never passes.
-      {
-        const int kP = pd.w & 0xffff, kQ = pd.w >> 16;
-        if (kQ != ckQ) {
-          const float4 sq = L.spheres[kQ];
-          cQw = sysQ0 ? v3(sq) : t_apply(tq, v3(sq));
-          rQ = sq.w;
-          ckQ = kQ;
+      // ---- one iteration (collision4.h:266-300) ----------------------------
+      V3 Sp = sysP0 ? S : t_apply_rot_inv(tp, S);
+      V3 Sn{-S.x, -S.y, -S.z};
+      V3 Sq = sysQ0 ? Sn : t_apply_rot_inv(tq, Sn);
+      p = climb(L, p, Sp);
+      q = climb(L, q, Sq);
+      V3 vp = v3(L.verts[p]), vq = v3(L.verts[q]);
+      V3 p0 = sysP0 ? vp : t_apply(tp, vp);
+      V3 q0 = sysQ0 ? vq : t_apply(tq, vq);
+      V3 r{q0.x - p0.x, q0.y - p0.y, q0.z - p0.z};
+      normalize3(r);
+      float dp = dot3(S, r);
+
+      bool fin = false;
+      int code = 0;
+      if (dp >= -kEpsBelow) {
+        fin = true;
+      } else {
+        code = sva_slow(H, hs, k, l, (p - vbP) | ((q - vbQ) << 16), r, dp, S, w);
+        if (code != 0) {
+          fin = true;
+        } else if (++l >= kMaxIter) {
+          fin = true;
+          code = -1;
         }
-        const float4 sp = L.spheres[kP];
-        const V3 cPw = sysP0 ? v3(sp) : t_apply(tp, v3(sp));
-        const float rad = sp.w + rQ;
-        const float sn1 = fabsf(S.x) + fabsf(S.y) + fabsf(S.z);
-        const float gap = dot3(S, V3{cQw.x - cPw.x, cQw.y - cPw.y, cQw.z - cPw.z}) - sn1 * rad;
-        const float mag = fabsf(cQw.x) + fabsf(cQw.y) + fabsf(cQw.z) + fabsf(cPw.x) + fabsf(cPw.y) +
-                          fabsf(cPw.z) + rad;
-        if (gap > 0x1p-10f * mag * sn1) {
-          ++n_bound;
-          ++n_sep;
-          if (++pair == pend) {
-            item = -1;
-            ++n_units;
+      }
+      if (fin) {
+        if (code != 0) {
+          ++n_col;
+          if (EDGE) {
+            __hip_atomic_store(&A.edge_res[bedge], code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           } else {
-            need_init = true;
+            atomicMin(&A.word[bitem], ((crs + 1) << 4) | (code + 1));
           }
-          continue;
-        }
-      }
-    }
-    ++n_iter;
-
-    // ---- one iteration (collision4.h:266-300) ------------------------------
-    V3 Sp = sysP0 ? S : t_apply_rot_inv(tp, S);
-    V3 Sn{-S.x, -S.y, -S.z};
-    V3 Sq = sysQ0 ? Sn : t_apply_rot_inv(tq, Sn);
-    p = climb(L, p, Sp);
-    q = climb(L, q, Sq);
-    V3 vp = v3(L.verts[p]), vq = v3(L.verts[q]);
-    V3 p0 = sysP0 ? vp : t_apply(tp, vp);
-    V3 q0 = sysQ0 ? vq : t_apply(tq, vq);
-    V3 r{q0.x - p0.x, q0.y - p0.y, q0.z - p0.z};
-    normalize3(r);
-    float dp = dot3(S, r);
-
-    bool fin = false;
-    int code = 0;
-    if (dp >= -kEpsBelow) {
-      fin = true;
-    } else {
-      code = sva_slow(H, hs, k, l, (p - vbP) | ((q - vbQ) << 16), r, dp, S, w);
-      if (code != 0) {
-        fin = true;
-      } else if (++l >= kMaxIter) {
-        fin = true;
-        code = -1;
-      }
-    }
-
-    if (fin) {
-      if (code != 0) {
-        ++n_col;
-        if (EDGE) {
-          __hip_atomic_store(&A.edge_res[edge], code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          gate = 0;
         } else {
-          atomicMin(&A.word[item], ((crs + 1) << 4) | (code + 1));
-          gate = min(gate, crs + 1);
+          ++n_sep;
         }
-      } else {
-        ++n_sep;
-      }
-      if (++pair == pend) {
-        item = -1;
-        ++n_units;
-      } else {
-        need_init = true;
+        busy = false;
       }
     }
+    // phase B overwrote the frame registers
+    fsP = fsQ = ckQ = -1;
   }
 
   cnt.sep += n_sep;
@@ -850,12 +915,15 @@ __device__ __forceinline__ void add_tally(const SvaCounts& c, int4* smem, unsign
 template <int EDGE, int BLOCK, int WAVES>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) sva_kernel(SvaArgs A) {
   extern __shared__ int4 smem[];
-  const Lds L = stage_scene<BLOCK>(A.img, smem);
+  // dynamic LDS: the waves' deferral rings (BLOCK/64 x kDeferCap int2 =
+  // BLOCK int4), then the scene image
+  const Lds L = stage_scene<BLOCK>(A.img, smem + BLOCK);
+  int2* const dq = reinterpret_cast<int2*>(smem) + (threadIdx.x >> 6) * kDeferCap;
   __syncthreads();
   float4* __restrict__ H = A.hist + (size_t)blockIdx.x * BLOCK + threadIdx.x;
   const size_t hs = (size_t)gridDim.x * BLOCK;
   SvaCounts c;
-  sva_units<EDGE>(A, L, H, hs, c);
+  sva_units<EDGE>(A, L, dq, H, hs, c);
   if (A.tally != nullptr) add_tally<BLOCK>(c, smem, A.tally);
 }
 
@@ -881,6 +949,8 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVE
   if (threadIdx.x == 0) atomicMin(t_start, (unsigned long long)wall_clock64());
   float4* __restrict__ H = tasks[0].a.hist + (size_t)blockIdx.x * BLOCK + threadIdx.x;
   const size_t hs = (size_t)gridDim.x * BLOCK;
+  // dynamic LDS as in sva_kernel: deferral rings, then the scene image
+  int2* const dq = reinterpret_cast<int2*>(smem) + (threadIdx.x >> 6) * kDeferCap;
   SvaCounts c;
   for (int r = 0; r < ntasks; ++r) {
     const int t = (int)((blockIdx.x + (unsigned)r) % (unsigned)ntasks);
@@ -891,10 +961,10 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVE
       s_live = __hip_atomic_load(A.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < nunits ? 1 : 0;
     __syncthreads();
     if (s_live) {
-      const Lds L = stage_scene<BLOCK>(A.img, smem);
+      const Lds L = stage_scene<BLOCK>(A.img, smem + BLOCK);
       __syncthreads();
       const unsigned before = c.claimed;
-      sva_units<EDGE>(A, L, H, hs, c);
+      sva_units<EDGE>(A, L, dq, H, hs, c);
       const unsigned mine = wave_sum(c.claimed - before);
       if ((threadIdx.x & 63) == 0 && mine > 0) {
         const unsigned old = atomicAdd(tasks[t].done, mine);
@@ -954,7 +1024,9 @@ __global__ void finalize_points_multi_kernel(const FkTask* __restrict__ tasks, i
 // launchers
 // ----------------------------------------------------------------------------
 
-size_t sva_lds_bytes(const SceneImg& img) { return (size_t)img.bytes; }
+// Dynamic LDS of a separating-vector block: the deferral rings (16 B per
+// thread) and the scene image.
+size_t sva_lds_bytes(const SceneImg& img, int block) { return (size_t)img.bytes + 16 * (size_t)block; }
 
 int sva_max_lds_bytes() { return 160 * 1024; }
 
@@ -1040,7 +1112,8 @@ static hipError_t resident_grid(const void* const fns[2], int block, size_t lds,
   return hipSuccess;
 }
 
-hipError_t sva_multi_shape_finish(size_t lds, SvaShape* sh, int device) {
+hipError_t sva_multi_shape_finish(size_t img_bytes, SvaShape* sh, int device) {
+  const size_t lds = img_bytes + 16 * (size_t)sh->block;
   if (lds > (size_t)sva_max_lds_bytes() - 16) return hipErrorInvalidValue;
   if (sh->ppu < 1 || sh->ppu > 256) return hipErrorInvalidValue;
   const void* fns[2] = {sva_multi_variant(0, sh->block, sh->waves), sva_multi_variant(1, sh->block, sh->waves)};
@@ -1048,7 +1121,7 @@ hipError_t sva_multi_shape_finish(size_t lds, SvaShape* sh, int device) {
 }
 
 hipError_t sva_shape_finish(const SceneImg& img, SvaShape* sh, int device) {
-  const size_t lds = sva_lds_bytes(img);
+  const size_t lds = sva_lds_bytes(img, sh->block);
   if (lds > (size_t)sva_max_lds_bytes()) return hipErrorInvalidValue;
   if (sh->ppu < 1 || sh->ppu > 256) return hipErrorInvalidValue;
   const void* fns[2] = {sva_variant(0, sh->block, sh->waves), sva_variant(1, sh->block, sh->waves)};
@@ -1058,11 +1131,11 @@ hipError_t sva_shape_finish(const SceneImg& img, SvaShape* sh, int device) {
 SvaShape sva_default_shape(const SceneImg& img) {
   SvaShape sh;
   // five 256-thread blocks per CU when five scene copies fit in LDS, else
-  // one 1024-thread block sharing a single copy
-  if (5 * sva_lds_bytes(img) <= (size_t)sva_max_lds_bytes()) {
+  // four, else one 1024-thread block sharing a single copy
+  if (5 * sva_lds_bytes(img, 256) <= (size_t)sva_max_lds_bytes()) {
     sh.block = 256;
     sh.waves = 5;
-  } else if (4 * sva_lds_bytes(img) <= (size_t)sva_max_lds_bytes()) {
+  } else if (4 * sva_lds_bytes(img, 256) <= (size_t)sva_max_lds_bytes()) {
     sh.block = 256;
     sh.waves = 4;
   } else {
@@ -1074,7 +1147,7 @@ SvaShape sva_default_shape(const SceneImg& img) {
 }
 
 hipError_t launch_sva(const SvaArgs& a, int edge_mode, const SvaShape& sh, hipStream_t st) {
-  const size_t lds = sva_lds_bytes(a.img);
+  const size_t lds = sva_lds_bytes(a.img, sh.block);
   if (lds > (size_t)sva_max_lds_bytes() || a.hist == nullptr || sh.grid < 1) return hipErrorInvalidValue;
   const void* fn = sva_variant(edge_mode, sh.block, sh.waves);
   if (!fn) return hipErrorInvalidValue;
@@ -1082,9 +1155,10 @@ hipError_t launch_sva(const SvaArgs& a, int edge_mode, const SvaShape& sh, hipSt
   return hipLaunchKernel(fn, dim3(sh.grid), dim3(sh.block), args, lds, st);
 }
 
-hipError_t launch_sva_multi(const SceneTask* d_tasks, int ntasks, int edge_mode, const SvaShape& sh, size_t lds,
+hipError_t launch_sva_multi(const SceneTask* d_tasks, int ntasks, int edge_mode, const SvaShape& sh, size_t img_bytes,
                             unsigned long long* t_start, unsigned long long* tally, hipStream_t st) {
   if (ntasks <= 0) return hipSuccess;
+  size_t lds = img_bytes + 16 * (size_t)sh.block;
   if (lds > (size_t)sva_max_lds_bytes() - 16 || sh.grid < 1) return hipErrorInvalidValue;
   const void* fn = sva_multi_variant(edge_mode, sh.block, sh.waves);
   if (!fn) return hipErrorInvalidValue;

@@ -96,7 +96,8 @@ struct EdgeArgs {
   int* edge_res;           // [edge]
 };
 
-size_t sva_lds_bytes(const SceneImg& img);
+// Dynamic LDS of a separating-vector block of `block` threads on this scene.
+size_t sva_lds_bytes(const SceneImg& img, int block);
 int sva_max_lds_bytes();
 
 // Kinematics of n configurations; frames component-major with item stride n.
@@ -109,7 +110,7 @@ hipError_t launch_sva(const SvaArgs& a, int edge_mode, const SvaShape& sh, hipSt
 // finalize grids have nblocks = sum over tasks of ceil(n / kFkBlock) blocks.
 hipError_t launch_fk_points_multi(const FkTask* d_tasks, int ntasks, int nblocks, int ndof, hipStream_t st);
 hipError_t launch_finalize_points_multi(const FkTask* d_tasks, int ntasks, int nblocks, hipStream_t st);
-hipError_t launch_sva_multi(const SceneTask* d_tasks, int ntasks, int edge_mode, const SvaShape& sh, size_t lds,
+hipError_t launch_sva_multi(const SceneTask* d_tasks, int ntasks, int edge_mode, const SvaShape& sh, size_t img_bytes,
                             unsigned long long* t_start, unsigned long long* tally, hipStream_t st);
 hipError_t launch_edge_prologue(const DevRobot& rb, float dq, EdgeArgs ea, int* testnum_out,
                                 int* stride_out, int* max_testnum, unsigned long long* total,
@@ -152,8 +153,8 @@ struct SvaShape {
 // Variants: (256, 5), (256, 4), (1024, 4).  Fills in grid; an error for an
 // unknown variant, a scene beyond LDS or ppu outside [1, 256].
 hipError_t sva_shape_finish(const SceneImg& img, SvaShape* sh, int device);
-// The same for the multi-scene kernel with lds bytes of scene image.
-hipError_t sva_multi_shape_finish(size_t lds, SvaShape* sh, int device);
+// The same for the multi-scene kernel whose largest scene image has img_bytes.
+hipError_t sva_multi_shape_finish(size_t img_bytes, SvaShape* sh, int device);
 // The shape used until a caller (or the autotuner) sets another.
 SvaShape sva_default_shape(const SceneImg& img);
 
