@@ -304,12 +304,11 @@ int gprm_batch_create(gprm_ctx* const* ctxs, int nscenes, gprm_batch** out) {
   }
   for (auto& bk : b->buckets) {
     if (bk.scenes.empty()) continue;
-    // default: as many scene copies per CU as LDS holds (5, 4 or 1)
+    // default: four scene copies per CU when LDS holds them, else one
     const size_t cap = (size_t)sva_max_lds_bytes() - 16;
-    if (5 * (bk.lds + 16 * 256) <= cap) { bk.shape.block = 256; bk.shape.waves = 5; }
-    else if (4 * (bk.lds + 16 * 256) <= cap) { bk.shape.block = 256; bk.shape.waves = 4; }
+    if (4 * (bk.lds + 16 * 256) <= cap) { bk.shape.block = 256; bk.shape.waves = 4; }
     else { bk.shape.block = 1024; bk.shape.waves = 4; }
-    bk.shape.ppu = 32;
+    bk.shape.ppu = 64;
     e = sva_multi_shape_finish(bk.lds, &bk.shape, dev);
     if (e != hipSuccess) {
       int rc = bhip(nullptr, e, "scene too large for the multi-scene kernel");
@@ -372,7 +371,7 @@ int gprm_batch_autotune(gprm_batch* b, const float* const* d_q, const long long*
   int rc = signature(b, d_q, offset, n, d_res, &sg);
   if (rc != GPRM_OK) return rc;
   const gprm_launch_shape variants[] = {{256, 5, 0, 0}, {256, 4, 0, 0}, {1024, 4, 0, 0}};
-  const int ppus[] = {16, 32, 64};
+  const int ppus[] = {32, 64, 128, 256};
   for (int i = 0; i < kBuckets; ++i) {
     Bucket& bk = b->buckets[i];
     if (bk.scenes.empty()) continue;

@@ -98,7 +98,7 @@ int build_image(gprm_ctx* c, std::vector<char>* out) {
   img.off_verts = (int)off; off = align16(off + 16 * (size_t)sumn);
   img.off_pairs = (int)off; off = align16(off + 16 * (size_t)M);
   img.off_adj = (int)off;   off = align16(off + 2 * nadj);
-  img.off_spheres = (int)off; off = align16(off + 16 * (size_t)N);
+  img.off_spheres = (int)off; off = align16(off + 32 * (size_t)N);
   img.bytes = (int)off;
   out->assign(off, 0);
   char* b = out->data();
@@ -117,8 +117,10 @@ int build_image(gprm_ctx* c, std::vector<char>* out) {
       }
     }
   }
-  // bounding spheres: centre of the vertex bounding box, radius the largest
-  // vertex distance (double), enlarged by a relative 1e-6 and rounded up
+  // bounds per polytope, in its own frame: the centre c of the vertex
+  // bounding box with the largest vertex distance from it (sphere radius),
+  // then the largest |v_a - c_a| per axis (box half extents); both from
+  // doubles, enlarged by a relative 1e-6 and rounded up
   for (int k = 0; k < N; ++k) {
     const Polytope& p = s.polys[k];
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -138,11 +140,17 @@ int build_image(gprm_ctx* c, std::vector<char>* out) {
       }
       r2 = std::max(r2, d2);
     }
-    double r = std::sqrt(r2) * (1.0 + 1e-6) + 1e-30;
-    float rf = (float)r;
-    if ((double)rf < r) rf = std::nextafter(rf, INFINITY);
-    float sp[4] = {c[0], c[1], c[2], rf};
-    std::memcpy(b + img.off_spheres + 16 * (size_t)k, sp, 16);
+    auto up = [](double x) {
+      x = x * (1.0 + 1e-6) + 1e-30;
+      float f = (float)x;
+      if ((double)f < x) f = std::nextafter(f, INFINITY);
+      return f;
+    };
+    double h[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < p.n(); ++i)
+      for (int a = 0; a < 3; ++a) h[a] = std::max(h[a], std::fabs((double)p.verts[3 * i + a] - (double)c[a]));
+    float sp[8] = {c[0], c[1], c[2], up(std::sqrt(r2)), up(h[0]), up(h[1]), up(h[2]), 0.0f};
+    std::memcpy(b + img.off_spheres + 32 * (size_t)k, sp, 32);
   }
   std::vector<int> order(M);
   for (int i = 0; i < M; ++i) order[i] = i;
@@ -242,12 +250,22 @@ int prof_harvest(gprm_ctx* c, Slot& sl) {
   return GPRM_OK;
 }
 
-int units_per_item(const gprm_ctx* c) { return (c->img.M + c->shape.ppu - 1) / c->shape.ppu; }
+// Pairs per work unit of a launch over about n items: the shape's, or fewer
+// so that there are at least two units (chunk x 64-item group) per resident
+// wave; never below 8.
+int unit_pairs(const gprm_ctx* c, long long n) {
+  const long long waves = (long long)c->shape.grid * (c->shape.block / 64);
+  const long long groups = std::max(1LL, (n + 63) / 64);
+  long long want = groups * c->img.M / std::max(1LL, 2 * waves);
+  want = std::max(8LL, std::min((long long)c->shape.ppu, want));
+  return (int)want;
+}
 
 // Items of one launch: at most kItemBudget, and few enough that the unit
-// count items * units_per_item stays below 2^31.
+// count stays below 2^31 even with the smallest units (8 pairs).
 int item_budget(const gprm_ctx* c) {
-  return std::max(1, std::min(kItemBudget, 0x7fffffff / std::max(1, units_per_item(c))));
+  const int upi = (c->img.M + 7) / 8;  // units per item at the smallest unit
+  return std::max(1, std::min(kItemBudget, 0x7fffffff / std::max(1, upi)));
 }
 
 // Point checks of n items on the device (q SoA with offset, res[n]).
@@ -277,8 +295,8 @@ int run_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n,
     a.hist = sl.hist.as<float4>();
     a.nitems = cn;
     a.counter = sl.counters.as<unsigned>() + ci;
-    a.pairs_per_unit = c->shape.ppu;
-    a.units_per_item = units_per_item(c);
+    a.pairs_per_unit = unit_pairs(c, cn);
+    a.units_per_item = (c->img.M + a.pairs_per_unit - 1) / a.pairs_per_unit;
     a.word = sl.status.as<int>();
     a.tally = tally_ptr(c, sl);
     GPRM_HIP(prof_begin(c, sl, 0, st, &ev), "event");
@@ -370,8 +388,8 @@ int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long 
     a.counter = reinterpret_cast<unsigned*>(cnt + 1);
     a.item_edge = sl.item_edge.as<int>();
     a.edge_res = d_res;
-    a.pairs_per_unit = c->shape.ppu;
-    a.units_per_item = units_per_item(c);
+    a.pairs_per_unit = unit_pairs(c, pcap);
+    a.units_per_item = (c->img.M + a.pairs_per_unit - 1) / a.pairs_per_unit;
     a.tally = tally_ptr(c, sl);
     GPRM_HIP(prof_begin(c, sl, 0, st, &ev), "event");
     GPRM_HIP(launch_sva(a, 1, c->shape, st), "separating-vector launch");
@@ -736,7 +754,7 @@ int gprm_autotune(gprm_ctx* c, const float* d_q, const float* d_qe, int n, long 
   GPRM_HIP(hipMalloc(&d_res, sizeof(int) * (size_t)n), "hipMalloc results");
   struct Free { int* p; ~Free() { (void)hipFree(p); } } fr{d_res};
   const gprm_launch_shape variants[] = {{256, 5, 0, 0}, {256, 4, 0, 0}, {1024, 4, 0, 0}};
-  const int ppus[] = {16, 32, 64};
+  const int ppus[] = {32, 64, 128, 256};
   gprm_launch_shape keep{};
   if ((rc = gprm_get_launch_shape(c, &keep)) != GPRM_OK) return rc;
   double best_ms = 1e300;

@@ -480,7 +480,7 @@ __global__ void fill_u32_kernel(unsigned int* p, unsigned int v, long long n) {
 struct Lds {
   const float4* verts;          // (x, y, z, info bits): info = row offset << 8 | degree
   const int4* pairs;            // (vbP | vbQ << 16, sysP | sysQ << 8, CRS index, kP | kQ << 16)
-  const float4* spheres;        // per polytope: bounding sphere (centre, radius), own frame
+  const float4* spheres;        // per polytope, own frame: (centre, sphere radius), (box half extents, 0)
   const unsigned short* adj;    // rows of 4k neighbour ids, padded with the row's own vertex
 };
 
@@ -623,20 +623,137 @@ __device__ __forceinline__ Lds stage_scene(const SceneImg& img, int4* smem) {
 }
 
 // Deferred pair tests of one wave: (item, pair) entries in LDS, a ring of
-// kDeferCap slots.  A wave fills it while fewer than 64 entries wait, so it
+// kDeferCap slots.  Phase B starts as soon as 64 entries wait, so the ring
 // never holds more than 127.
 constexpr int kDeferCap = 128;
 
-// Units of one scene (its image in L) until its queue is exhausted and every
-// lane of the wave is idle.  Work alternates between two wave-wide phases so
-// that the lanes of a wave run the same code:
-//   A. every lane starts the next pair of its unit (gate, frames, S0) and
-//      applies the bounding-sphere test, which decides most pairs; a pair it
-//      does not decide is appended to the wave's deferral ring;
-//   B. once 64 pairs wait (or no unit is left), every lane takes a deferred
-//      pair and runs the full iteration until the ring is empty.
+// Work units of a separating-vector launch: (chunk of pairs_per_unit pairs,
+// group of 64 consecutive items), numbered chunk-major.
+__device__ __forceinline__ unsigned sva_unit_count(const SvaArgs& A, int nitems) {
+  return (unsigned)((nitems + 63) >> 6) * (unsigned)A.units_per_item;
+}
+
+// Phase B of sva_units: every lane takes a deferred pair from the wave's
+// ring and runs the full separating-vector test (collision4.h:227-332) until
+// the ring is empty.  The gate is re-read when a pair is taken.  Clobbers
+// the frame registers tp, tq.
+template <int EDGE>
+__device__ __forceinline__ void sva_drain(const SvaArgs& A, const Lds& L, int2* __restrict__ dq,
+                                          float4* __restrict__ H, size_t hs, unsigned& d_head, unsigned d_tail,
+                                          T12& tp, T12& tq, unsigned& n_sep, unsigned& n_col, unsigned& n_dropped,
+                                          unsigned& n_iter) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt_mask = (1ull << lane) - 1ull;
+  // the ring's entries were written by other lanes of this wave
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  bool busy = false;
+  int bitem = 0, bedge = 0, crs = 0, k = 0, l = 0, p = 0, q = 0, vbP = 0, vbQ = 0;
+  bool sysP0 = true, sysQ0 = true;
+  V3 S{0, 0, 0}, w{0, 0, 0};
+  for (;;) {
+    const unsigned long long wm = __ballot(!busy);
+    const unsigned avail = d_tail - d_head;
+    const unsigned take = (unsigned)__popcll(wm) < avail ? (unsigned)__popcll(wm) : avail;
+    if (!busy && (unsigned)__popcll(wm & lt_mask) < take) {
+      const int2 e = dq[(d_head + (unsigned)__popcll(wm & lt_mask)) & (kDeferCap - 1)];
+      bitem = e.x;
+      const int4 pd = L.pairs[e.y];
+      crs = pd.z;
+      bool drop;
+      if (EDGE) {
+        bedge = A.item_edge[bitem];
+        drop = __hip_atomic_load(&A.edge_res[bedge], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+      } else {
+        drop = (__hip_atomic_load(&A.word[bitem], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 4) <= crs;
+      }
+      if (drop) {
+        ++n_dropped;
+      } else {
+        // start of the pair (collision4.h:228-244)
+        vbP = pd.x & 0xffff;
+        vbQ = pd.x >> 16;
+        const int sP = pd.y & 255, sQ = pd.y >> 8;
+        sysP0 = sP == 0;
+        sysQ0 = sQ == 0;
+        if (!sysP0) tp = load_frame(A.frames, bitem, A.fstride, sP);
+        if (!sysQ0) tq = load_frame(A.frames, bitem, A.fstride, sQ);
+        const V3 vq = v3(L.verts[vbQ]), vp = v3(L.verts[vbP]);
+        const V3 Pq = sysQ0 ? vq : t_apply(tq, vq);
+        const V3 Pp = sysP0 ? vp : t_apply(tp, vp);
+        S = V3{Pq.x - Pp.x, Pq.y - Pp.y, Pq.z - Pp.z};
+        const float norm = normalize3(S);
+        if (norm <= kEpsBelow) S.x = 1.0f;
+        p = vbP;
+        q = vbQ;
+        k = 0;
+        l = 0;
+        w = V3{0.0f, 0.0f, 0.0f};
+        busy = true;
+      }
+    }
+    d_head += take;
+    if (__ballot(busy) == 0ull) {
+      if (d_head == d_tail) break;
+      continue;
+    }
+    if (!busy) continue;
+    ++n_iter;
+
+    // ---- one iteration (collision4.h:266-300) ------------------------------
+    V3 Sp = sysP0 ? S : t_apply_rot_inv(tp, S);
+    V3 Sn{-S.x, -S.y, -S.z};
+    V3 Sq = sysQ0 ? Sn : t_apply_rot_inv(tq, Sn);
+    p = climb(L, p, Sp);
+    q = climb(L, q, Sq);
+    V3 vp = v3(L.verts[p]), vq = v3(L.verts[q]);
+    V3 p0 = sysP0 ? vp : t_apply(tp, vp);
+    V3 q0 = sysQ0 ? vq : t_apply(tq, vq);
+    V3 r{q0.x - p0.x, q0.y - p0.y, q0.z - p0.z};
+    normalize3(r);
+    float dp = dot3(S, r);
+
+    bool fin = false;
+    int code = 0;
+    if (dp >= -kEpsBelow) {
+      fin = true;
+    } else {
+      code = sva_slow(H, hs, k, l, (p - vbP) | ((q - vbQ) << 16), r, dp, S, w);
+      if (code != 0) {
+        fin = true;
+      } else if (++l >= kMaxIter) {
+        fin = true;
+        code = -1;
+      }
+    }
+    if (fin) {
+      if (code != 0) {
+        ++n_col;
+        if (EDGE) {
+          __hip_atomic_store(&A.edge_res[bedge], code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          atomicMin(&A.word[bitem], ((crs + 1) << 4) | (code + 1));
+        }
+      } else {
+        ++n_sep;
+      }
+      busy = false;
+    }
+  }
+}
+
+// Units of one scene (its image in L) until its queue is exhausted, every
+// deferred pair included.  A wave claims a unit (one chunk of pairs, 64
+// consecutive items: lane l takes item 64 g + l) and walks the chunk's pairs
+// in step, so the pair record, the frame changes and the caches below are
+// the same in every lane.  Work alternates between two wave-wide phases:
+//   A. for each pair every lane starts the pair on its item (gate, frames,
+//      the start direction) and applies the bound test, which decides most
+//      pairs; a pair it does not decide is appended to the wave's ring;
+//   B. once 64 pairs wait (and at the end), every lane takes deferred pairs
+//      and runs the full test until the ring is empty (sva_drain).
 // H: this lane's iteration history (stride hs); dq: the wave's ring.  Waves
-// run independently: there is no block barrier inside.
+// run independently: there is no block barrier inside.  cnt.claimed counts
+// the wave's units (the same in every lane).
 template <int EDGE>
 __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, int2* __restrict__ dq,
                                           float4* __restrict__ H, size_t hs, SvaCounts& cnt) {
@@ -645,242 +762,131 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, int2* 
   const int nitems = A.nitems >= 0 ? A.nitems : *A.nitems_dev;
   const int M = A.img.M;
   const int PC = A.pairs_per_unit;
-  const int nch = A.units_per_item;
-  const unsigned nunits = (unsigned)nitems * (unsigned)nch;
+  const unsigned ngroups = (unsigned)((nitems + 63) >> 6);
+  const unsigned nunits = sva_unit_count(A, nitems);
 
-  // phase A state: the lane's unit (item, pairs [pair, pend) of the stored
-  // order), its gate and the frames cached for it
-  int item = -1, edge = 0, pair = 0, pstart = 0, pend = 0, gate = 0x7fffffff;
-  bool retired = false;
-  int fsP = -1, fsQ = -1, ckQ = -1;
-  V3 cQw{0, 0, 0};
-  float rQ = 0.0f;
   T12 tp, tq;
 #pragma unroll
   for (int i = 0; i < 12; ++i) tp.m[i] = tq.m[i] = 0.0f;
-
-  unsigned q_base = 0, q_left = 0;
-  bool exhausted = false;
   unsigned d_head = 0, d_tail = 0;  // the ring's cursors (the same in every lane)
   unsigned n_sep = 0, n_col = 0, n_dropped = 0, n_units = 0, n_iter = 0, n_bound = 0, n_claimed = 0;
 
   for (;;) {
-    // ================= phase A: pair starts and bounding-sphere tests ======
-    bool any_unit = true;
-    while (d_tail - d_head < 64) {
-      // ---- refill idle lanes from the wave's queue of units ---------------
-      bool need = (item < 0) && !retired;
-      unsigned long long m = __ballot(need);
-      while (m != 0ull && !exhausted) {
-        if (q_left == 0) {
-          unsigned b = 0;
-          if (lane == 0) b = atomicAdd(A.counter, (unsigned)kChunk);
-          b = __shfl(b, 0, 64);
-          if (b >= nunits) {
-            exhausted = true;
-            break;
-          }
-          q_base = b;
-          q_left = nunits - b < (unsigned)kChunk ? nunits - b : (unsigned)kChunk;
-        }
-        unsigned rank = (unsigned)__popcll(m & lt_mask);
-        unsigned avail = (unsigned)__popcll(m);
-        unsigned take = avail < q_left ? avail : q_left;
-        if (need && rank < take) {
-          // chunk-major order: the lanes of a wave run the same pairs on
-          // consecutive items, so they walk the same polytopes in step
-          unsigned u = q_base + rank;
-          int ch = (int)(u / (unsigned)nitems);
-          item = (int)(u - (unsigned)ch * (unsigned)nitems);
-          pair = ch * PC;
-          pstart = pair;
-          pend = min(M, pair + PC);
-          need = false;
-          ++n_claimed;
-          fsP = fsQ = ckQ = -1;
-          // early exit: in edge mode once any sample of the edge is blocked;
-          // in point mode for pairs after (in CRS order) a pair known to
-          // collide, since the reference returns the first colliding pair's code
-          if (EDGE) {
-            edge = A.item_edge[item];
-            gate = __hip_atomic_load(&A.edge_res[edge], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ? 0
-                                                                                                        : 0x7fffffff;
-          } else {
-            gate = __hip_atomic_load(&A.word[item], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 4;
-          }
-        }
-        q_base += take;
-        q_left -= take;
-        m = __ballot(need);
+    unsigned u = 0;
+    if (lane == 0) u = atomicAdd(A.counter, 1u);
+    u = __shfl(u, 0, 64);
+    if (u >= nunits) break;
+    ++n_claimed;
+    const unsigned ch = u / ngroups;
+    const int item = (int)((u - ch * ngroups) << 6) + lane;
+    const bool active = item < nitems;
+    const int p0 = (int)ch * PC, p1 = min(M, p0 + PC);
+    // early exit: in edge mode once any sample of the edge is blocked; in
+    // point mode for pairs after (in CRS order) a pair known to collide,
+    // since the reference returns the first colliding pair's code.  The gate
+    // is read lazily so that its memory latency overlaps pair work: the
+    // item's word (point mode) or edge index (edge mode) is loaded here, the
+    // edge's result two pairs later, and the gate is consulted from the
+    // unit's fourth pair on.
+    int edge = 0, gate = 0x7fffffff;
+    if (active) {
+      if (EDGE) edge = A.item_edge[item];
+      else gate = __hip_atomic_load(&A.word[item], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 4;
+    }
+    // wave-uniform caches: the frames of links fsP, fsQ; Q's world bounds for
+    // polytope ckQ and its start vertex cvQ
+    int fsP = -1, fsQ = -1, ckQ = -1, cvQ = -1;
+    V3 cQw{0, 0, 0}, PqW{0, 0, 0}, hQ{0, 0, 0};
+    float rQ = 0.0f;
+    for (int pair = p0; pair < p1; ++pair) {
+      const int4 pd = L.pairs[pair];
+      const int crs = pd.z;
+      const int vbP = pd.x & 0xffff, vbQ = pd.x >> 16;
+      const int sP = pd.y & 255, sQ = pd.y >> 8;
+      const bool sysP0 = sP == 0, sysQ0 = sQ == 0;
+      if (!sysP0 && sP != fsP) {
+        if (active) tp = load_frame(A.frames, item, A.fstride, sP);
+        fsP = sP;
       }
-      if (need && exhausted) retired = true;
-      any_unit = __ballot(item >= 0) != 0ull;
-      if (!any_unit) break;
-
+      if (!sysQ0 && sQ != fsQ) {
+        if (active) tq = load_frame(A.frames, item, A.fstride, sQ);
+        fsQ = sQ;
+        ckQ = cvQ = -1;
+      }
+      const int kP = pd.w & 0xffff, kQ = pd.w >> 16;
+      if (vbQ != cvQ) {
+        const V3 vq = v3(L.verts[vbQ]);
+        PqW = sysQ0 ? vq : t_apply(tq, vq);
+        cvQ = vbQ;
+      }
+      if (kQ != ckQ) {
+        const float4 b0 = L.spheres[2 * kQ], b1 = L.spheres[2 * kQ + 1];
+        cQw = sysQ0 ? v3(b0) : t_apply(tq, v3(b0));
+        rQ = b0.w;
+        hQ = v3(b1);
+        ckQ = kQ;
+      }
+      if (EDGE && active && pair == p0 + 2)
+        gate = __hip_atomic_load(&A.edge_res[edge], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ? 0 : 0x7fffffff;
       bool defer = false;
-      int ditem = 0, dpair = 0;
-      if (item >= 0) {
-        const int4 pd = L.pairs[pair];
-        const int crs = pd.z;
-        // the gate loaded at the unit's start is consulted from its second
-        // pair on, so its memory latency overlaps the first pair's work
-        if (pair != pstart && (EDGE ? gate == 0 : gate <= crs)) {
+      if (active) {
+        if (pair >= p0 + 4 && (EDGE ? gate == 0 : gate <= crs)) {
           ++n_dropped;
         } else {
-          // start of the pair (collision4.h:228-244) and the bounding-sphere
-          // decision of iteration 0: iteration 0 returns 0 whenever
-          // S.normalize(q0 - p0) >= -1e-6 for the support vertices p0, q0 the
-          // climbs find.  If the bounding spheres show S.(q - p) > 0 with a
-          // wide margin for every vertex p of P and q of Q, that holds
-          // whichever vertices the climbs would pick, so the pair test ends
-          // here exactly as the reference's does (code 0, one iteration).  The
-          // margin, 2^-10 of the magnitudes involved, exceeds the rounding
-          // error of every quantity in the test and in the reference's
-          // iteration by orders of magnitude; a NaN direction never passes.
-          const int vbP = pd.x & 0xffff, vbQ = pd.x >> 16;
-          const int sP = pd.y & 255, sQ = pd.y >> 8;
-          const bool sysP0 = sP == 0, sysQ0 = sQ == 0;
-          if (!sysP0 && sP != fsP) { tp = load_frame(A.frames, item, A.fstride, sP); fsP = sP; }
-          if (!sysQ0 && sQ != fsQ) { tq = load_frame(A.frames, item, A.fstride, sQ); fsQ = sQ; ckQ = -1; }
-          const V3 vq = v3(L.verts[vbQ]), vp = v3(L.verts[vbP]);
-          const V3 Pq = sysQ0 ? vq : t_apply(tq, vq);
+          // Start of the pair (collision4.h:228-244) and the bound decision
+          // of iteration 0: iteration 0 returns 0 whenever
+          // S.normalize(q0 - p0) >= -1e-6 for the support vertices p0, q0
+          // the climbs find, S = normalize(D) with D the difference of the
+          // two start vertices.  If the bounds (sphere or box) show
+          // S.(q - p) > 0 with a wide margin for every vertex p of P and q of
+          // Q, that holds whichever vertices the climbs would pick, so the
+          // pair test ends here exactly as the reference's does (code 0, one
+          // iteration).  The margin, 2^-10 of the magnitudes involved,
+          // exceeds the rounding error of every quantity in the test and in
+          // the reference's iteration by orders of magnitude.  Both sides of
+          // the test scale with |S|, so it runs on D itself: the reference's
+          // S differs from D/|D| by a relative 1e-7, far inside the margin.
+          // The test also requires |D|_1 > 1e-5, so |D|_2 > 5.7e-6 stays
+          // clear of the reference's override of S for |D| <= 1e-6; a NaN D
+          // never passes (such pairs are deferred to the exact test).
+          const V3 vp = v3(L.verts[vbP]);
           const V3 Pp = sysP0 ? vp : t_apply(tp, vp);
-          V3 S{Pq.x - Pp.x, Pq.y - Pp.y, Pq.z - Pp.z};
-          const float norm = normalize3(S);
-          if (norm <= kEpsBelow) S.x = 1.0f;
-          const int kP = pd.w & 0xffff, kQ = pd.w >> 16;
-          if (kQ != ckQ) {
-            const float4 sq = L.spheres[kQ];
-            cQw = sysQ0 ? v3(sq) : t_apply(tq, v3(sq));
-            rQ = sq.w;
-            ckQ = kQ;
-          }
-          const float4 sp = L.spheres[kP];
-          const V3 cPw = sysP0 ? v3(sp) : t_apply(tp, v3(sp));
-          const float rad = sp.w + rQ;
+          const V3 S{PqW.x - Pp.x, PqW.y - Pp.y, PqW.z - Pp.z};
+          // support extents along S: the smaller of the sphere's |S| r and
+          // the box's sum_a |S.axis_a| h_a (axes: the frame's rotation
+          // columns); |S| from the hardware square root, whose error is far
+          // inside the margin
+          const float4 b0 = L.spheres[2 * kP], b1 = L.spheres[2 * kP + 1];
+          const V3 cPw = sysP0 ? v3(b0) : t_apply(tp, v3(b0));
+          const V3 sq = sysQ0 ? S : t_apply_rot_inv(tq, S);
+          const V3 sp = sysP0 ? S : t_apply_rot_inv(tp, S);
+          const float nS = __builtin_amdgcn_sqrtf(dot3(S, S));
+          const float extQ = fminf(nS * rQ, (fabsf(sq.x) * hQ.x + fabsf(sq.y) * hQ.y) + fabsf(sq.z) * hQ.z);
+          const float extP = fminf(nS * b0.w, (fabsf(sp.x) * b1.x + fabsf(sp.y) * b1.y) + fabsf(sp.z) * b1.z);
           const float sn1 = fabsf(S.x) + fabsf(S.y) + fabsf(S.z);
-          const float gap = dot3(S, V3{cQw.x - cPw.x, cQw.y - cPw.y, cQw.z - cPw.z}) - sn1 * rad;
+          const float gap = dot3(S, V3{cQw.x - cPw.x, cQw.y - cPw.y, cQw.z - cPw.z}) - extQ - extP;
           const float mag = fabsf(cQw.x) + fabsf(cQw.y) + fabsf(cQw.z) + fabsf(cPw.x) + fabsf(cPw.y) +
-                            fabsf(cPw.z) + rad;
-          if (gap > 0x1p-10f * mag * sn1) {
+                            fabsf(cPw.z) + rQ + b0.w;
+          if (sn1 > 1e-5f && gap > 0x1p-10f * mag * sn1) {
             ++n_bound;
             ++n_sep;
           } else {
             defer = true;
-            ditem = item;
-            dpair = pair;
           }
         }
-        if (++pair == pend) {
-          item = -1;
-          ++n_units;
-        }
       }
-      // append the undecided pairs to the ring
+      // append the undecided pairs to the ring; drain it once 64 wait
       const unsigned long long dm = __ballot(defer);
-      if (defer) dq[(d_tail + (unsigned)__popcll(dm & lt_mask)) & (kDeferCap - 1)] = make_int2(ditem, dpair);
+      if (defer) dq[(d_tail + (unsigned)__popcll(dm & lt_mask)) & (kDeferCap - 1)] = make_int2(item, pair);
       d_tail += (unsigned)__popcll(dm);
-    }
-    if (!any_unit && d_tail == d_head) break;
-    // the ring's entries were written by other lanes of this wave
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-
-    // ================= phase B: full iterations of the deferred pairs ======
-    bool busy = false;
-    int bitem = 0, bedge = 0, crs = 0, k = 0, l = 0, p = 0, q = 0, vbP = 0, vbQ = 0;
-    bool sysP0 = true, sysQ0 = true;
-    V3 S{0, 0, 0}, w{0, 0, 0};
-    for (;;) {
-      const unsigned long long wm = __ballot(!busy);
-      const unsigned avail = d_tail - d_head;
-      const unsigned take = (unsigned)__popcll(wm) < avail ? (unsigned)__popcll(wm) : avail;
-      if (!busy && (unsigned)__popcll(wm & lt_mask) < take) {
-        const int2 e = dq[(d_head + (unsigned)__popcll(wm & lt_mask)) & (kDeferCap - 1)];
-        bitem = e.x;
-        const int4 pd = L.pairs[e.y];
-        crs = pd.z;
-        bool drop;
-        if (EDGE) {
-          bedge = A.item_edge[bitem];
-          drop = __hip_atomic_load(&A.edge_res[bedge], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-        } else {
-          drop = (__hip_atomic_load(&A.word[bitem], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 4) <= crs;
-        }
-        if (drop) {
-          ++n_dropped;
-        } else {
-          vbP = pd.x & 0xffff;
-          vbQ = pd.x >> 16;
-          const int sP = pd.y & 255, sQ = pd.y >> 8;
-          sysP0 = sP == 0;
-          sysQ0 = sQ == 0;
-          if (!sysP0) tp = load_frame(A.frames, bitem, A.fstride, sP);
-          if (!sysQ0) tq = load_frame(A.frames, bitem, A.fstride, sQ);
-          const V3 vq = v3(L.verts[vbQ]), vp = v3(L.verts[vbP]);
-          const V3 Pq = sysQ0 ? vq : t_apply(tq, vq);
-          const V3 Pp = sysP0 ? vp : t_apply(tp, vp);
-          S = V3{Pq.x - Pp.x, Pq.y - Pp.y, Pq.z - Pp.z};
-          const float norm = normalize3(S);
-          if (norm <= kEpsBelow) S.x = 1.0f;
-          p = vbP;
-          q = vbQ;
-          k = 0;
-          l = 0;
-          w = V3{0.0f, 0.0f, 0.0f};
-          busy = true;
-        }
-      }
-      d_head += take;
-      if (__ballot(busy) == 0ull) {
-        if (d_head == d_tail) break;
-        continue;
-      }
-      if (!busy) continue;
-      ++n_iter;
-
-      // ---- one iteration (collision4.h:266-300) ----------------------------
-      V3 Sp = sysP0 ? S : t_apply_rot_inv(tp, S);
-      V3 Sn{-S.x, -S.y, -S.z};
-      V3 Sq = sysQ0 ? Sn : t_apply_rot_inv(tq, Sn);
-      p = climb(L, p, Sp);
-      q = climb(L, q, Sq);
-      V3 vp = v3(L.verts[p]), vq = v3(L.verts[q]);
-      V3 p0 = sysP0 ? vp : t_apply(tp, vp);
-      V3 q0 = sysQ0 ? vq : t_apply(tq, vq);
-      V3 r{q0.x - p0.x, q0.y - p0.y, q0.z - p0.z};
-      normalize3(r);
-      float dp = dot3(S, r);
-
-      bool fin = false;
-      int code = 0;
-      if (dp >= -kEpsBelow) {
-        fin = true;
-      } else {
-        code = sva_slow(H, hs, k, l, (p - vbP) | ((q - vbQ) << 16), r, dp, S, w);
-        if (code != 0) {
-          fin = true;
-        } else if (++l >= kMaxIter) {
-          fin = true;
-          code = -1;
-        }
-      }
-      if (fin) {
-        if (code != 0) {
-          ++n_col;
-          if (EDGE) {
-            __hip_atomic_store(&A.edge_res[bedge], code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          } else {
-            atomicMin(&A.word[bitem], ((crs + 1) << 4) | (code + 1));
-          }
-        } else {
-          ++n_sep;
-        }
-        busy = false;
+      if (d_tail - d_head >= 64) {
+        sva_drain<EDGE>(A, L, dq, H, hs, d_head, d_tail, tp, tq, n_sep, n_col, n_dropped, n_iter);
+        fsP = fsQ = ckQ = cvQ = -1;  // the frame registers were clobbered
       }
     }
-    // phase B overwrote the frame registers
-    fsP = fsQ = ckQ = -1;
+    ++n_units;
   }
+  if (d_tail != d_head) sva_drain<EDGE>(A, L, dq, H, hs, d_head, d_tail, tp, tq, n_sep, n_col, n_dropped, n_iter);
 
   cnt.sep += n_sep;
   cnt.col += n_col;
@@ -927,13 +933,6 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVE
   if (A.tally != nullptr) add_tally<BLOCK>(c, smem, A.tally);
 }
 
-// Sum over the 64 lanes of a wave (every lane receives it).
-__device__ __forceinline__ unsigned wave_sum(unsigned v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
 // Many scenes in one persistent launch.  Blocks start on different scenes
 // (block b on scene b mod ntasks) and visit every scene once in rotation:
 // a block whose next scene still has unclaimed units stages that scene's
@@ -956,7 +955,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVE
     const int t = (int)((blockIdx.x + (unsigned)r) % (unsigned)ntasks);
     const SvaArgs A = tasks[t].a;
     const int ni = A.nitems >= 0 ? A.nitems : *A.nitems_dev;
-    const unsigned nunits = (unsigned)ni * (unsigned)A.units_per_item;
+    const unsigned nunits = sva_unit_count(A, ni);
     if (threadIdx.x == 0)
       s_live = __hip_atomic_load(A.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < nunits ? 1 : 0;
     __syncthreads();
@@ -965,7 +964,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVE
       __syncthreads();
       const unsigned before = c.claimed;
       sva_units<EDGE>(A, L, dq, H, hs, c);
-      const unsigned mine = wave_sum(c.claimed - before);
+      const unsigned mine = c.claimed - before;  // the wave's units (the same in every lane)
       if ((threadIdx.x & 63) == 0 && mine > 0) {
         const unsigned old = atomicAdd(tasks[t].done, mine);
         if (old + mine == nunits) tasks[t].t_done[0] = (unsigned long long)wall_clock64();
@@ -1130,19 +1129,18 @@ hipError_t sva_shape_finish(const SceneImg& img, SvaShape* sh, int device) {
 
 SvaShape sva_default_shape(const SceneImg& img) {
   SvaShape sh;
-  // five 256-thread blocks per CU when five scene copies fit in LDS, else
-  // four, else one 1024-thread block sharing a single copy
-  if (5 * sva_lds_bytes(img, 256) <= (size_t)sva_max_lds_bytes()) {
-    sh.block = 256;
-    sh.waves = 5;
-  } else if (4 * sva_lds_bytes(img, 256) <= (size_t)sva_max_lds_bytes()) {
+  // four 256-thread blocks per CU when four scene copies fit in LDS, else
+  // one 1024-thread block sharing a single copy; units of up to 128 pairs
+  // (measured fastest on the edge-checking workload; launches shrink them
+  // when there are too few items, see unit_pairs in the host runtime)
+  if (4 * sva_lds_bytes(img, 256) <= (size_t)sva_max_lds_bytes()) {
     sh.block = 256;
     sh.waves = 4;
   } else {
     sh.block = 1024;
     sh.waves = 4;
   }
-  sh.ppu = 32;
+  sh.ppu = 128;
   return sh;
 }
 

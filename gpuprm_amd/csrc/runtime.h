@@ -9,7 +9,6 @@ namespace gprm {
 
 constexpr int kFkBlock = 256;       // threads per block of the kinematics kernels
 constexpr int kMaxIter = 10;        // collision4.h:17-18 (max_for_loop, max_for_loop_whole)
-constexpr int kChunk = 64;          // items claimed per wave from the work counter
 constexpr int kMaxDofDev = 16;
 
 // Robot parameters (modified DH, robot.h:10-22) and joint bounds.
@@ -22,10 +21,15 @@ struct DevRobot {
 
 // One contiguous device image of a scene, staged whole into LDS by the
 // separating-vector kernel.  Sections (byte offsets, 16-byte aligned):
-//   verts  float4[sumn]   vertex (x,y,z,0) in the polytope's own frame
-//   pairs  int4[M]        (vertex base of P, vertex base of Q, frame of P, frame of Q)
-//   vinfo  int[sumn]      (adjacency offset << 8) | adjacency count
-//   adj    uint16[summ]   neighbour vertex ids (scene-global), rows as stored
+//   verts    float4[sumn]  vertex (x,y,z) in the polytope's own frame; w holds
+//                          the bits of (adjacency row offset << 8) | degree
+//   pairs    int4[M]       (vbase P | vbase Q << 16, frame P | frame Q << 8,
+//                          CRS index, polytope P | polytope Q << 16), grouped
+//                          by the frames they need
+//   adj      uint16[...]   neighbour vertex ids (scene-global), rows as stored,
+//                          each padded to a multiple of 4 with the row's vertex
+//   spheres  float4[2N]    per polytope, own frame: (centre, sphere radius),
+//                          (box half extents, 0), rounded outwards
 struct SceneImg {
   const int4* blob;        // device pointer to the image
   int bytes;               // image size, multiple of 16
@@ -37,14 +41,15 @@ struct SceneImg {
 struct SvaArgs {
   SceneImg img;
   int ndof;
-  const float* frames;     // link frames, component-major: entry c of link s (1..ndof)
-                           // of item i at frames[(12 (s-1) + c) fstride + i]
+  const float* frames;     // link frames, link-major: the 12 floats of link s (1..ndof)
+                           // of item i at frames[((s-1) fstride + i) 12]
   long long fstride;       // item stride of frames (n of the kinematics launch, or its cap)
   float4* hist;            // [kMaxIter][grid*block] per-lane iteration history
   int nitems;              // item count, or -1: read *nitems_dev
   const int* nitems_dev;
   unsigned int* counter;   // work counter (zero before launch)
-  int pairs_per_unit;      // pairs of one work unit (a lane's claim)
+  int pairs_per_unit;      // pairs of one work unit (a wave claims one chunk of them
+                           // for 64 consecutive items)
   int units_per_item;      // ceil(M / pairs_per_unit)
   // point mode
   int* word;               // [item] first colliding pair and code, see fk_points_kernel
@@ -53,8 +58,8 @@ struct SvaArgs {
   int* edge_res;           // [edge] nonzero once any sample collides
   // optional counters, added once per block at exit (may be null):
   // [0] pair tests ending separated, [1] pair tests ending in a collision,
-  // [2] pair tests skipped by early exit, [3] units completed, [4] iterations
-  // run, [5] pair tests decided separated by the bounding-sphere test
+  // [2] pair tests skipped by early exit, [3] units completed (per lane),
+  // [4] iterations run, [5] pair tests decided separated by the bound test
   unsigned long long* tally;
 };
 constexpr int kTallyCount = 6;
@@ -73,7 +78,7 @@ struct FkTask {
   long long offset;
   int n;
   int block0;                   // first block of this task (tasks sorted by block0)
-  float* frames;                // component-major, item stride n
+  float* frames;                // link-major, item stride n
   int* word;                    // [n] result words
   int* codes;                   // [n] codes written by the finalize kernel
 };
@@ -100,7 +105,7 @@ struct EdgeArgs {
 size_t sva_lds_bytes(const SceneImg& img, int block);
 int sva_max_lds_bytes();
 
-// Kinematics of n configurations; frames component-major with item stride n.
+// Kinematics of n configurations; frames link-major with item stride n.
 hipError_t launch_fk_points(const DevRobot& rb, const float* q, long long offset, int n, float* frames,
                             int* word, int check_bounds, hipStream_t st);
 hipError_t launch_finalize_points(const int* word, int* codes, int n, hipStream_t st);
@@ -117,7 +122,7 @@ hipError_t launch_edge_prologue(const DevRobot& rb, float dq, EdgeArgs ea, int* 
                                 hipStream_t st);
 hipError_t launch_edge_compact(EdgeArgs ea, int e0, int e1, int lo, int hi, int* item_edge,
                                int* item_pos, int* nitems, hipStream_t st);
-// Kinematics of the *nitems (<= cap) compacted samples; frames component-major
+// Kinematics of the *nitems (<= cap) compacted samples; frames link-major
 // with item stride cap.
 hipError_t launch_edge_fk(const DevRobot& rb, EdgeArgs ea, const int* item_edge, const int* item_pos,
                           const int* nitems, int cap, float* frames, hipStream_t st);

@@ -193,7 +193,7 @@ def test_launch_shapes_agree(workload_dir, block, waves):
     wante, _ = o.indicator2(QS, QE, nthreads=8)
     q = torch.from_numpy(Q).cuda()
     qs, qe = torch.from_numpy(QS).cuda(), torch.from_numpy(QE).cuda()
-    for ppu in (1, 7, 16, 64):
+    for ppu in (1, 7, 64, 256):
         sh = s.set_launch_shape(block, waves, ppu)
         assert (sh.block, sh.waves, sh.pairs_per_unit) == (block, waves, ppu) and sh.grid >= 256
         r = torch.zeros(Q.shape[1], dtype=torch.int32, device="cuda")
@@ -214,7 +214,7 @@ def test_autotune_keeps_results(workload_dir):
     Q = rng.uniform(-np.pi, np.pi, size=(6, 20000)).astype(np.float32)
     q = torch.from_numpy(Q).cuda()
     sh, ms = s.autotune(q, reps=2)
-    assert sh.pairs_per_unit in (16, 32, 64) and (sh.block, sh.waves) in ((256, 5), (256, 4), (1024, 4))
+    assert sh.pairs_per_unit in (32, 64, 128, 256) and (sh.block, sh.waves) in ((256, 5), (256, 4), (1024, 4))
     assert ms > 0 and s.get_launch_shape().as_dict() == sh.as_dict()
     r = torch.zeros(Q.shape[1], dtype=torch.int32, device="cuda")
     s.indicator_dev(q, r)
