@@ -19,7 +19,8 @@ import numpy as np
 
 __all__ = ["RobotConfigspace", "Surrogate", "GprmError", "EdgeStats", "ProfileStats", "PrmParams", "PrmResult", "lib",
            "generate_scene", "LIB_PATH", "LaunchShape",
-           "BucketInfo", "SceneBatch", "radius_neighbours"]
+           "BucketInfo", "SceneBatch", "radius_neighbours",
+           "read_prm_output"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgprm.so")
 
@@ -79,6 +80,7 @@ class PrmParams(C.Structure):
         ("bias_sampling", C.c_int),
         ("order_edges", C.c_int),
         ("edge_budget", C.c_int),
+        ("output_dir", C.c_char_p),
     ]
 
 
@@ -244,6 +246,30 @@ def radius_neighbours(nodes, queries, mins, maxs, D, cap=None):
     q, nn, d = oq[:k].cpu().numpy(), on[:k].cpu().numpy(), od[:k].cpu().numpy()
     o = np.lexsort((nn, q))
     return q[o], nn[o], d[o]
+
+
+def read_prm_output(path, ndof):
+    """Read a planner output directory the way prm/matlab/lib/prmoutputread.m
+    does (0-based indices here): node arrays of both trees, their edges (each
+    once, from < to), the connection nodes and the path from goal to start."""
+    def rd(name, dt):
+        return np.fromfile(os.path.join(path, name), dtype=dt)
+    out = {}
+    for side in ("l", "r"):
+        g = "graph" + side
+        Q = rd(g + "/qstoragec.bin", np.float32).reshape(-1, ndof)
+        ef, et = rd(g + "/edgesfromc.bin", np.int32), rd(g + "/edgestoc.bin", np.int32)
+        keep = ef < et
+        out["Q" + side] = Q
+        out["E" + side] = np.stack([ef[keep], et[keep]], axis=1)
+    out["conl"] = int(rd("graphl/endc.bin", np.int32)[0])
+    out["conr"] = int(rd("graphr/startc.bin", np.int32)[0])
+    out["connected"] = int(rd("connection.bin", np.int32)[0])
+    if out["connected"]:
+        out["pathl"] = rd("graphl/pathc.bin", np.int32)
+        out["pathr"] = rd("graphr/pathc.bin", np.int32)
+        out["Qpath"] = np.concatenate([out["Qr"][out["pathr"]], out["Ql"][out["pathl"]]], axis=0)
+    return out
 
 
 class SceneBatch:
@@ -571,16 +597,18 @@ class RobotConfigspace:
     # -- single-process PRM ----------------------------------------------
     def prm_solve(self, qstart, qgoal, num=2048, D=0.5, max_nodes=1 << 18, max_steps=200, seed=1,
                   nbuf=1 << 21, surrogate=None, bias_sampling=False, order_edges=False, edge_budget=0,
-                  path_cap=1 << 16):
+                  path_cap=1 << 16, output_dir=None):
         """Bidirectional PRM from qstart to qgoal (the reference's PRMSolver
         process, prm/src/lib/prm/prmsolver.cpp:1973-2187, on one GPU).
         edge_budget=0 checks every candidate edge, as the reference does; k > 0
         checks at most k per (new node, tree), ranked by the surrogate when
-        order_edges is set, else shortest first.  Returns (PrmResult, path
-        (n, ndof) or None)."""
+        order_edges is set, else shortest first.  output_dir: write both trees
+        in the reference's output layout (see read_prm_output).  Returns
+        (PrmResult, path (n, ndof) or None)."""
         p = PrmParams(int(num), float(D), int(max_nodes), int(max_steps), int(seed), int(nbuf),
                       C.c_void_p(surrogate.handle if surrogate is not None else None),
-                      int(bool(bias_sampling)), int(bool(order_edges)), int(edge_budget))
+                      int(bool(bias_sampling)), int(bool(order_edges)), int(edge_budget),
+                      output_dir.encode() if output_dir else None)
         r = PrmResult()
         path = np.zeros((path_cap, self.ndof), np.float32)
         _check(lib().gprm_prm_solve(self._ctx, _f32(qstart).ctypes.data_as(_FP), _f32(qgoal).ctypes.data_as(_FP),

@@ -221,8 +221,8 @@ int prof_prepare(gprm_ctx* c, Slot& sl, hipStream_t st) {
   sl.ev_used = 0;
   sl.tally_pending = false;
   if (!c->profile) return GPRM_OK;
-  GPRM_HIP(sl.tally.ensure(sizeof(unsigned long long) * kTallyCount), "hipMalloc tally");
-  GPRM_HIP(launch_fill_u32(sl.tally.as<unsigned>(), 0u, 2 * kTallyCount, st), "fill");
+  GPRM_HIP(sl.tally.ensure(sizeof(unsigned long long) * kTallyCount * kTallySlots), "hipMalloc tally");
+  GPRM_HIP(launch_fill_u32(sl.tally.as<unsigned>(), 0u, 2 * kTallyCount * kTallySlots, st), "fill");
   sl.tally_pending = true;
   return GPRM_OK;
 }
@@ -237,8 +237,12 @@ int prof_harvest(gprm_ctx* c, Slot& sl) {
   }
   sl.ev_used = 0;
   if (sl.tally_pending) {
-    unsigned long long t[kTallyCount];
-    GPRM_HIP(hipMemcpy(t, sl.tally.p, sizeof(t), hipMemcpyDeviceToHost), "copy tally");
+    std::vector<unsigned long long> slots((size_t)kTallyCount * kTallySlots);
+    GPRM_HIP(hipMemcpy(slots.data(), sl.tally.p, sizeof(unsigned long long) * slots.size(), hipMemcpyDeviceToHost),
+             "copy tally");
+    unsigned long long t[kTallyCount] = {};
+    for (int k = 0; k < kTallySlots; ++k)
+      for (int i = 0; i < kTallyCount; ++i) t[i] += slots[(size_t)k * kTallyCount + i];
     c->prof.pairs_separated += (long long)t[0];
     c->prof.pairs_colliding += (long long)t[1];
     c->prof.units_dropped += (long long)t[2];
@@ -299,6 +303,7 @@ int run_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n,
     a.units_per_item = (c->img.M + a.pairs_per_unit - 1) / a.pairs_per_unit;
     a.word = sl.status.as<int>();
     a.tally = tally_ptr(c, sl);
+    a.static_first = 1;
     GPRM_HIP(prof_begin(c, sl, 0, st, &ev), "event");
     GPRM_HIP(launch_sva(a, 0, c->shape, st), "separating-vector launch");
     GPRM_HIP(prof_end(c, sl, ev, st), "event");
@@ -391,6 +396,7 @@ int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long 
     a.pairs_per_unit = unit_pairs(c, pcap);
     a.units_per_item = (c->img.M + a.pairs_per_unit - 1) / a.pairs_per_unit;
     a.tally = tally_ptr(c, sl);
+    a.static_first = 1;
     GPRM_HIP(prof_begin(c, sl, 0, st, &ev), "event");
     GPRM_HIP(launch_sva(a, 1, c->shape, st), "separating-vector launch");
     GPRM_HIP(prof_end(c, sl, ev, st), "event");

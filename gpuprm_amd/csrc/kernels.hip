@@ -771,11 +771,51 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, int2* 
   unsigned d_head = 0, d_tail = 0;  // the ring's cursors (the same in every lane)
   unsigned n_sep = 0, n_col = 0, n_dropped = 0, n_units = 0, n_iter = 0, n_bound = 0, n_claimed = 0;
 
+  // Work distribution.  Claims on one shared counter serialise at the
+  // memory side, so most units are assigned statically: with static_first,
+  // wave w takes units w, w + nwaves, ... for three quarters of the full
+  // rounds (at least one round; all units when there are fewer units than
+  // waves).  The rest is claimed from
+  // the counter in guided chunks of max(1, remaining / (2 nwaves)) units; once
+  // fewer units than waves may remain, a wave reads the counter before
+  // claiming, so that waves finding the queue drained leave without an
+  // atomic.
+  const unsigned nwaves = (gridDim.x * blockDim.x) >> 6;
+  const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  unsigned nstatic = 0;
+  if (A.static_first) {
+    const unsigned rounds = nunits / nwaves;
+    // fewer units than waves: one each, nothing left to claim
+    nstatic = rounds == 0 ? nunits : (rounds * 3 / 4 > 1 ? rounds * 3 / 4 : 1u) * nwaves;
+  }
+  unsigned su = wave;          // next static unit
+  unsigned cur = 0, cend = 0;  // units [cur, cend) claimed dynamically
+  unsigned seen = 0;           // the counter as last observed by this wave
   for (;;) {
-    unsigned u = 0;
-    if (lane == 0) u = atomicAdd(A.counter, 1u);
-    u = __shfl(u, 0, 64);
-    if (u >= nunits) break;
+    unsigned u;
+    if (su < nstatic) {
+      u = su;
+      su += nwaves;
+    } else if (cur < cend) {
+      u = cur++;
+    } else {
+      const unsigned left = nunits - nstatic > seen ? nunits - nstatic - seen : 0u;
+      if (left == 0) break;
+      const unsigned k = left / (2 * nwaves) > 1 ? left / (2 * nwaves) : 1u;
+      unsigned b = 0xffffffffu;
+      if (lane == 0) {
+        bool go = true;
+        if (left < nwaves) go = __hip_atomic_load(A.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                                nunits - nstatic;
+        if (go) b = atomicAdd(A.counter, k);
+      }
+      b = __shfl(b, 0, 64);
+      if (b == 0xffffffffu || b >= nunits - nstatic) break;
+      seen = b + k;
+      cur = nstatic + b;
+      cend = cur + k < nunits ? cur + k : nunits;
+      u = cur++;
+    }
     ++n_claimed;
     const unsigned ch = u / ngroups;
     const int item = (int)((u - ch * ngroups) << 6) + lane;
@@ -913,7 +953,8 @@ __device__ __forceinline__ void add_tally(const SvaCounts& c, int4* smem, unsign
   atomicAdd(&t[4], c.iter);
   atomicAdd(&t[5], c.bound);
   __syncthreads();
-  if (tid < kTallyCount) atomicAdd(&tally[tid], (unsigned long long)t[tid]);
+  if (tid < kTallyCount)
+    atomicAdd(&tally[(blockIdx.x % kTallySlots) * kTallyCount + tid], (unsigned long long)t[tid]);
 }
 
 // WAVES: occupancy target in waves per SIMD; the compiler keeps the register

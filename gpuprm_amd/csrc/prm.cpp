@@ -31,8 +31,11 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <filesystem>
+#include <fstream>
 #include <numeric>
 #include <queue>
+#include <string>
 #include <vector>
 
 #include "../../include/gprm.h"
@@ -149,6 +152,51 @@ bool dijkstra(const Tree& t, int from, int to, std::vector<int>* path) {
     }
   }
   return false;
+}
+
+template <class T>
+bool write_bin(const std::string& path, const T* v, size_t n) {
+  std::ofstream f(path, std::ios::binary | std::ios::trunc);
+  if (!f) return false;
+  f.write(reinterpret_cast<const char*>(v), (std::streamsize)(sizeof(T) * n));
+  return (bool)f;
+}
+
+// One tree in the reference's output layout (store_graph,
+// prmsolver.cpp:2976-3055, read by prm/matlab/lib/prmoutputread.m).  The
+// reference's trees are block-indexed with gaps; these are dense, so the
+// compressed files (...c.bin) equal the raw ones and the index mapping is the
+// identity.  Edge weights are written as int, truncated, as the reference's
+// std::vector<int> edgesweight stores them.  blocknum.bin ends up holding the
+// block count (the reference writes it twice; the second write wins): one
+// block holding every node.
+bool store_tree(const std::string& dir, const Tree& t, const std::vector<int>* path, int start, int end) {
+  std::error_code ec;
+  std::filesystem::create_directories(dir, ec);
+  if (ec) return false;
+  std::vector<int> from, to, weight;
+  for (int l = 0; l < t.n; ++l)
+    for (auto& e : t.adj[l]) {
+      from.push_back(l);
+      to.push_back(e.first);
+      weight.push_back((int)e.second);
+    }
+  const int n = t.n, zero = 0, one = 1;
+  bool ok = write_bin(dir + "/qstorage.bin", t.q.data(), (size_t)t.ndof * n) &&
+            write_bin(dir + "/surrnum.bin", t.surrnum.data(), (size_t)n) &&
+            write_bin(dir + "/edgesfrom.bin", from.data(), from.size()) &&
+            write_bin(dir + "/edgesto.bin", to.data(), to.size()) &&
+            write_bin(dir + "/edgesfromc.bin", from.data(), from.size()) &&
+            write_bin(dir + "/edgestoc.bin", to.data(), to.size()) &&
+            write_bin(dir + "/edgesweight.bin", weight.data(), weight.size()) &&
+            write_bin(dir + "/qstoragec.bin", t.q.data(), (size_t)t.ndof * n) &&
+            write_bin(dir + "/blockpos.bin", &zero, 1) && write_bin(dir + "/blocknum.bin", &one, 1) &&
+            write_bin(dir + "/newblockpos.bin", &n, 1) && write_bin(dir + "/numc.bin", &n, 1) &&
+            write_bin(dir + "/startc.bin", &start, 1) && write_bin(dir + "/endc.bin", &end, 1);
+  if (ok && path)
+    ok = write_bin(dir + "/path.bin", path->data(), path->size()) &&
+         write_bin(dir + "/pathc.bin", path->data(), path->size());
+  return ok;
 }
 
 // Model probabilities of n host configurations (SoA with stride n).
@@ -472,10 +520,11 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
   }
   res->nodes_left = T[0].n;
   res->nodes_right = T[1].n;
+  std::vector<int> pl, pr;
+  bool connected = false;
   if (conn_left >= 0) {
-    std::vector<int> pl, pr;
-    bool ok = dijkstra(T[0], 0, conn_left, &pl) && dijkstra(T[1], conn_right, 0, &pr);
-    if (ok) {
+    connected = dijkstra(T[0], 0, conn_left, &pl) && dijkstra(T[1], conn_right, 0, &pr);
+    if (connected) {
       // pl = [conn_left, ..., start], pr = [goal, ..., conn_right]; conn_left
       // and conn_right hold the same configuration
       std::vector<const float*> seq;
@@ -489,6 +538,18 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
   }
   if (hipStreamSynchronize(st) != hipSuccess) return GPRM_E_HIP;
   res->ms_total = ms_since(t0);
+  if (prm->output_dir) {
+    // store_results (prmsolver.cpp:2947-2973): graphl from the start root to
+    // the connection, graphr from the connection to the goal root
+    const std::string root(prm->output_dir);
+    const int conn = connected ? 1 : 0;
+    std::error_code ec;
+    std::filesystem::create_directories(root, ec);
+    if (ec || !store_tree(root + "/graphl", T[0], connected ? &pl : nullptr, 0, std::max(conn_left, 0)) ||
+        !store_tree(root + "/graphr", T[1], connected ? &pr : nullptr, std::max(conn_right, 0), 0) ||
+        !write_bin(root + "/connection.bin", &conn, 1))
+      return GPRM_E_INVALID;
+  }
   return GPRM_OK;
 }
 

@@ -56,13 +56,19 @@ struct SvaArgs {
   // edge mode
   const int* item_edge;    // [item] owning edge
   int* edge_res;           // [edge] nonzero once any sample collides
-  // optional counters, added once per block at exit (may be null):
-  // [0] pair tests ending separated, [1] pair tests ending in a collision,
-  // [2] pair tests skipped by early exit, [3] units completed (per lane),
-  // [4] iterations run, [5] pair tests decided separated by the bound test
+  // optional counters (may be null): kTallySlots slots of kTallyCount, block
+  // b adds its sums into slot b mod kTallySlots, spreading the atomics; a
+  // slot holds [0] pair tests ending separated, [1] pair tests ending in a
+  // collision, [2] pair tests skipped by early exit, [3] units completed (per
+  // lane), [4] iterations run, [5] pair tests decided separated by the bound
+  // test
   unsigned long long* tally;
+  // single-scene launches: wave w of the grid takes unit w without a claim,
+  // and claims from the counter start after the grid's waves
+  int static_first;
 };
 constexpr int kTallyCount = 6;
+constexpr int kTallySlots = 64;
 
 // One scene of a multi-scene separating-vector launch (device memory).
 struct SceneTask {

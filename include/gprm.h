@@ -164,6 +164,9 @@ typedef struct {
   int edge_budget;          /* 0: check every candidate edge (the reference's rule);
                                k > 0: at most k per (new node, tree), lowest rank first
                                (by the model with order_edges, else shortest first) */
+  const char* output_dir;   /* NULL, or a directory that receives both trees and the
+                               path in the reference's output layout (store_results,
+                               prmsolver.cpp:2947-3055; prm/matlab/lib/prmoutputread.m) */
 } gprm_prm_params;
 typedef struct {
   int found, steps, path_len;

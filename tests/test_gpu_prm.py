@@ -26,17 +26,43 @@ def _verify(o, path, qs, qg, D):
     assert np.all(res == 0), np.flatnonzero(res)
 
 
-def test_prm_reference_task_config1(golden_dir):
+def test_prm_reference_task_config1(golden_dir, tmp_path):
     # prm/src/main.cpp:128-133: start (0,0,0,0), goal (3,3,3,3), D = 1.0
+    import gpuprm_amd
     z, d = golden_dir("config1")
     s = _space(d)
     o = OracleScene(d)
     qs = np.zeros(4, np.float32)
     qg = np.full(4, 3.0, np.float32)
-    r, path = s.prm_solve(qs, qg, num=1024, D=1.0, max_nodes=1 << 17, max_steps=300, seed=3)
+    out = str(tmp_path / "prmoutput")
+    r, path = s.prm_solve(qs, qg, num=1024, D=1.0, max_nodes=1 << 17, max_steps=300, seed=3, output_dir=out)
     print(r.as_dict())
     assert r.found == 1 and path is not None
     _verify(o, path, qs, qg, 1.0)
+    # the output directory, read back as prm/matlab/lib/prmoutputread.m reads it
+    g = gpuprm_amd.read_prm_output(out, 4)
+    assert g["connected"] == 1
+    assert g["Ql"].shape == (r.nodes_left, 4) and g["Qr"].shape == (r.nodes_right, 4)
+    assert np.array_equal(g["Ql"][0], qs) and np.array_equal(g["Qr"][0], qg)
+    assert np.array_equal(g["Ql"][g["conl"]], g["Qr"][g["conr"]])
+    # Qpath runs goal -> connection (twice: once per tree) -> start
+    Qp = g["Qpath"]
+    assert len(Qp) == len(path) + 1
+    assert np.array_equal(np.delete(Qp, len(g["pathr"]), axis=0)[::-1], path)
+    for side in ("l", "r"):
+        Q, E = g["Q" + side], g["E" + side]
+        assert np.all(np.linalg.norm(Q[E[:, 0]] - Q[E[:, 1]], axis=1) < 1.0)
+        raw_f = np.fromfile(out + "/graph%s/edgesfrom.bin" % side, np.int32)
+        raw_t = np.fromfile(out + "/graph%s/edgesto.bin" % side, np.int32)
+        assert sorted(zip(raw_f, raw_t)) == sorted(zip(raw_t, raw_f))  # both directions stored
+        surr = np.fromfile(out + "/graph%s/surrnum.bin" % side, np.int32)
+        assert np.array_equal(surr, np.bincount(raw_f, minlength=len(Q)))
+        w = np.fromfile(out + "/graph%s/edgesweight.bin" % side, np.int32)
+        dist = np.linalg.norm(Q[raw_f].astype(np.float64) - Q[raw_t], axis=1)
+        assert np.array_equal(w, dist.astype(np.int32))  # truncated like the reference (all 0 below D = 1)
+        pth = g["path" + side]
+        edges = set(zip(raw_f.tolist(), raw_t.tolist()))
+        assert all((int(a), int(b)) in edges for a, b in zip(pth[:-1], pth[1:]))
 
 
 def test_prm_7dof_workload(workload_dir):
