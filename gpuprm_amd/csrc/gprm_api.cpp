@@ -149,7 +149,11 @@ int build_image(gprm_ctx* c, std::vector<char>* out) {
     double h[3] = {0.0, 0.0, 0.0};
     for (int i = 0; i < p.n(); ++i)
       for (int a = 0; a < 3; ++a) h[a] = std::max(h[a], std::fabs((double)p.verts[3 * i + a] - (double)c[a]));
-    float sp[8] = {c[0], c[1], c[2], up(std::sqrt(r2)), up(h[0]), up(h[1]), up(h[2]), 0.0f};
+    const float rf = up(std::sqrt(r2));
+    // w of the second record: |c|_1 + r, the polytope's magnitude term of the
+    // bound test's margin in its own frame (the world frame for obstacles)
+    const float mag = (float)(std::fabs((double)c[0]) + std::fabs((double)c[1]) + std::fabs((double)c[2]) + rf);
+    float sp[8] = {c[0], c[1], c[2], rf, up(h[0]), up(h[1]), up(h[2]), mag};
     std::memcpy(b + img.off_spheres + 32 * (size_t)k, sp, 32);
   }
   std::vector<int> order(M);

@@ -836,8 +836,9 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, int2* 
     // wave-uniform caches: the frames of links fsP, fsQ; Q's world bounds for
     // polytope ckQ and its start vertex cvQ
     int fsP = -1, fsQ = -1, ckQ = -1, cvQ = -1;
-    V3 cQw{0, 0, 0}, PqW{0, 0, 0}, hQ{0, 0, 0};
-    float rQ = 0.0f;
+    V3 cQw{0, 0, 0}, PqW{0, 0, 0};
+    V3 aQ0{0, 0, 0}, aQ1{0, 0, 0}, aQ2{0, 0, 0};  // Q's box axes in the world frame, times its half extents
+    float rQ = 0.0f, magQ = 0.0f;
     for (int pair = p0; pair < p1; ++pair) {
       const int4 pd = L.pairs[pair];
       const int crs = pd.z;
@@ -863,7 +864,16 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, int2* 
         const float4 b0 = L.spheres[2 * kQ], b1 = L.spheres[2 * kQ + 1];
         cQw = sysQ0 ? v3(b0) : t_apply(tq, v3(b0));
         rQ = b0.w;
-        hQ = v3(b1);
+        magQ = fabsf(cQw.x) + fabsf(cQw.y) + fabsf(cQw.z) + rQ;
+        if (sysQ0) {
+          aQ0 = V3{b1.x, 0.0f, 0.0f};
+          aQ1 = V3{0.0f, b1.y, 0.0f};
+          aQ2 = V3{0.0f, 0.0f, b1.z};
+        } else {
+          aQ0 = V3{tq.m[0] * b1.x, tq.m[1] * b1.x, tq.m[2] * b1.x};
+          aQ1 = V3{tq.m[3] * b1.y, tq.m[4] * b1.y, tq.m[5] * b1.y};
+          aQ2 = V3{tq.m[6] * b1.z, tq.m[7] * b1.z, tq.m[8] * b1.z};
+        }
         ckQ = kQ;
       }
       if (EDGE && active && pair == p0 + 2)
@@ -895,18 +905,26 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, int2* 
           // support extents along S: the smaller of the sphere's |S| r and
           // the box's sum_a |S.axis_a| h_a (axes: the frame's rotation
           // columns); |S| from the hardware square root, whose error is far
-          // inside the margin
+          // inside the margin.  The magnitude term of the margin is |c|_1 + r
+          // per side (precomputed for world-frame polytopes).
           const float4 b0 = L.spheres[2 * kP], b1 = L.spheres[2 * kP + 1];
           const V3 cPw = sysP0 ? v3(b0) : t_apply(tp, v3(b0));
-          const V3 sq = sysQ0 ? S : t_apply_rot_inv(tq, S);
-          const V3 sp = sysP0 ? S : t_apply_rot_inv(tp, S);
           const float nS = __builtin_amdgcn_sqrtf(dot3(S, S));
-          const float extQ = fminf(nS * rQ, (fabsf(sq.x) * hQ.x + fabsf(sq.y) * hQ.y) + fabsf(sq.z) * hQ.z);
-          const float extP = fminf(nS * b0.w, (fabsf(sp.x) * b1.x + fabsf(sp.y) * b1.y) + fabsf(sp.z) * b1.z);
+          const float boxQ = (fabsf(dot3(S, aQ0)) + fabsf(dot3(S, aQ1))) + fabsf(dot3(S, aQ2));
+          float boxP, magP;
+          if (sysP0) {
+            boxP = (fabsf(S.x) * b1.x + fabsf(S.y) * b1.y) + fabsf(S.z) * b1.z;
+            magP = b1.w;
+          } else {
+            const V3 sp = t_apply_rot_inv(tp, S);
+            boxP = (fabsf(sp.x) * b1.x + fabsf(sp.y) * b1.y) + fabsf(sp.z) * b1.z;
+            magP = fabsf(cPw.x) + fabsf(cPw.y) + fabsf(cPw.z) + b0.w;
+          }
+          const float extQ = fminf(nS * rQ, boxQ);
+          const float extP = fminf(nS * b0.w, boxP);
           const float sn1 = fabsf(S.x) + fabsf(S.y) + fabsf(S.z);
           const float gap = dot3(S, V3{cQw.x - cPw.x, cQw.y - cPw.y, cQw.z - cPw.z}) - extQ - extP;
-          const float mag = fabsf(cQw.x) + fabsf(cQw.y) + fabsf(cQw.z) + fabsf(cPw.x) + fabsf(cPw.y) +
-                            fabsf(cPw.z) + rQ + b0.w;
+          const float mag = magQ + magP;
           if (sn1 > 1e-5f && gap > 0x1p-10f * mag * sn1) {
             ++n_bound;
             ++n_sep;

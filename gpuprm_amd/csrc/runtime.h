@@ -29,7 +29,8 @@ struct DevRobot {
 //   adj      uint16[...]   neighbour vertex ids (scene-global), rows as stored,
 //                          each padded to a multiple of 4 with the row's vertex
 //   spheres  float4[2N]    per polytope, own frame: (centre, sphere radius),
-//                          (box half extents, 0), rounded outwards
+//                          (box half extents, |centre|_1 + radius), rounded
+//                          outwards
 struct SceneImg {
   const int4* blob;        // device pointer to the image
   int bytes;               // image size, multiple of 16
