@@ -58,6 +58,8 @@ struct GemmArgs {
   unsigned short* Ch;                           // BF16 output (activation / activation grad)
   const float* bias;                            // forward: bias[n]
   const float* aux;                             // backward: pre-activation A_l[m*ldc + n]
+  float* ws;                                    // weight gradients: per-split partial sums
+  long long zstride;                            //   (ws + z*zstride), summed into Cf afterwards
 };
 
 template <int EPI>
@@ -70,13 +72,21 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
   f32x4 acc[2][2];
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  for (int k0 = 0; k0 < g.K; k0 += kTK) {
+  // split K: block z of gridDim.z covers K range [kb, ke)
+  const int kc = (g.K + (int)gridDim.z - 1) / (int)gridDim.z;
+  const int kb = (int)blockIdx.z * kc, ke = min(g.K, kb + kc);
+  for (int k0 = kb; k0 < ke; k0 += kTK) {
     // stage A (64 x 32) and B^T (64 x 32) tiles, zero outside the matrices
+    // and the block's K range; consecutive threads walk each operand's
+    // contiguous dimension
     for (int e = tid; e < kTile * kTK; e += 256) {
-      const int r = e / kTK, c = e % kTK;
-      const int m = m0 + r, n = n0 + r, k = k0 + c;
-      As[r][c] = (m < g.M && k < g.K) ? g.A[m * g.lam + k * g.lak] : (unsigned short)0;
-      Bs[r][c] = (n < g.N && k < g.K) ? g.B[k * g.lbk + n * g.lbn] : (unsigned short)0;
+      int r, c;
+      if (g.lak == 1) { r = e / kTK; c = e % kTK; } else { r = e % kTile; c = e / kTile; }
+      const int m = m0 + r, k = k0 + c;
+      As[r][c] = (m < g.M && k < ke) ? g.A[m * g.lam + k * g.lak] : (unsigned short)0;
+      if (g.lbk == 1) { r = e / kTK; c = e % kTK; } else { r = e % kTile; c = e / kTile; }
+      const int n = n0 + r, k2 = k0 + c;
+      Bs[r][c] = (n < g.N && k2 < ke) ? g.B[k2 * g.lbk + n * g.lbn] : (unsigned short)0;
     }
     __syncthreads();
 #pragma unroll
@@ -115,6 +125,8 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
           g.Ch[o] = f2bf(silu(a));
         } else if (EPI == kEpiBwd) {
           g.Ch[o] = f2bf(v * dsilu(g.aux[o]));
+        } else if (gridDim.z > 1) {
+          g.ws[blockIdx.z * g.zstride + o] = v;  // split K: this block's partial sum
         } else {
           g.Cf[o] = v;
         }
@@ -148,7 +160,7 @@ __global__ void head_fwd_kernel(const unsigned short* __restrict__ H3, const flo
 // dz = (p - y)/n; dA3 = dz w4 * silu'(A3) (bf16); loss terms.
 __global__ void head_bwd_kernel(const float* __restrict__ p, const float* __restrict__ z, const float* __restrict__ y,
                                 const float* __restrict__ w4, const float* __restrict__ A3, int n,
-                                unsigned short* __restrict__ dA3, float* __restrict__ dz, float* __restrict__ loss) {
+                                unsigned short* __restrict__ dA3, float* __restrict__ dz, float* __restrict__ lrow) {
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (row >= n) return;
   const float d = (p[row] - y[row]) / (float)n;
@@ -158,19 +170,21 @@ __global__ void head_bwd_kernel(const float* __restrict__ p, const float* __rest
     // BCE from the logit, stable form: max(z,0) - z*y + log(1 + exp(-|z|))
     float zz = z[row];
     float l = fmaxf(zz, 0.0f) - zz * y[row] + log1pf(__expf(-fabsf(zz)));
-    atomicAdd(loss, l / (float)n);
+    lrow[row] = l / (float)n;
   }
 }
 
 // dW4[j] = sum_b dz[b] H3[b][j]; db4 = sum dz; and column sums of bf16 dA
-// (bias gradients), one thread per column.
+// (bias gradients): one thread per column, block y sums rows
+// [y * rows, (y + 1) * rows) into out[y * cols + j].
 __global__ void colsum_kernel(const unsigned short* __restrict__ Hh, const float* __restrict__ w, int n, int cols,
-                              float* __restrict__ out) {
+                              int rows, float* __restrict__ out) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= cols) return;
+  const int b0 = blockIdx.y * rows, b1 = min(n, b0 + rows);
   float s = 0.0f;
-  for (int b = 0; b < n; ++b) s += (w ? w[b] : 1.0f) * bf2f(Hh[(long long)b * cols + j]);
-  out[j] = s;
+  for (int b = b0; b < b1; ++b) s += (w ? w[b] : 1.0f) * bf2f(Hh[(long long)b * cols + j]);
+  out[(long long)blockIdx.y * cols + j] = s;
 }
 __global__ void sum_kernel(const float* __restrict__ v, int n, float* __restrict__ out) {
   __shared__ float sh[256];
@@ -227,6 +241,8 @@ struct Surrogate {
                  *dA3 = nullptr;
   float *A1 = nullptr, *A2 = nullptr, *A3 = nullptr, *z = nullptr, *p = nullptr, *y = nullptr, *dz = nullptr,
         *loss = nullptr;
+  float* ws = nullptr;     // split-K partial sums of the weight gradients and column sums
+  float* lrow = nullptr;   // per-row loss terms
   std::vector<void*> allocs;
 };
 
@@ -241,11 +257,163 @@ bool dalloc(Surrogate* s, T** p, size_t n) {
   return true;
 }
 
-hipError_t gemm(int epi, const GemmArgs& g, hipStream_t st) {
-  dim3 grid((g.N + kTile - 1) / kTile, (g.M + kTile - 1) / kTile), block(256);
+// ---- fused inference ------------------------------------------------------
+// The whole forward of 64 rows per 256-thread block, activations kept in LDS
+// (two 64 x 264 bf16 buffers; rows padded to spread the banks), so no
+// activation goes to HBM.  Wave w computes output columns [64w, 64w + 64) of
+// every layer for all 64 rows: 4 x 4 tiles of 16 x 16 on
+// v_mfma_f32_16x16x16bf16_1k, K in the same order as the layered GEMM, so
+// the hidden activations equal the training path's bit for bit.  Weight
+// fragments (8 bytes per lane) come straight from global memory, where the
+// 2 x 128 KB of W2, W3 stay resident in L2.
+constexpr int kFR = 64;          // rows per block
+constexpr int kHP = kH + 8;      // LDS row pitch (bf16 elements)
+constexpr size_t kPredictLds = 2 * (size_t)kFR * kHP * sizeof(unsigned short);
+
+__device__ __forceinline__ bf16x4 ld_bf16x4(const unsigned short* p) {
+  const uint2 v = *reinterpret_cast<const uint2*>(p);
+  bf16x4 r;
+  r[0] = (short)(v.x & 0xffff); r[1] = (short)(v.x >> 16);
+  r[2] = (short)(v.y & 0xffff); r[3] = (short)(v.y >> 16);
+  return r;
+}
+
+// acc (+ bias, SiLU, bf16) into the LDS activation buffer: D(row, col) of a
+// 16x16 tile sits in lane (col % 16), rows 4 (lane / 16) + r.
+__device__ __forceinline__ void fused_epilogue(const f32x4 (&acc)[4][4], const float* __restrict__ bias,
+                                               unsigned short* __restrict__ out, int wave, int lane) {
+#pragma unroll
+  for (int cj = 0; cj < 4; ++cj) {
+    const int col = 64 * wave + 16 * cj + (lane & 15);
+    const float b = bias[col];
+#pragma unroll
+    for (int ri = 0; ri < 4; ++ri)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * ri + 4 * (lane >> 4) + r;
+        out[row * kHP + col] = f2bf(silu(acc[ri][cj][r] + b));
+      }
+  }
+}
+
+// One hidden layer 256 -> 256 from LDS buffer `in` into `out`.
+__device__ __forceinline__ void fused_layer(const unsigned short* __restrict__ in, const unsigned short* __restrict__ W,
+                                            const float* __restrict__ bias, unsigned short* __restrict__ out,
+                                            int wave, int lane) {
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int li = lane & 15, lk = 4 * (lane >> 4);
+  for (int k0 = 0; k0 < kH; k0 += 16) {
+    bf16x4 a[4], b[4];
+#pragma unroll
+    for (int ri = 0; ri < 4; ++ri) a[ri] = ld_bf16x4(in + (16 * ri + li) * kHP + k0 + lk);
+#pragma unroll
+    for (int cj = 0; cj < 4; ++cj) b[cj] = ld_bf16x4(W + (size_t)(64 * wave + 16 * cj + li) * kH + k0 + lk);
+#pragma unroll
+    for (int ri = 0; ri < 4; ++ri)
+#pragma unroll
+      for (int cj = 0; cj < 4; ++cj)
+        acc[ri][cj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[ri], b[cj], acc[ri][cj], 0, 0, 0);
+  }
+  fused_epilogue(acc, bias, out, wave, lane);
+}
+
+__global__ void __launch_bounds__(256)
+    mlp_predict_kernel(const float* __restrict__ q, long long off, int n, int nd, const unsigned short* __restrict__ W1h,
+                       const float* __restrict__ b1, const unsigned short* __restrict__ W2h,
+                       const float* __restrict__ b2, const unsigned short* __restrict__ W3h,
+                       const float* __restrict__ b3, const float* __restrict__ w4, const float* __restrict__ b4,
+                       float* __restrict__ prob) {
+  extern __shared__ unsigned short hs[];
+  unsigned short* const h0 = hs;
+  unsigned short* const h1 = hs + kFR * kHP;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r0 = blockIdx.x * kFR;
+  const int li = lane & 15, lk = 4 * (lane >> 4);
+  // layer 1: the inputs bf16(q / pi) (zero beyond ndof and n) as A
+  // fragments, K = 16 in one step
+  {
+    bf16x4 xa[4];
+#pragma unroll
+    for (int ri = 0; ri < 4; ++ri) {
+      const int row = r0 + 16 * ri + li;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int k = lk + j;
+        xa[ri][j] = (short)((row < n && k < nd) ? f2bf(q[row + off * k] * 0.3183098861837907f) : 0);
+      }
+    }
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int cj = 0; cj < 4; ++cj) {
+      const bf16x4 wb = ld_bf16x4(W1h + (size_t)(64 * wave + 16 * cj + li) * kInPad + lk);
+#pragma unroll
+      for (int ri = 0; ri < 4; ++ri)
+        acc[ri][cj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xa[ri], wb, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+    }
+    fused_epilogue(acc, b1, h0, wave, lane);
+  }
+  __syncthreads();
+  fused_layer(h0, W2h, b2, h1, wave, lane);
+  __syncthreads();
+  fused_layer(h1, W3h, b3, h0, wave, lane);
+  __syncthreads();
+  // head: z = h3 . w4 + b4 (four threads per row, a quarter of the inputs
+  // each), p = sigmoid(z)
+  {
+    const int row = tid >> 2, part = tid & 3;
+    float sacc = 0.0f;
+    for (int k = 64 * part; k < 64 * part + 64; ++k) sacc += bf2f(h0[row * kHP + k]) * w4[k];
+    sacc += __shfl_xor(sacc, 1, 64);
+    sacc += __shfl_xor(sacc, 2, 64);
+    if (part == 0 && r0 + row < n) prob[r0 + row] = 1.0f / (1.0f + __expf(-(sacc + b4[0])));
+  }
+}
+
+// out[i] = sum over z < nz of ws[z * stride + i]
+__global__ void split_sum_kernel(const float* __restrict__ ws, long long stride, int nz, int n, float* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float acc = 0.0f;
+  for (int z = 0; z < nz; ++z) acc += ws[z * stride + i];
+  out[i] = acc;
+}
+
+constexpr int kMaxSplit = 64;
+
+// Column sums of n rows (bias gradients) in up to kMaxSplit row chunks,
+// partials through the workspace.
+hipError_t colsum(const unsigned short* Hh, const float* w, int n, float* ws, float* out, hipStream_t st) {
+  const int chunks = std::max(1, std::min(kMaxSplit, (n + 255) / 256));
+  const int rows = (n + chunks - 1) / chunks;
+  hipLaunchKernelGGL(colsum_kernel, dim3(1, chunks), dim3(kH), 0, st, Hh, w, n, kH, rows, ws);
+  hipLaunchKernelGGL(split_sum_kernel, dim3(1), dim3(kH), 0, st, ws, (long long)kH, chunks, kH, out);
+  return hipGetLastError();
+}
+
+// Weight gradients reduce over the batch (K = rows): K is split over
+// gridDim.z blocks of about 256 rows each, whose partial sums go to the
+// workspace and are added up by split_sum_kernel.
+hipError_t gemm(int epi, const GemmArgs& g0, hipStream_t st) {
+  GemmArgs g = g0;
+  int split = 1;
+  if (epi == kEpiGrad && g.K > 512 && g.ws) {
+    split = (g.K + 255) / 256;
+    if (split > kMaxSplit) split = kMaxSplit;
+    g.zstride = (long long)g.M * g.ldc;
+  }
+  dim3 grid((g.N + kTile - 1) / kTile, (g.M + kTile - 1) / kTile, split), block(256);
   if (epi == kEpiFwd) hipLaunchKernelGGL(gemm_bf16_kernel<kEpiFwd>, grid, block, 0, st, g);
   else if (epi == kEpiBwd) hipLaunchKernelGGL(gemm_bf16_kernel<kEpiBwd>, grid, block, 0, st, g);
   else hipLaunchKernelGGL(gemm_bf16_kernel<kEpiGrad>, grid, block, 0, st, g);
+  if (split > 1) {
+    const int nout = g.M * g.ldc;
+    hipLaunchKernelGGL(split_sum_kernel, dim3((nout + 255) / 256), dim3(256), 0, st, g.ws, g.zstride, split, nout,
+                       g.Cf);
+  }
   return hipGetLastError();
 }
 
@@ -276,18 +444,20 @@ hipError_t forward(Surrogate* s, int n, hipStream_t st) {
 
 // Backward of the last forward against labels in y; gradients into G.
 hipError_t backward(Surrogate* s, int n, hipStream_t st) {
-  hipError_t e = hipMemsetAsync(s->loss, 0, sizeof(float), st);
-  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(head_bwd_kernel, dim3((n + 3) / 4), dim3(256), 0, st, s->p, s->z, s->y, s->P[6], s->A3, n,
-                     s->dA3, s->dz, s->loss);
-  hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(256), 0, st, s->H3, s->dz, n, kH, s->G[6]);
+                     s->dA3, s->dz, s->lrow);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, st, s->lrow, n, s->loss);
+  if ((e = colsum(s->H3, s->dz, n, s->ws, s->G[6], st)) != hipSuccess) return e;
   hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, st, s->dz, n, s->G[7]);
   // layer 3: dW3 = dA3^T H2, db3, dA2 = (dA3 W3) * silu'(A2)
   GemmArgs g{};
   g.M = kH; g.N = kH; g.K = n; g.ldc = kH;
   g.A = s->dA3; g.lam = 1; g.lak = kH; g.B = s->H2; g.lbk = kH; g.lbn = 1; g.Cf = s->G[4];
+  g.ws = s->ws;
   if ((e = gemm(kEpiGrad, g, st)) != hipSuccess) return e;
-  hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(256), 0, st, s->dA3, (const float*)nullptr, n, kH, s->G[5]);
+  if ((e = colsum(s->dA3, nullptr, n, s->ws, s->G[5], st)) != hipSuccess) return e;
   GemmArgs d{};
   d.M = n; d.N = kH; d.K = kH; d.ldc = kH;
   d.A = s->dA3; d.lam = kH; d.lak = 1; d.B = s->W3h; d.lbk = kH; d.lbn = 1; d.Ch = s->dA2; d.aux = s->A2;
@@ -295,13 +465,13 @@ hipError_t backward(Surrogate* s, int n, hipStream_t st) {
   // layer 2
   g.A = s->dA2; g.B = s->H1; g.Cf = s->G[2];
   if ((e = gemm(kEpiGrad, g, st)) != hipSuccess) return e;
-  hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(256), 0, st, s->dA2, (const float*)nullptr, n, kH, s->G[3]);
+  if ((e = colsum(s->dA2, nullptr, n, s->ws, s->G[3], st)) != hipSuccess) return e;
   d.A = s->dA2; d.B = s->W2h; d.Ch = s->dA1; d.aux = s->A1;
   if ((e = gemm(kEpiBwd, d, st)) != hipSuccess) return e;
   // layer 1: dW1 = dA1^T X (256 x 16), db1
   g.N = kInPad; g.ldc = kInPad; g.A = s->dA1; g.B = s->X; g.lbk = kInPad; g.Cf = s->G[0];
   if ((e = gemm(kEpiGrad, g, st)) != hipSuccess) return e;
-  hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(256), 0, st, s->dA1, (const float*)nullptr, n, kH, s->G[1]);
+  if ((e = colsum(s->dA1, nullptr, n, s->ws, s->G[1], st)) != hipSuccess) return e;
   return hipGetLastError();
 }
 
@@ -309,12 +479,20 @@ hipError_t backward(Surrogate* s, int n, hipStream_t st) {
 
 int surrogate_predict(Surrogate* s, const float* q, int n, long long offset, float* prob, hipStream_t st) {
   if (!s || n < 0) return -1;
-  for (int c0 = 0; c0 < n; c0 += s->cap) {
-    const int cn = std::min(s->cap, n - c0);
-    hipLaunchKernelGGL(prep_kernel, dim3((cn + 255) / 256), dim3(256), 0, st, q + c0, offset, cn, s->ndof, s->X,
-                       (const int*)nullptr, (float*)nullptr);
-    if (forward(s, cn, st) != hipSuccess) return -1;
-    if (hipMemcpyAsync(prob + c0, s->p, sizeof(float) * cn, hipMemcpyDeviceToDevice, st) != hipSuccess) return -1;
+  if (n == 0) return 0;
+  static bool attr = false;
+  if (!attr) {
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&mlp_predict_kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPredictLds) != hipSuccess)
+      return -1;
+    attr = true;
+  }
+  // grid.x is limited; 2^30 rows per launch
+  for (long long c0 = 0; c0 < n; c0 += (1LL << 30)) {
+    const int cn = (int)std::min<long long>(1LL << 30, n - c0);
+    hipLaunchKernelGGL(mlp_predict_kernel, dim3((cn + kFR - 1) / kFR), dim3(256), kPredictLds, st, q + c0, offset,
+                       cn, s->ndof, s->W1h, s->P[1], s->W2h, s->P[3], s->W3h, s->P[5], s->P[6], s->P[7], prob + c0);
+    if (hipGetLastError() != hipSuccess) return -1;
   }
   return 0;
 }
@@ -344,7 +522,8 @@ int gprm_surrogate_create(int ndof, int batch, unsigned long long seed, void** o
        dalloc(s, &s->H1, B * kH) && dalloc(s, &s->H2, B * kH) && dalloc(s, &s->H3, B * kH) &&
        dalloc(s, &s->dA1, B * kH) && dalloc(s, &s->dA2, B * kH) && dalloc(s, &s->dA3, B * kH) &&
        dalloc(s, &s->A1, B * kH) && dalloc(s, &s->A2, B * kH) && dalloc(s, &s->A3, B * kH) && dalloc(s, &s->z, B) &&
-       dalloc(s, &s->p, B) && dalloc(s, &s->y, B) && dalloc(s, &s->dz, B) && dalloc(s, &s->loss, 1);
+       dalloc(s, &s->p, B) && dalloc(s, &s->y, B) && dalloc(s, &s->dz, B) && dalloc(s, &s->loss, 1) &&
+       dalloc(s, &s->ws, (size_t)kMaxSplit * kH * kH) && dalloc(s, &s->lrow, B);
   if (!ok) {
     for (void* p : s->allocs) (void)hipFree(p);
     delete s;
