@@ -310,10 +310,11 @@ def leg_prm():
     t_label = time.perf_counter() - t0
     model = gpuprm_amd.Surrogate(ndof=6, batch=B, seed=3)
     ntr = N - ntest
+    epochs = 3
     t0 = time.perf_counter()
-    for ep in range(2):
+    for ep in range(epochs):
         for it in range(ntr // B):
-            model.train_step(Q[:, it * B:(it + 1) * B], codes[it * B:(it + 1) * B], lr=2e-3 if ep == 0 else 7e-4,
+            model.train_step(Q[:, it * B:(it + 1) * B], codes[it * B:(it + 1) * B], lr=2e-3 if ep < 2 else 5e-4,
                              offset=N)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
@@ -330,7 +331,7 @@ def leg_prm():
     flop = 2 * (16 * 256 + 2 * 256 * 256 + 256)
     modes = {"plain": {}, "surrogate": {"order_edges": True, "bias_sampling": True, "edge_budget": 4}}
     res = {k: [] for k in modes}
-    for i, (a, b) in enumerate(_tasks(o2, 6, 4, SEED + 13)):
+    for i, (a, b) in enumerate(_tasks(o2, 6, 8, SEED + 13)):
         for k, kw in modes.items():
             r, path = sp2.prm_solve(a, b, num=1024, D=1.0, max_nodes=1 << 18, max_steps=300, seed=i,
                                     surrogate=model if kw else None, **kw)
@@ -339,8 +340,9 @@ def leg_prm():
                            "ms_model": r.ms_model})
     med = {k: float(np.median([x["ms"] for x in v])) for k, v in res.items()}
     out["surrogate_vs_plain"] = {
-        "workload": "6-DOF arm, 16 hulls of 12 points (edge-checking scene), D=1.0, 1024 new nodes/step, 4 tasks",
-        "labels": N, "label_ms": 1000 * t_label, "train_epochs": 2, "train_s": t_train, "auc": auc,
+        "workload": "6-DOF arm, 16 hulls of 12 points (edge-checking scene), D=1.0, 1024 new nodes/step, 8 tasks",
+        "labels": N, "label_ms": 1000 * t_label, "train_epochs": epochs, "train_batch": B, "train_s": t_train,
+        "train_tflops": epochs * (ntr // B) * B * 3 * flop / t_train / 1e12, "auc": auc,
         "predict_configs_per_s": N / t_pred, "predict_tflops": N * flop / t_pred / 1e12,
         "predict_mfma_frac": N * flop / t_pred / 2.5e15,
         "ms_median": med, "speedup": med["plain"] / med["surrogate"],
