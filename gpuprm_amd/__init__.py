@@ -12,6 +12,7 @@ is never a silent CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import json
 import os
 from typing import Optional
 
@@ -280,14 +281,23 @@ class SceneBatch:
     ndof; they must outlive the batch."""
 
     BUCKETS = 4
+    # per-bucket shapes measured by the autotuner on MI355X (see the file)
+    TUNING_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_mi355x.json")
 
-    def __init__(self, spaces):
+    def __init__(self, spaces, tuning="default"):
+        """tuning: "default" applies the committed MI355X table, None keeps
+        the library's LDS-based defaults, or a table of tuning() rows."""
         self.spaces = list(spaces)
         arr = (C.c_void_p * len(self.spaces))(*[s._ctx.value for s in self.spaces])
         self._h = C.c_void_p()
         rc = lib().gprm_batch_create(arr, len(self.spaces), C.byref(self._h))
         if rc < 0:
             raise GprmError(rc, (lib().gprm_batch_last_error(None) or b"").decode())
+        if tuning == "default":
+            with open(self.TUNING_FILE) as f:
+                tuning = json.load(f)["buckets"]
+        if tuning:
+            self.apply_tuning(tuning)
 
     def _ok(self, rc):
         if rc < 0:
@@ -325,13 +335,17 @@ class SceneBatch:
 
     def apply_tuning(self, table) -> None:
         """Set the shape of every bucket in use whose vertex range a row of
-        `table` names (rows of tuning())."""
+        `table` names (rows of tuning()).  A row whose shape does not fit the
+        bucket's scenes (LDS) is skipped and the bucket keeps its shape."""
         for i, b in enumerate(self.bucket_info()):
             if not b["scenes"]:
                 continue
             for row in table:
                 if (row["min_vertices"], row["max_vertices"]) == (b["min_vertices"], b["max_vertices"]):
-                    self.set_bucket_shape(i, row["block"], row["waves"], row["pairs_per_unit"])
+                    try:
+                        self.set_bucket_shape(i, row["block"], row["waves"], row["pairs_per_unit"])
+                    except GprmError:
+                        pass
 
     def _arrays(self, qs, res):
         S = len(self.spaces)

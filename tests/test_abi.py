@@ -200,3 +200,17 @@ int main(int argc, char** argv) {
     assert init_rc in (0, GPRM_E_HIP)
     assert out[1] == "wrong ndof %d" % GPRM_E_SCENE
     assert out[2] == "missing %d" % GPRM_E_SCENE
+
+
+def test_tuning_table_file():
+    """The committed per-bucket table names only shapes the kernel has and the
+    library's bucket ranges."""
+    import json
+    import gpuprm_amd
+    t = json.load(open(gpuprm_amd.SceneBatch.TUNING_FILE))
+    ranges = {(0, 512), (512, 1024), (1024, 2048), (2048, 1 << 30)}
+    assert t["buckets"]
+    for row in t["buckets"]:
+        assert (row["min_vertices"], row["max_vertices"]) in ranges
+        assert (row["block"], row["waves"]) in ((256, 5), (256, 4), (1024, 4))
+        assert 1 <= row["pairs_per_unit"] <= 256
