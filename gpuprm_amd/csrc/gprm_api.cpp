@@ -269,6 +269,16 @@ int unit_pairs(const gprm_ctx* c, long long n) {
   return (int)want;
 }
 
+// The context's shape with the grid cut to the blocks a launch of `units`
+// work units can use (small launches then dispatch and stage fewer blocks).
+SvaShape launch_shape(const gprm_ctx* c, long long units) {
+  SvaShape sh = c->shape;
+  const long long per_block = std::max(1, sh.block / 64);
+  const long long need = std::max(1LL, (units + per_block - 1) / per_block);
+  if (need < sh.grid) sh.grid = (int)need;
+  return sh;
+}
+
 // Items of one launch: at most kItemBudget, and few enough that the unit
 // count stays below 2^31 even with the smallest units (8 pairs).
 int item_budget(const gprm_ctx* c) {
@@ -309,7 +319,8 @@ int run_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n,
     a.tally = tally_ptr(c, sl);
     a.static_first = 1;
     GPRM_HIP(prof_begin(c, sl, 0, st, &ev), "event");
-    GPRM_HIP(launch_sva(a, 0, c->shape, st), "separating-vector launch");
+    GPRM_HIP(launch_sva(a, 0, launch_shape(c, (long long)((cn + 63) / 64) * a.units_per_item), st),
+             "separating-vector launch");
     GPRM_HIP(prof_end(c, sl, ev, st), "event");
     GPRM_HIP(launch_finalize_points(sl.status.as<int>(), d_res + c0, cn, st), "finalize launch");
   }
@@ -402,7 +413,8 @@ int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long 
     a.tally = tally_ptr(c, sl);
     a.static_first = 1;
     GPRM_HIP(prof_begin(c, sl, 0, st, &ev), "event");
-    GPRM_HIP(launch_sva(a, 1, c->shape, st), "separating-vector launch");
+    GPRM_HIP(launch_sva(a, 1, launch_shape(c, (long long)((pcap + 63) / 64) * a.units_per_item), st),
+             "separating-vector launch");
     GPRM_HIP(prof_end(c, sl, ev, st), "event");
   }
   if (stats) {

@@ -16,6 +16,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/gprm.h"
@@ -480,13 +481,14 @@ hipError_t backward(Surrogate* s, int n, hipStream_t st) {
 int surrogate_predict(Surrogate* s, const float* q, int n, long long offset, float* prob, hipStream_t st) {
   if (!s || n < 0) return -1;
   if (n == 0) return 0;
-  static bool attr = false;
-  if (!attr) {
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&mlp_predict_kernel),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPredictLds) != hipSuccess)
-      return -1;
-    attr = true;
-  }
+  // the kernel's dynamic LDS exceeds the 64 KB default: raise its limit once
+  static std::once_flag once;
+  static hipError_t attr = hipSuccess;
+  std::call_once(once, [] {
+    attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&mlp_predict_kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPredictLds);
+  });
+  if (attr != hipSuccess) return -1;
   // grid.x is limited; 2^30 rows per launch
   for (long long c0 = 0; c0 < n; c0 += (1LL << 30)) {
     const int cn = (int)std::min<long long>(1LL << 30, n - c0);
