@@ -21,6 +21,7 @@
 
 #include "../../include/gprm.h"
 #include "context.h"
+#include "glibc_trig.h"
 #include "runtime.h"
 #include "scene.h"
 
@@ -66,7 +67,11 @@ void fill_robot(const Scene& s, DevRobot* rb) {
   for (int i = 0; i < s.ndof; ++i) {
     rb->a[i] = s.a[i]; rb->alpha[i] = s.alpha[i]; rb->q[i] = s.q[i]; rb->d[i] = s.d[i];
     rb->mins[i] = s.mins[i]; rb->maxs[i] = s.maxs[i]; rb->types[i] = s.types[i];
+    rb->ca[i] = gprm_trig::cosf_ref(s.alpha[i]);
+    rb->sa[i] = gprm_trig::sinf_ref(s.alpha[i]);
   }
+  rb->c0 = gprm_trig::cosf_ref(0.0f);
+  rb->s0 = gprm_trig::sinf_ref(0.0f);
 }
 
 size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }

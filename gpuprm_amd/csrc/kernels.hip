@@ -91,10 +91,10 @@ __device__ __forceinline__ V3 t_apply_rot_inv(const T12& t, const V3& u) {
   r.z = (c[6] * u.x + c[7] * u.y) + c[8] * u.z;
   return r;
 }
-// trafo4::set(a, alpha, q, d) (geo4.h:194-203)
-__device__ __forceinline__ T12 t_set(float a, float alpha, float q, float d) {
+// trafo4::set(a, alpha, q, d) (geo4.h:194-203), with ca = cos(alpha) and
+// sa = sin(alpha) given (the twist is constant per joint; DevRobot holds them)
+__device__ __forceinline__ T12 t_set_cs(float a, float ca, float sa, float q, float d) {
   float cq = gprm_trig::cosf_ref(q), sq = gprm_trig::sinf_ref(q);
-  float ca = gprm_trig::cosf_ref(alpha), sa = gprm_trig::sinf_ref(alpha);
   T12 t;
   t.m[0] = cq;      t.m[3] = -sq;     t.m[6] = 0.0f;  t.m[9] = a;
   t.m[1] = sq * ca; t.m[4] = cq * ca; t.m[7] = -sa;   t.m[10] = -sa * d;
@@ -121,14 +121,21 @@ template <int NDOF>
 __device__ __forceinline__ void fk_store(const DevRobot& rb, const float* q, float* __restrict__ out, int item,
                                          long long stride) {
   const int nd = NDOF > 0 ? NDOF : rb.ndof;
-  T12 T = t_set(0.0f, 0.0f, 0.0f, 0.0f);
+  // T0 = set(0, 0, 0, 0) (kinematics.h:27-37), from cos/sin(0) held in rb
+  T12 T;
+  {
+    const float cq = rb.c0, sq = rb.s0, ca = rb.c0, sa = rb.s0, a = 0.0f, d = 0.0f;
+    T.m[0] = cq;      T.m[3] = -sq;     T.m[6] = 0.0f;  T.m[9] = a;
+    T.m[1] = sq * ca; T.m[4] = cq * ca; T.m[7] = -sa;   T.m[10] = -sa * d;
+    T.m[2] = sq * sa; T.m[5] = cq * sa; T.m[8] = ca;    T.m[11] = ca * d;
+  }
 #pragma unroll
   for (int i = 0; i < (NDOF > 0 ? NDOF : kMaxDofDev); ++i) {
     if (NDOF == 0 && i >= nd) break;
     float qset = rb.q[i], dset = rb.d[i];
     if (rb.types[i] == 0) qset = q[i];
     else dset = q[i];
-    T12 A = t_set(rb.a[i], rb.alpha[i], qset, dset);
+    T12 A = t_set_cs(rb.a[i], rb.ca[i], rb.sa[i], qset, dset);
     T = t_lapply(A, T);
     float4* o = reinterpret_cast<float4*>(out + ((size_t)i * stride + item) * 12);
     o[0] = make_float4(T.m[0], T.m[1], T.m[2], T.m[3]);

@@ -17,6 +17,11 @@ struct DevRobot {
   float a[kMaxDofDev], alpha[kMaxDofDev], q[kMaxDofDev], d[kMaxDofDev];
   float mins[kMaxDofDev], maxs[kMaxDofDev];
   int types[kMaxDofDev];
+  // cos/sin of the constant twists alpha and of 0 (for T0 = set(0,0,0,0)),
+  // evaluated once on the host by the same restated glibc routines the
+  // kernels use, so the bits are those the kernels would compute
+  float ca[kMaxDofDev], sa[kMaxDofDev];
+  float c0, s0;
 };
 
 // One contiguous device image of a scene, staged whole into LDS by the
