@@ -306,7 +306,7 @@ int gprm_batch_create(gprm_ctx* const* ctxs, int nscenes, gprm_batch** out) {
     if (bk.scenes.empty()) continue;
     // default: four scene copies per CU when LDS holds them, else one
     const size_t cap = (size_t)sva_max_lds_bytes() - 16;
-    if (4 * (bk.lds + 16 * 256) <= cap) { bk.shape.block = 256; bk.shape.waves = 4; }
+    if (4 * (bk.lds + sva_ring_bytes(256)) <= cap) { bk.shape.block = 256; bk.shape.waves = 4; }
     else { bk.shape.block = 1024; bk.shape.waves = 4; }
     bk.shape.ppu = 64;
     e = sva_multi_shape_finish(bk.lds, &bk.shape, dev);

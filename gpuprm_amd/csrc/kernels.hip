@@ -629,10 +629,22 @@ __device__ __forceinline__ Lds stage_scene(const SceneImg& img, int4* smem) {
   return L;
 }
 
-// Deferred pair tests of one wave: (item, pair) entries in LDS, a ring of
-// kDeferCap slots.  Phase B starts as soon as 64 entries wait, so the ring
-// never holds more than 127.
-constexpr int kDeferCap = 128;
+// Deferred pair tests of one wave: a ring of CAP = sva_ring_cap(BLOCK)
+// entries in LDS, (item, pair) in ent and the item's edge (edge mode) in
+// edge.  Phase B starts as soon as CAP - 64 entries wait, so the ring never
+// holds more than CAP - 1.
+struct Ring {
+  int2* ent;
+  int* edge;
+};
+template <int BLOCK>
+__device__ __forceinline__ Ring wave_ring(int4* smem) {
+  constexpr int CAP = sva_ring_cap(BLOCK);
+  int2* const ent = reinterpret_cast<int2*>(smem);
+  int* const edge = reinterpret_cast<int*>(ent + (BLOCK / 64) * CAP);
+  const int w = threadIdx.x >> 6;
+  return Ring{ent + w * CAP, edge + w * CAP};
+}
 
 // Work units of a separating-vector launch: (chunk of pairs_per_unit pairs,
 // group of 64 consecutive items), numbered chunk-major.
@@ -644,8 +656,8 @@ __device__ __forceinline__ unsigned sva_unit_count(const SvaArgs& A, int nitems)
 // ring and runs the full separating-vector test (collision4.h:227-332) until
 // the ring is empty.  The gate is re-read when a pair is taken.  Clobbers
 // the frame registers tp, tq.
-template <int EDGE>
-__device__ __forceinline__ void sva_drain(const SvaArgs& A, const Lds& L, int2* __restrict__ dq,
+template <int EDGE, int CAP>
+__device__ __forceinline__ void sva_drain(const SvaArgs& A, const Lds& L, const Ring& R,
                                           float4* __restrict__ H, size_t hs, unsigned& d_head, unsigned d_tail,
                                           T12& tp, T12& tq, unsigned& n_sep, unsigned& n_col, unsigned& n_dropped,
                                           unsigned& n_iter) {
@@ -662,28 +674,32 @@ __device__ __forceinline__ void sva_drain(const SvaArgs& A, const Lds& L, int2* 
     const unsigned avail = d_tail - d_head;
     const unsigned take = (unsigned)__popcll(wm) < avail ? (unsigned)__popcll(wm) : avail;
     if (!busy && (unsigned)__popcll(wm & lt_mask) < take) {
-      const int2 e = dq[(d_head + (unsigned)__popcll(wm & lt_mask)) & (kDeferCap - 1)];
+      const unsigned slot = (d_head + (unsigned)__popcll(wm & lt_mask)) & (CAP - 1);
+      const int2 e = R.ent[slot];
       bitem = e.x;
       const int4 pd = L.pairs[e.y];
       crs = pd.z;
-      bool drop;
+      // the gate and the frames are loaded together (one memory round
+      // trip); the frames of a pair that turns out dropped are not used
+      int gv;
       if (EDGE) {
-        bedge = A.item_edge[bitem];
-        drop = __hip_atomic_load(&A.edge_res[bedge], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+        bedge = R.edge[slot];
+        gv = __hip_atomic_load(&A.edge_res[bedge], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       } else {
-        drop = (__hip_atomic_load(&A.word[bitem], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 4) <= crs;
+        gv = __hip_atomic_load(&A.word[bitem], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
+      vbP = pd.x & 0xffff;
+      vbQ = pd.x >> 16;
+      const int sP = pd.y & 255, sQ = pd.y >> 8;
+      sysP0 = sP == 0;
+      sysQ0 = sQ == 0;
+      if (!sysP0) tp = load_frame(A.frames, bitem, A.fstride, sP);
+      if (!sysQ0) tq = load_frame(A.frames, bitem, A.fstride, sQ);
+      const bool drop = EDGE ? gv != 0 : (gv >> 4) <= crs;
       if (drop) {
         ++n_dropped;
       } else {
         // start of the pair (collision4.h:228-244)
-        vbP = pd.x & 0xffff;
-        vbQ = pd.x >> 16;
-        const int sP = pd.y & 255, sQ = pd.y >> 8;
-        sysP0 = sP == 0;
-        sysQ0 = sQ == 0;
-        if (!sysP0) tp = load_frame(A.frames, bitem, A.fstride, sP);
-        if (!sysQ0) tq = load_frame(A.frames, bitem, A.fstride, sQ);
         const V3 vq = v3(L.verts[vbQ]), vp = v3(L.verts[vbP]);
         const V3 Pq = sysQ0 ? vq : t_apply(tq, vq);
         const V3 Pp = sysP0 ? vp : t_apply(tp, vp);
@@ -761,8 +777,8 @@ __device__ __forceinline__ void sva_drain(const SvaArgs& A, const Lds& L, int2* 
 // H: this lane's iteration history (stride hs); dq: the wave's ring.  Waves
 // run independently: there is no block barrier inside.  cnt.claimed counts
 // the wave's units (the same in every lane).
-template <int EDGE>
-__device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, int2* __restrict__ dq,
+template <int EDGE, int CAP>
+__device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, const Ring& R,
                                           float4* __restrict__ H, size_t hs, SvaCounts& cnt) {
   const int lane = threadIdx.x & 63;
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
@@ -788,7 +804,9 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, int2* 
   // claiming, so that waves finding the queue drained leave without an
   // atomic.
   const unsigned nwaves = (gridDim.x * blockDim.x) >> 6;
-  const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  // wave-uniform values held in scalar registers: the unit index, and from
+  // it the pair loop, the pair records and the cache tags below
+  const unsigned wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   unsigned nstatic = 0;
   if (A.static_first) {
     const unsigned rounds = nunits / nwaves;
@@ -816,7 +834,7 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, int2* 
                                 nunits - nstatic;
         if (go) b = atomicAdd(A.counter, k);
       }
-      b = __shfl(b, 0, 64);
+      b = __builtin_amdgcn_readfirstlane(b);
       if (b == 0xffffffffu || b >= nunits - nstatic) break;
       seen = b + k;
       cur = nstatic + b;
@@ -845,9 +863,12 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, int2* 
     int fsP = -1, fsQ = -1, ckQ = -1, cvQ = -1;
     V3 cQw{0, 0, 0}, PqW{0, 0, 0};
     V3 aQ0{0, 0, 0}, aQ1{0, 0, 0}, aQ2{0, 0, 0};  // Q's box axes in the world frame, times its half extents
-    float rQ = 0.0f, magQ = 0.0f;
+    float magQ = 0.0f;
     for (int pair = p0; pair < p1; ++pair) {
-      const int4 pd = L.pairs[pair];
+      // the record is the same in every lane: keep it in scalar registers
+      const int4 pdv = L.pairs[pair];
+      const int4 pd = make_int4(__builtin_amdgcn_readfirstlane(pdv.x), __builtin_amdgcn_readfirstlane(pdv.y),
+                                __builtin_amdgcn_readfirstlane(pdv.z), __builtin_amdgcn_readfirstlane(pdv.w));
       const int crs = pd.z;
       const int vbP = pd.x & 0xffff, vbQ = pd.x >> 16;
       const int sP = pd.y & 255, sQ = pd.y >> 8;
@@ -870,8 +891,7 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, int2* 
       if (kQ != ckQ) {
         const float4 b0 = L.spheres[2 * kQ], b1 = L.spheres[2 * kQ + 1];
         cQw = sysQ0 ? v3(b0) : t_apply(tq, v3(b0));
-        rQ = b0.w;
-        magQ = fabsf(cQw.x) + fabsf(cQw.y) + fabsf(cQw.z) + rQ;
+        magQ = fabsf(cQw.x) + fabsf(cQw.y) + fabsf(cQw.z) + b0.w;
         if (sysQ0) {
           aQ0 = V3{b1.x, 0.0f, 0.0f};
           aQ1 = V3{0.0f, b1.y, 0.0f};
@@ -894,10 +914,10 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, int2* 
           // of iteration 0: iteration 0 returns 0 whenever
           // S.normalize(q0 - p0) >= -1e-6 for the support vertices p0, q0
           // the climbs find, S = normalize(D) with D the difference of the
-          // two start vertices.  If the bounds (sphere or box) show
-          // S.(q - p) > 0 with a wide margin for every vertex p of P and q of
-          // Q, that holds whichever vertices the climbs would pick, so the
-          // pair test ends here exactly as the reference's does (code 0, one
+          // two start vertices.  If the bounding boxes show S.(q - p) > 0
+          // with a wide margin for every vertex p of P and q of Q, that holds
+          // whichever vertices the climbs would pick, so the pair test ends
+          // here exactly as the reference's does (code 0, one
           // iteration).  The margin, 2^-10 of the magnitudes involved,
           // exceeds the rounding error of every quantity in the test and in
           // the reference's iteration by orders of magnitude.  Both sides of
@@ -909,14 +929,14 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, int2* 
           const V3 vp = v3(L.verts[vbP]);
           const V3 Pp = sysP0 ? vp : t_apply(tp, vp);
           const V3 S{PqW.x - Pp.x, PqW.y - Pp.y, PqW.z - Pp.z};
-          // support extents along S: the smaller of the sphere's |S| r and
-          // the box's sum_a |S.axis_a| h_a (axes: the frame's rotation
-          // columns); |S| from the hardware square root, whose error is far
-          // inside the margin.  The magnitude term of the margin is |c|_1 + r
-          // per side (precomputed for world-frame polytopes).
+          // support extents along S: the box's sum_a |S.axis_a| h_a (axes:
+          // the frame's rotation columns).  The bounding sphere's |S| r is
+          // not used: on the benchmark scenes it decides under 0.1% more pair
+          // tests than the box alone, for a square root and five operations
+          // per pair.  The magnitude term of the margin is |c|_1 + r per side
+          // (precomputed for world-frame polytopes).
           const float4 b0 = L.spheres[2 * kP], b1 = L.spheres[2 * kP + 1];
           const V3 cPw = sysP0 ? v3(b0) : t_apply(tp, v3(b0));
-          const float nS = __builtin_amdgcn_sqrtf(dot3(S, S));
           const float boxQ = (fabsf(dot3(S, aQ0)) + fabsf(dot3(S, aQ1))) + fabsf(dot3(S, aQ2));
           float boxP, magP;
           if (sysP0) {
@@ -927,10 +947,8 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, int2* 
             boxP = (fabsf(sp.x) * b1.x + fabsf(sp.y) * b1.y) + fabsf(sp.z) * b1.z;
             magP = fabsf(cPw.x) + fabsf(cPw.y) + fabsf(cPw.z) + b0.w;
           }
-          const float extQ = fminf(nS * rQ, boxQ);
-          const float extP = fminf(nS * b0.w, boxP);
           const float sn1 = fabsf(S.x) + fabsf(S.y) + fabsf(S.z);
-          const float gap = dot3(S, V3{cQw.x - cPw.x, cQw.y - cPw.y, cQw.z - cPw.z}) - extQ - extP;
+          const float gap = dot3(S, V3{cQw.x - cPw.x, cQw.y - cPw.y, cQw.z - cPw.z}) - boxQ - boxP;
           const float mag = magQ + magP;
           if (sn1 > 1e-5f && gap > 0x1p-10f * mag * sn1) {
             ++n_bound;
@@ -940,18 +958,22 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, int2* 
           }
         }
       }
-      // append the undecided pairs to the ring; drain it once 64 wait
+      // append the undecided pairs to the ring; drain it once CAP - 64 wait
       const unsigned long long dm = __ballot(defer);
-      if (defer) dq[(d_tail + (unsigned)__popcll(dm & lt_mask)) & (kDeferCap - 1)] = make_int2(item, pair);
+      if (defer) {
+        const unsigned slot = (d_tail + (unsigned)__popcll(dm & lt_mask)) & (CAP - 1);
+        R.ent[slot] = make_int2(item, pair);
+        if (EDGE) R.edge[slot] = edge;
+      }
       d_tail += (unsigned)__popcll(dm);
-      if (d_tail - d_head >= 64) {
-        sva_drain<EDGE>(A, L, dq, H, hs, d_head, d_tail, tp, tq, n_sep, n_col, n_dropped, n_iter);
+      if (d_tail - d_head >= CAP - 64) {
+        sva_drain<EDGE, CAP>(A, L, R, H, hs, d_head, d_tail, tp, tq, n_sep, n_col, n_dropped, n_iter);
         fsP = fsQ = ckQ = cvQ = -1;  // the frame registers were clobbered
       }
     }
     ++n_units;
   }
-  if (d_tail != d_head) sva_drain<EDGE>(A, L, dq, H, hs, d_head, d_tail, tp, tq, n_sep, n_col, n_dropped, n_iter);
+  if (d_tail != d_head) sva_drain<EDGE, CAP>(A, L, R, H, hs, d_head, d_tail, tp, tq, n_sep, n_col, n_dropped, n_iter);
 
   cnt.sep += n_sep;
   cnt.col += n_col;
@@ -987,15 +1009,15 @@ __device__ __forceinline__ void add_tally(const SvaCounts& c, int4* smem, unsign
 template <int EDGE, int BLOCK, int WAVES>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) sva_kernel(SvaArgs A) {
   extern __shared__ int4 smem[];
-  // dynamic LDS: the waves' deferral rings (BLOCK/64 x kDeferCap int2 =
-  // BLOCK int4), then the scene image
-  const Lds L = stage_scene<BLOCK>(A.img, smem + BLOCK);
-  int2* const dq = reinterpret_cast<int2*>(smem) + (threadIdx.x >> 6) * kDeferCap;
+  // dynamic LDS: the waves' deferral rings (sva_ring_bytes), then the
+  // scene image
+  const Lds L = stage_scene<BLOCK>(A.img, smem + sva_ring_bytes(BLOCK) / 16);
+  const Ring R = wave_ring<BLOCK>(smem);
   __syncthreads();
   float4* __restrict__ H = A.hist + (size_t)blockIdx.x * BLOCK + threadIdx.x;
   const size_t hs = (size_t)gridDim.x * BLOCK;
   SvaCounts c;
-  sva_units<EDGE>(A, L, dq, H, hs, c);
+  sva_units<EDGE, sva_ring_cap(BLOCK)>(A, L, R, H, hs, c);
   if (A.tally != nullptr) add_tally<BLOCK>(c, smem, A.tally);
 }
 
@@ -1015,7 +1037,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVE
   float4* __restrict__ H = tasks[0].a.hist + (size_t)blockIdx.x * BLOCK + threadIdx.x;
   const size_t hs = (size_t)gridDim.x * BLOCK;
   // dynamic LDS as in sva_kernel: deferral rings, then the scene image
-  int2* const dq = reinterpret_cast<int2*>(smem) + (threadIdx.x >> 6) * kDeferCap;
+  const Ring R = wave_ring<BLOCK>(smem);
   SvaCounts c;
   for (int r = 0; r < ntasks; ++r) {
     const int t = (int)((blockIdx.x + (unsigned)r) % (unsigned)ntasks);
@@ -1026,10 +1048,10 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVE
       s_live = __hip_atomic_load(A.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < nunits ? 1 : 0;
     __syncthreads();
     if (s_live) {
-      const Lds L = stage_scene<BLOCK>(A.img, smem + BLOCK);
+      const Lds L = stage_scene<BLOCK>(A.img, smem + sva_ring_bytes(BLOCK) / 16);
       __syncthreads();
       const unsigned before = c.claimed;
-      sva_units<EDGE>(A, L, dq, H, hs, c);
+      sva_units<EDGE, sva_ring_cap(BLOCK)>(A, L, R, H, hs, c);
       const unsigned mine = c.claimed - before;  // the wave's units (the same in every lane)
       if ((threadIdx.x & 63) == 0 && mine > 0) {
         const unsigned old = atomicAdd(tasks[t].done, mine);
@@ -1091,7 +1113,7 @@ __global__ void finalize_points_multi_kernel(const FkTask* __restrict__ tasks, i
 
 // Dynamic LDS of a separating-vector block: the deferral rings (16 B per
 // thread) and the scene image.
-size_t sva_lds_bytes(const SceneImg& img, int block) { return (size_t)img.bytes + 16 * (size_t)block; }
+size_t sva_lds_bytes(const SceneImg& img, int block) { return (size_t)img.bytes + sva_ring_bytes(block); }
 
 int sva_max_lds_bytes() { return 160 * 1024; }
 
@@ -1178,7 +1200,7 @@ static hipError_t resident_grid(const void* const fns[2], int block, size_t lds,
 }
 
 hipError_t sva_multi_shape_finish(size_t img_bytes, SvaShape* sh, int device) {
-  const size_t lds = img_bytes + 16 * (size_t)sh->block;
+  const size_t lds = img_bytes + sva_ring_bytes(sh->block);
   if (lds > (size_t)sva_max_lds_bytes() - 16) return hipErrorInvalidValue;
   if (sh->ppu < 1 || sh->ppu > 256) return hipErrorInvalidValue;
   const void* fns[2] = {sva_multi_variant(0, sh->block, sh->waves), sva_multi_variant(1, sh->block, sh->waves)};
@@ -1222,7 +1244,7 @@ hipError_t launch_sva(const SvaArgs& a, int edge_mode, const SvaShape& sh, hipSt
 hipError_t launch_sva_multi(const SceneTask* d_tasks, int ntasks, int edge_mode, const SvaShape& sh, size_t img_bytes,
                             unsigned long long* t_start, unsigned long long* tally, hipStream_t st) {
   if (ntasks <= 0) return hipSuccess;
-  size_t lds = img_bytes + 16 * (size_t)sh.block;
+  size_t lds = img_bytes + sva_ring_bytes(sh.block);
   if (lds > (size_t)sva_max_lds_bytes() - 16 || sh.grid < 1) return hipErrorInvalidValue;
   const void* fn = sva_multi_variant(edge_mode, sh.block, sh.waves);
   if (!fn) return hipErrorInvalidValue;

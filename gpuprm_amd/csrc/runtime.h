@@ -113,6 +113,12 @@ struct EdgeArgs {
   int* edge_res;           // [edge]
 };
 
+// Deferral rings of the separating-vector kernel: per wave, a ring of
+// sva_ring_cap(block) entries of 12 bytes (item, pair, edge).  Larger rings
+// start the full tests in larger batches, which shortens their tails; the
+// 1024-thread variant keeps smaller rings to leave LDS to large scenes.
+constexpr int sva_ring_cap(int block) { return block >= 1024 ? 128 : 256; }
+constexpr size_t sva_ring_bytes(int block) { return (size_t)(block / 64) * (size_t)sva_ring_cap(block) * 12; }
 // Dynamic LDS of a separating-vector block of `block` threads on this scene.
 size_t sva_lds_bytes(const SceneImg& img, int block);
 int sva_max_lds_bytes();
