@@ -654,18 +654,19 @@ __device__ __forceinline__ unsigned sva_unit_count(const SvaArgs& A, int nitems)
 
 // Phase B of sva_units: every lane takes a deferred pair from the wave's
 // ring and runs the full separating-vector test (collision4.h:227-332) until
-// the ring is empty.  The gate is re-read when a pair is taken.  Clobbers
-// the frame registers tp, tq.
+// the ring is empty.  The gate is re-read when a pair is taken.
 template <int EDGE, int CAP>
 __device__ __forceinline__ void sva_drain(const SvaArgs& A, const Lds& L, const Ring& R,
                                           float4* __restrict__ H, size_t hs, unsigned& d_head, unsigned d_tail,
-                                          T12& tp, T12& tq, unsigned& n_sep, unsigned& n_col, unsigned& n_dropped,
-                                          unsigned& n_iter) {
+                                          unsigned& n_sep, unsigned& n_col, unsigned& n_dropped, unsigned& n_iter) {
   const int lane = threadIdx.x & 63;
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
   // the ring's entries were written by other lanes of this wave
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   bool busy = false;
+  T12 tp, tq;
+#pragma unroll
+  for (int i = 0; i < 12; ++i) tp.m[i] = tq.m[i] = 0.0f;
   int bitem = 0, bedge = 0, crs = 0, k = 0, l = 0, p = 0, q = 0, vbP = 0, vbQ = 0;
   bool sysP0 = true, sysQ0 = true;
   V3 S{0, 0, 0}, w{0, 0, 0};
@@ -788,9 +789,6 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, const 
   const unsigned ngroups = (unsigned)((nitems + 63) >> 6);
   const unsigned nunits = sva_unit_count(A, nitems);
 
-  T12 tp, tq;
-#pragma unroll
-  for (int i = 0; i < 12; ++i) tp.m[i] = tq.m[i] = 0.0f;
   unsigned d_head = 0, d_tail = 0;  // the ring's cursors (the same in every lane)
   unsigned n_sep = 0, n_col = 0, n_dropped = 0, n_units = 0, n_iter = 0, n_bound = 0, n_claimed = 0;
 
@@ -858,122 +856,134 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, const 
       if (EDGE) edge = A.item_edge[item];
       else gate = __hip_atomic_load(&A.word[item], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 4;
     }
-    // wave-uniform caches: the frames of links fsP, fsQ; Q's world bounds for
-    // polytope ckQ and its start vertex cvQ
-    int fsP = -1, fsQ = -1, ckQ = -1, cvQ = -1;
-    V3 cQw{0, 0, 0}, PqW{0, 0, 0};
-    V3 aQ0{0, 0, 0}, aQ1{0, 0, 0}, aQ2{0, 0, 0};  // Q's box axes in the world frame, times its half extents
-    float magQ = 0.0f;
-    for (int pair = p0; pair < p1; ++pair) {
-      // the record is the same in every lane: keep it in scalar registers
-      const int4 pdv = L.pairs[pair];
-      const int4 pd = make_int4(__builtin_amdgcn_readfirstlane(pdv.x), __builtin_amdgcn_readfirstlane(pdv.y),
-                                __builtin_amdgcn_readfirstlane(pdv.z), __builtin_amdgcn_readfirstlane(pdv.w));
-      const int crs = pd.z;
-      const int vbP = pd.x & 0xffff, vbQ = pd.x >> 16;
-      const int sP = pd.y & 255, sQ = pd.y >> 8;
-      const bool sysP0 = sP == 0, sysQ0 = sQ == 0;
-      if (!sysP0 && sP != fsP) {
-        if (active) tp = load_frame(A.frames, item, A.fstride, sP);
-        fsP = sP;
-      }
-      if (!sysQ0 && sQ != fsQ) {
-        if (active) tq = load_frame(A.frames, item, A.fstride, sQ);
-        fsQ = sQ;
-        ckQ = cvQ = -1;
-      }
-      const int kP = pd.w & 0xffff, kQ = pd.w >> 16;
-      if (vbQ != cvQ) {
-        const V3 vq = v3(L.verts[vbQ]);
-        PqW = sysQ0 ? vq : t_apply(tq, vq);
-        cvQ = vbQ;
-      }
-      if (kQ != ckQ) {
-        const float4 b0 = L.spheres[2 * kQ], b1 = L.spheres[2 * kQ + 1];
-        cQw = sysQ0 ? v3(b0) : t_apply(tq, v3(b0));
-        magQ = fabsf(cQw.x) + fabsf(cQw.y) + fabsf(cQw.z) + b0.w;
-        if (sysQ0) {
-          aQ0 = V3{b1.x, 0.0f, 0.0f};
-          aQ1 = V3{0.0f, b1.y, 0.0f};
-          aQ2 = V3{0.0f, 0.0f, b1.z};
-        } else {
-          aQ0 = V3{tq.m[0] * b1.x, tq.m[1] * b1.x, tq.m[2] * b1.x};
-          aQ1 = V3{tq.m[3] * b1.y, tq.m[4] * b1.y, tq.m[5] * b1.y};
-          aQ2 = V3{tq.m[6] * b1.z, tq.m[7] * b1.z, tq.m[8] * b1.z};
+    // The unit's pairs run in segments that end where the ring is drained.
+    // The caches are local to a segment, so nothing of them is live across
+    // a drain, which has registers of its own.
+    int pair = p0;
+    while (pair < p1) {
+      // wave-uniform caches: the frames of links fsP, fsQ; Q's world bounds for
+      // polytope ckQ and its start vertex cvQ
+      T12 tp, tq;
+#pragma unroll
+      for (int i = 0; i < 12; ++i) tp.m[i] = tq.m[i] = 0.0f;
+      int fsP = -1, fsQ = -1, ckQ = -1, cvQ = -1;
+      V3 cQw{0, 0, 0}, PqW{0, 0, 0};
+      // Q's box axes in the world frame, times its half extents
+      V3 aQ0{0, 0, 0}, aQ1{0, 0, 0}, aQ2{0, 0, 0};
+      float magQ = 0.0f;
+      for (; pair < p1; ++pair) {
+        // the record is the same in every lane: keep it in scalar registers
+        const int4 pdv = L.pairs[pair];
+        const int4 pd = make_int4(__builtin_amdgcn_readfirstlane(pdv.x), __builtin_amdgcn_readfirstlane(pdv.y),
+                                  __builtin_amdgcn_readfirstlane(pdv.z), __builtin_amdgcn_readfirstlane(pdv.w));
+        const int crs = pd.z;
+        const int vbP = pd.x & 0xffff, vbQ = pd.x >> 16;
+        const int sP = pd.y & 255, sQ = pd.y >> 8;
+        const bool sysP0 = sP == 0, sysQ0 = sQ == 0;
+        if (!sysP0 && sP != fsP) {
+          if (active) tp = load_frame(A.frames, item, A.fstride, sP);
+          fsP = sP;
         }
-        ckQ = kQ;
-      }
-      if (EDGE && active && pair == p0 + 2)
-        gate = __hip_atomic_load(&A.edge_res[edge], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ? 0 : 0x7fffffff;
-      bool defer = false;
-      if (active) {
-        if (pair >= p0 + 4 && (EDGE ? gate == 0 : gate <= crs)) {
-          ++n_dropped;
-        } else {
-          // Start of the pair (collision4.h:228-244) and the bound decision
-          // of iteration 0: iteration 0 returns 0 whenever
-          // S.normalize(q0 - p0) >= -1e-6 for the support vertices p0, q0
-          // the climbs find, S = normalize(D) with D the difference of the
-          // two start vertices.  If the bounding boxes show S.(q - p) > 0
-          // with a wide margin for every vertex p of P and q of Q, that holds
-          // whichever vertices the climbs would pick, so the pair test ends
-          // here exactly as the reference's does (code 0, one
-          // iteration).  The margin, 2^-10 of the magnitudes involved,
-          // exceeds the rounding error of every quantity in the test and in
-          // the reference's iteration by orders of magnitude.  Both sides of
-          // the test scale with |S|, so it runs on D itself: the reference's
-          // S differs from D/|D| by a relative 1e-7, far inside the margin.
-          // The test also requires |D|_1 > 1e-5, so |D|_2 > 5.7e-6 stays
-          // clear of the reference's override of S for |D| <= 1e-6; a NaN D
-          // never passes (such pairs are deferred to the exact test).
-          const V3 vp = v3(L.verts[vbP]);
-          const V3 Pp = sysP0 ? vp : t_apply(tp, vp);
-          const V3 S{PqW.x - Pp.x, PqW.y - Pp.y, PqW.z - Pp.z};
-          // support extents along S: the box's sum_a |S.axis_a| h_a (axes:
-          // the frame's rotation columns).  The bounding sphere's |S| r is
-          // not used: on the benchmark scenes it decides under 0.1% more pair
-          // tests than the box alone, for a square root and five operations
-          // per pair.  The magnitude term of the margin is |c|_1 + r per side
-          // (precomputed for world-frame polytopes).
-          const float4 b0 = L.spheres[2 * kP], b1 = L.spheres[2 * kP + 1];
-          const V3 cPw = sysP0 ? v3(b0) : t_apply(tp, v3(b0));
-          const float boxQ = (fabsf(dot3(S, aQ0)) + fabsf(dot3(S, aQ1))) + fabsf(dot3(S, aQ2));
-          float boxP, magP;
-          if (sysP0) {
-            boxP = (fabsf(S.x) * b1.x + fabsf(S.y) * b1.y) + fabsf(S.z) * b1.z;
-            magP = b1.w;
+        if (!sysQ0 && sQ != fsQ) {
+          if (active) tq = load_frame(A.frames, item, A.fstride, sQ);
+          fsQ = sQ;
+          ckQ = cvQ = -1;
+        }
+        const int kP = pd.w & 0xffff, kQ = pd.w >> 16;
+        if (vbQ != cvQ) {
+          const V3 vq = v3(L.verts[vbQ]);
+          PqW = sysQ0 ? vq : t_apply(tq, vq);
+          cvQ = vbQ;
+        }
+        if (kQ != ckQ) {
+          const float4 b0 = L.spheres[2 * kQ], b1 = L.spheres[2 * kQ + 1];
+          cQw = sysQ0 ? v3(b0) : t_apply(tq, v3(b0));
+          magQ = fabsf(cQw.x) + fabsf(cQw.y) + fabsf(cQw.z) + b0.w;
+          if (sysQ0) {
+            aQ0 = V3{b1.x, 0.0f, 0.0f};
+            aQ1 = V3{0.0f, b1.y, 0.0f};
+            aQ2 = V3{0.0f, 0.0f, b1.z};
           } else {
-            const V3 sp = t_apply_rot_inv(tp, S);
-            boxP = (fabsf(sp.x) * b1.x + fabsf(sp.y) * b1.y) + fabsf(sp.z) * b1.z;
-            magP = fabsf(cPw.x) + fabsf(cPw.y) + fabsf(cPw.z) + b0.w;
+            aQ0 = V3{tq.m[0] * b1.x, tq.m[1] * b1.x, tq.m[2] * b1.x};
+            aQ1 = V3{tq.m[3] * b1.y, tq.m[4] * b1.y, tq.m[5] * b1.y};
+            aQ2 = V3{tq.m[6] * b1.z, tq.m[7] * b1.z, tq.m[8] * b1.z};
           }
-          const float sn1 = fabsf(S.x) + fabsf(S.y) + fabsf(S.z);
-          const float gap = dot3(S, V3{cQw.x - cPw.x, cQw.y - cPw.y, cQw.z - cPw.z}) - boxQ - boxP;
-          const float mag = magQ + magP;
-          if (sn1 > 1e-5f && gap > 0x1p-10f * mag * sn1) {
-            ++n_bound;
-            ++n_sep;
+          ckQ = kQ;
+        }
+        if (EDGE && active && pair == p0 + 2)
+          gate = __hip_atomic_load(&A.edge_res[edge], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ? 0 : 0x7fffffff;
+        bool defer = false;
+        if (active) {
+          if (pair >= p0 + 4 && (EDGE ? gate == 0 : gate <= crs)) {
+            ++n_dropped;
           } else {
-            defer = true;
+            // Start of the pair (collision4.h:228-244) and the bound decision
+            // of iteration 0: iteration 0 returns 0 whenever
+            // S.normalize(q0 - p0) >= -1e-6 for the support vertices p0, q0
+            // the climbs find, S = normalize(D) with D the difference of the
+            // two start vertices.  If the bounding boxes show S.(q - p) > 0
+            // with a wide margin for every vertex p of P and q of Q, that holds
+            // whichever vertices the climbs would pick, so the pair test ends
+            // here exactly as the reference's does (code 0, one
+            // iteration).  The margin, 2^-10 of the magnitudes involved,
+            // exceeds the rounding error of every quantity in the test and in
+            // the reference's iteration by orders of magnitude.  Both sides of
+            // the test scale with |S|, so it runs on D itself: the reference's
+            // S differs from D/|D| by a relative 1e-7, far inside the margin.
+            // The test also requires |D|_1 > 1e-5, so |D|_2 > 5.7e-6 stays
+            // clear of the reference's override of S for |D| <= 1e-6; a NaN D
+            // never passes (such pairs are deferred to the exact test).
+            const V3 vp = v3(L.verts[vbP]);
+            const V3 Pp = sysP0 ? vp : t_apply(tp, vp);
+            const V3 S{PqW.x - Pp.x, PqW.y - Pp.y, PqW.z - Pp.z};
+            // support extents along S: the box's sum_a |S.axis_a| h_a (axes:
+            // the frame's rotation columns).  The bounding sphere's |S| r is
+            // not used: on the benchmark scenes it decides under 0.1% more pair
+            // tests than the box alone, for a square root and five operations
+            // per pair.  The magnitude term of the margin is |c|_1 + r per side
+            // (precomputed for world-frame polytopes).
+            const float4 b0 = L.spheres[2 * kP], b1 = L.spheres[2 * kP + 1];
+            const V3 cPw = sysP0 ? v3(b0) : t_apply(tp, v3(b0));
+            const float boxQ = (fabsf(dot3(S, aQ0)) + fabsf(dot3(S, aQ1))) + fabsf(dot3(S, aQ2));
+            float boxP, magP;
+            if (sysP0) {
+              boxP = (fabsf(S.x) * b1.x + fabsf(S.y) * b1.y) + fabsf(S.z) * b1.z;
+              magP = b1.w;
+            } else {
+              const V3 sp = t_apply_rot_inv(tp, S);
+              boxP = (fabsf(sp.x) * b1.x + fabsf(sp.y) * b1.y) + fabsf(sp.z) * b1.z;
+              magP = fabsf(cPw.x) + fabsf(cPw.y) + fabsf(cPw.z) + b0.w;
+            }
+            const float sn1 = fabsf(S.x) + fabsf(S.y) + fabsf(S.z);
+            const float gap = dot3(S, V3{cQw.x - cPw.x, cQw.y - cPw.y, cQw.z - cPw.z}) - boxQ - boxP;
+            const float mag = magQ + magP;
+            if (sn1 > 1e-5f && gap > 0x1p-10f * mag * sn1) {
+              ++n_bound;
+              ++n_sep;
+            } else {
+              defer = true;
+            }
           }
         }
+        // append the undecided pairs to the ring; drain it once CAP - 64 wait
+        const unsigned long long dm = __ballot(defer);
+        if (defer) {
+          const unsigned slot = (d_tail + (unsigned)__popcll(dm & lt_mask)) & (CAP - 1);
+          R.ent[slot] = make_int2(item, pair);
+          if (EDGE) R.edge[slot] = edge;
+        }
+        d_tail += (unsigned)__popcll(dm);
+        if (d_tail - d_head >= CAP - 64) {
+          ++pair;
+          break;
+        }
       }
-      // append the undecided pairs to the ring; drain it once CAP - 64 wait
-      const unsigned long long dm = __ballot(defer);
-      if (defer) {
-        const unsigned slot = (d_tail + (unsigned)__popcll(dm & lt_mask)) & (CAP - 1);
-        R.ent[slot] = make_int2(item, pair);
-        if (EDGE) R.edge[slot] = edge;
-      }
-      d_tail += (unsigned)__popcll(dm);
-      if (d_tail - d_head >= CAP - 64) {
-        sva_drain<EDGE, CAP>(A, L, R, H, hs, d_head, d_tail, tp, tq, n_sep, n_col, n_dropped, n_iter);
-        fsP = fsQ = ckQ = cvQ = -1;  // the frame registers were clobbered
-      }
+      if (d_tail - d_head >= CAP - 64)
+        sva_drain<EDGE, CAP>(A, L, R, H, hs, d_head, d_tail, n_sep, n_col, n_dropped, n_iter);
     }
     ++n_units;
   }
-  if (d_tail != d_head) sva_drain<EDGE, CAP>(A, L, R, H, hs, d_head, d_tail, tp, tq, n_sep, n_col, n_dropped, n_iter);
+  if (d_tail != d_head) sva_drain<EDGE, CAP>(A, L, R, H, hs, d_head, d_tail, n_sep, n_col, n_dropped, n_iter);
 
   cnt.sep += n_sep;
   cnt.col += n_col;
@@ -1005,7 +1015,8 @@ __device__ __forceinline__ void add_tally(const SvaCounts& c, int4* smem, unsign
 }
 
 // WAVES: occupancy target in waves per SIMD; the compiler keeps the register
-// budget at 512/WAVES VGPRs (5 waves: 96 VGPRs, a few spilled dwords).
+// budget at 512/WAVES VGPRs (5 waves: 96 VGPRs; the edge-mode build spills
+// two dwords).
 template <int EDGE, int BLOCK, int WAVES>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) sva_kernel(SvaArgs A) {
   extern __shared__ int4 smem[];
@@ -1217,11 +1228,15 @@ hipError_t sva_shape_finish(const SceneImg& img, SvaShape* sh, int device) {
 
 SvaShape sva_default_shape(const SceneImg& img) {
   SvaShape sh;
-  // four 256-thread blocks per CU when four scene copies fit in LDS, else
-  // one 1024-thread block sharing a single copy; units of up to 128 pairs
-  // (measured fastest on the edge-checking workload; launches shrink them
-  // when there are too few items, see unit_pairs in the host runtime)
-  if (4 * sva_lds_bytes(img, 256) <= (size_t)sva_max_lds_bytes()) {
+  // 256-thread blocks at five waves per SIMD (five blocks per CU) when five
+  // scene copies fit in LDS, at four when four fit, else one 1024-thread
+  // block sharing a single copy; units of up to 128 pairs (measured fastest
+  // on the edge-checking workload; launches shrink them when there are too
+  // few items, see unit_pairs in the host runtime)
+  if (5 * sva_lds_bytes(img, 256) <= (size_t)sva_max_lds_bytes()) {
+    sh.block = 256;
+    sh.waves = 5;
+  } else if (4 * sva_lds_bytes(img, 256) <= (size_t)sva_max_lds_bytes()) {
     sh.block = 256;
     sh.waves = 4;
   } else {

@@ -219,7 +219,7 @@ int gprm_surrogate_predict(void* model, const float* d_q, int n, long long offse
 typedef struct {
   int block;            /* threads per block */
   int waves;            /* occupancy target, waves per SIMD */
-  int pairs_per_unit;   /* pairs a lane claims at once, 1..256 */
+  int pairs_per_unit;   /* pairs of one work unit (a wave takes them for 64 items), 1..256 */
   int grid;             /* set by the library: resident blocks over all CUs */
 } gprm_launch_shape;
 int gprm_get_launch_shape(gprm_ctx* ctx, gprm_launch_shape* out);
