@@ -210,6 +210,7 @@ def lib():
     L.gprm_batch_bucket_info.argtypes = [vp, C.c_int, C.POINTER(BucketInfo)]
     L.gprm_batch_set_bucket_shape.argtypes = [vp, C.c_int, C.POINTER(LaunchShape)]
     L.gprm_batch_indicator_dev.argtypes = [vp, PP, LL, _IP, PP, C.c_int, _FP]
+    L.gprm_batch_indicator2_dev.argtypes = [vp, PP, PP, LL, _IP, PP, C.c_int, _FP, C.POINTER(C.c_longlong)]
     L.gprm_batch_autotune.argtypes = [vp, PP, LL, _IP, PP, C.c_int]
     L.gprm_batch_last_error.argtypes = [vp]
     L.gprm_batch_last_error.restype = C.c_char_p
@@ -274,11 +275,13 @@ def read_prm_output(path, ndof):
 
 
 class SceneBatch:
-    """Point checks of many scenes as one launch sequence (gprm_batch_*):
-    one kinematics launch, one persistent separating-vector launch per
-    vertex-count bucket, one finalize launch; optionally a replayed hipGraph.
-    `spaces` are initialised RobotConfigspace objects on one device with one
-    ndof; they must outlive the batch."""
+    """Checks of many scenes as one launch sequence (gprm_batch_*): point
+    checks (one kinematics launch, one persistent separating-vector launch per
+    vertex-count bucket, one finalize launch) and edge checks (one prologue,
+    then rounds of multi-scene compaction, kinematics and separating-vector
+    launches); optionally replayed from a hipGraph.  `spaces` are initialised
+    RobotConfigspace objects on one device with one ndof; they must outlive
+    the batch."""
 
     BUCKETS = 4
     # per-bucket shapes measured by the autotuner on MI355X (see the file)
@@ -365,6 +368,24 @@ class SceneBatch:
         self._ok(lib().gprm_batch_indicator_dev(self._h, qa, off, n, ra, int(bool(use_graph)),
                                                 done.ctypes.data_as(_FP)))
         return done
+
+    def indicator2_dev(self, qs, qe, res, use_graph: bool = False):
+        """Edge checks: qs[s], qe[s] float32 device tensors (ndof, n_s)
+        contiguous (start and end configurations); res[s] int32 device tensor
+        (n_s,), nonzero where the edge collides.  Returns (each scene's
+        completion time in ms after the first separating-vector block started,
+        the reference's sample count sum(testnum))."""
+        S = len(self.spaces)
+        assert len(qs) == S and len(qe) == S and len(res) == S
+        for a, b in zip(qs, qe):
+            assert a.shape == b.shape
+        qa, off, n, ra = self._arrays(qs, res)
+        ea = (C.c_void_p * S)(*[q.data_ptr() for q in qe])
+        done = np.zeros(S, np.float32)
+        total = C.c_longlong(0)
+        self._ok(lib().gprm_batch_indicator2_dev(self._h, qa, ea, off, n, ra, int(bool(use_graph)),
+                                                 done.ctypes.data_as(_FP), C.byref(total)))
+        return done, int(total.value)
 
     def autotune(self, qs, res, reps: int = 2):
         qa, off, n, ra = self._arrays(qs, res)

@@ -15,11 +15,24 @@
 // shape (block size and occupancy target, i.e. how many scene copies share
 // a CU's LDS, and pairs per work unit), set by the caller or picked by
 // gprm_batch_autotune from timed runs of the caller's workload.
+//
+// Edge checks of many scenes (gprm_batch_indicator2_dev) follow the
+// single-scene edge pipeline (run_edges in gprm_api.cpp) with every stage a
+// multi-scene launch: ONE prologue over all scenes' edges (testnum, sample
+// order, largest testnum per scene, read back once), then launch steps, each
+// ONE compaction and ONE kinematics launch over every scene's part of the
+// step and one persistent separating-vector launch per bucket.  Steps walk
+// the order-position windows [0,16), [16,48), ... shared by all scenes; a
+// window whose items exceed a scene's per-step budget takes several steps,
+// each over a range of that scene's edges.  The steps after the prologue
+// are what a hipGraph captures; the graph is kept while the arrays, sizes
+// and the per-scene largest testnums stay the same.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -51,6 +64,28 @@ struct Signature {
   bool operator==(const Signature& o) const { return q == o.q && res == o.res && off == o.off && n == o.n; }
 };
 
+constexpr int kMaxBatchEdges = 1 << 22;   // edges per scene and call
+constexpr int kEdgeStepItems = 1 << 19;   // samples per scene and launch step
+
+struct EdgeSignature {
+  std::vector<const void*> qs, qe, res;
+  std::vector<long long> off;
+  std::vector<int> n;
+  bool operator==(const EdgeSignature& o) const {
+    return qs == o.qs && qe == o.qe && res == o.res && off == o.off && n == o.n;
+  }
+};
+
+// One launch step of a multi-scene edge call.
+struct EdgeStep {
+  std::vector<EdgeTask> tasks;   // one per scene with a part in the step, by scene
+  std::vector<int> scene;        // the scene of each task
+  int cblocks = 0, fblocks = 0;  // compaction and kinematics grids
+  DevBuf d_tasks;                // EdgeTask[tasks.size()]
+  DevBuf sva[4];                 // per bucket: SceneTask of the bucket's tasks
+  int nsva[4] = {0, 0, 0, 0};
+};
+
 }  // namespace
 }  // namespace gprm
 
@@ -70,10 +105,25 @@ struct gprm_batch {
   hipGraphExec_t exec = nullptr;
   int clock_khz = 0;
   std::string err;
+  // multi-scene edge calls
+  EdgeSignature esig;
+  bool have_esig = false;
+  std::vector<long long> e_base;    // per scene: first edge in e_testnum / e_stride
+  DevBuf e_testnum, e_stride, e_max, e_total, e_clock, e_pro;
+  int e_pro_blocks = 0, e_pro_tasks = 0;
+  std::vector<int> e_plan_key;      // per scene largest testnum of the current plan
+  bool have_eplan = false;
+  std::vector<std::unique_ptr<EdgeStep>> e_steps;
+  DevBuf e_items, e_frames, e_ctl;  // step buffers; e_ctl: (items, work counter, units done) per task
+  int e_ntasks = 0;
+  hipGraph_t egraph = nullptr;
+  hipGraphExec_t eexec = nullptr;
   ~gprm_batch() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (eexec) (void)hipGraphExecDestroy(eexec);
+    if (egraph) (void)hipGraphDestroy(egraph);
     for (auto& b : buckets) {
       if (b.ev0) (void)hipEventDestroy(b.ev0);
       if (b.ev1) (void)hipEventDestroy(b.ev1);
@@ -102,12 +152,21 @@ int bhip(gprm_batch* b, hipError_t e, const char* what) {
     if (e_ != hipSuccess) return bhip(b, e_, what);      \
   } while (0)
 
+void drop_edge_plan(gprm_batch* b) {
+  if (b->eexec) (void)hipGraphExecDestroy(b->eexec);
+  if (b->egraph) (void)hipGraphDestroy(b->egraph);
+  b->eexec = nullptr;
+  b->egraph = nullptr;
+  b->have_eplan = false;
+}
+
 void drop_graph(gprm_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
   b->have_sig = false;
+  drop_edge_plan(b);  // its separating-vector tables carry the bucket shapes too
 }
 
 // Control words: per scene (work counter, units done), then per scene the
@@ -134,7 +193,9 @@ int prepare(gprm_batch* b, const Signature& sg) {
   size_t hist = 0;
   for (auto& bk : b->buckets)
     if (!bk.scenes.empty()) hist = std::max(hist, (size_t)kMaxIter * bk.shape.grid * bk.shape.block);
+  const void* old_hist = b->hist.p;
   BHIP(b, b->hist.ensure(sizeof(float) * 4 * std::max<size_t>(hist, 1)), "hipMalloc history");
+  if (b->hist.p != old_hist) drop_edge_plan(b);  // the edge steps' tables hold the old address
   std::vector<FkTask> fk(S);
   int blocks = 0;
   for (int s = 0; s < S; ++s) {
@@ -262,6 +323,298 @@ int signature(gprm_batch* b, const float* const* d_q, const long long* offset, c
   return GPRM_OK;
 }
 
+// ---- multi-scene edge calls -------------------------------------------------
+
+int edge_signature(gprm_batch* b, const float* const* d_qs, const float* const* d_qe, const long long* offset,
+                   const int* n, int* const* d_res, EdgeSignature* sg) {
+  const int S = (int)b->ctx.size();
+  if (!d_qs || !d_qe || !offset || !n || !d_res) return bfail(b, GPRM_E_INVALID, "null argument");
+  sg->qs.resize(S);
+  sg->qe.resize(S);
+  sg->res.resize(S);
+  sg->off.resize(S);
+  sg->n.resize(S);
+  for (int s = 0; s < S; ++s) {
+    if (n[s] < 0 || n[s] > kMaxBatchEdges || offset[s] < n[s] ||
+        (n[s] > 0 && (!d_qs[s] || !d_qe[s] || !d_res[s])))
+      return bfail(b, GPRM_E_INVALID, "bad arguments for scene " + std::to_string(s));
+    sg->qs[s] = d_qs[s];
+    sg->qe[s] = d_qe[s];
+    sg->res[s] = d_res[s];
+    sg->off[s] = offset[s];
+    sg->n[s] = n[s];
+  }
+  return GPRM_OK;
+}
+
+EdgeArgs edge_args(gprm_batch* b, const EdgeSignature& sg, int s) {
+  EdgeArgs ea{};
+  ea.qs = static_cast<const float*>(sg.qs[s]);
+  ea.qe = static_cast<const float*>(sg.qe[s]);
+  ea.offset = sg.off[s];
+  ea.nedges = sg.n[s];
+  ea.testnum = b->e_testnum.as<int>() + b->e_base[s];
+  ea.stride = b->e_stride.as<int>() + b->e_base[s];
+  ea.edge_res = static_cast<int*>(const_cast<void*>(sg.res[s]));
+  return ea;
+}
+
+// Per-signature buffers and the prologue task table (outside any capture).
+int edge_prepare(gprm_batch* b, const EdgeSignature& sg) {
+  const int S = (int)b->ctx.size();
+  b->e_base.assign(S + 1, 0);
+  for (int s = 0; s < S; ++s) b->e_base[s + 1] = b->e_base[s] + sg.n[s];
+  const size_t E = (size_t)std::max(1LL, b->e_base[S]);
+  BHIP(b, b->e_testnum.ensure(sizeof(int) * E), "hipMalloc testnum");
+  BHIP(b, b->e_stride.ensure(sizeof(int) * E), "hipMalloc stride");
+  BHIP(b, b->e_max.ensure(sizeof(int) * (size_t)S), "hipMalloc");
+  BHIP(b, b->e_total.ensure(sizeof(unsigned long long) * (size_t)S), "hipMalloc");
+  // completion clocks per scene, then the start clock
+  BHIP(b, b->e_clock.ensure(sizeof(unsigned long long) * (size_t)(S + 1)), "hipMalloc");
+  std::vector<EdgeTask> pro;
+  int blocks = 0;
+  for (int s = 0; s < S; ++s) {
+    if (sg.n[s] == 0) continue;
+    EdgeTask t{};
+    t.ea = edge_args(b, sg, s);
+    t.rb = b->ctx[s]->rb;
+    t.dq = b->ctx[s]->dq;
+    t.testnum = b->e_testnum.as<int>() + b->e_base[s];
+    t.stride = b->e_stride.as<int>() + b->e_base[s];
+    t.max_testnum = b->e_max.as<int>() + s;
+    t.total = b->e_total.as<unsigned long long>() + s;
+    t.block0 = blocks;
+    blocks += (sg.n[s] + 255) / 256;
+    pro.push_back(t);
+  }
+  b->e_pro_blocks = blocks;
+  b->e_pro_tasks = (int)pro.size();
+  if (!pro.empty()) {
+    BHIP(b, b->e_pro.ensure(sizeof(EdgeTask) * pro.size()), "hipMalloc tasks");
+    BHIP(b, hipMemcpy(b->e_pro.p, pro.data(), sizeof(EdgeTask) * pro.size(), hipMemcpyHostToDevice),
+         "copy tasks");
+  }
+  return GPRM_OK;
+}
+
+// Launch steps for the per-scene largest testnums maxT (outside any capture).
+int edge_plan(gprm_batch* b, const EdgeSignature& sg, const std::vector<int>& maxT) {
+  const int S = (int)b->ctx.size();
+  const int nd = b->ndof;
+  int gmax = 0;
+  for (int s = 0; s < S; ++s) gmax = std::max(gmax, sg.n[s] > 0 ? maxT[s] : 0);
+  // per scene: item budget of a step (the unit count must stay below 2^31
+  // for the smallest units, one pair each) and the capacity it needs
+  std::vector<int> budget(S), cap(S, 0);
+  for (int s = 0; s < S; ++s)
+    budget[s] = std::max(1, std::min(kEdgeStepItems, 0x7fffffff / std::max(1, b->ctx[s]->img.M)));
+  struct Part { int s, e0, e1, lo, hi, cap; };
+  std::vector<std::vector<Part>> steps;
+  for (const auto& w : edge_windows(gmax)) {
+    std::vector<int> per(S, 0), nch(S, 0);
+    int nsteps = 0;
+    for (int s = 0; s < S; ++s) {
+      if (sg.n[s] == 0 || maxT[s] <= w.first) continue;
+      const int width = std::min(w.second, maxT[s]) - w.first;
+      per[s] = std::max(1, budget[s] / width);
+      nch[s] = (sg.n[s] + per[s] - 1) / per[s];
+      nsteps = std::max(nsteps, nch[s]);
+    }
+    for (int j = 0; j < nsteps; ++j) {
+      std::vector<Part> parts;
+      for (int s = 0; s < S; ++s) {
+        if (j >= nch[s]) continue;
+        const int width = std::min(w.second, maxT[s]) - w.first;
+        const int e0 = j * per[s], e1 = std::min(sg.n[s], e0 + per[s]);
+        const long long c = (long long)(e1 - e0) * width;
+        parts.push_back({s, e0, e1, w.first, w.second, (int)c});
+        cap[s] = std::max(cap[s], (int)c);
+      }
+      steps.push_back(std::move(parts));
+    }
+  }
+  // step buffers: items and frames per scene (regions sized by its largest
+  // step), control words per task
+  std::vector<long long> ibase(S + 1, 0);
+  for (int s = 0; s < S; ++s) ibase[s + 1] = ibase[s] + cap[s];
+  int ntasks = 0;
+  for (const auto& st : steps) ntasks += (int)st.size();
+  BHIP(b, b->e_items.ensure(sizeof(int) * 2 * (size_t)std::max(1LL, ibase[S])), "hipMalloc items");
+  BHIP(b, b->e_frames.ensure(sizeof(float) * 12 * nd * (size_t)std::max(1LL, ibase[S])), "hipMalloc frames");
+  BHIP(b, b->e_ctl.ensure(sizeof(int) * 3 * (size_t)std::max(1, ntasks)), "hipMalloc control words");
+  size_t hist = 0;
+  for (auto& bk : b->buckets)
+    if (!bk.scenes.empty()) hist = std::max(hist, (size_t)kMaxIter * bk.shape.grid * bk.shape.block);
+  const void* old_hist = b->hist.p;
+  BHIP(b, b->hist.ensure(sizeof(float) * 4 * std::max<size_t>(hist, 1)), "hipMalloc history");
+  if (b->hist.p != old_hist) {  // the point path's task tables hold the old address
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->have_sig = false;
+  }
+  int* const items = b->e_items.as<int>();
+  const long long icap = std::max(1LL, ibase[S]);
+  std::vector<int> bucket_of(S, -1);
+  for (int i = 0; i < kBuckets; ++i)
+    for (int s : b->buckets[i].scenes) bucket_of[s] = i;
+  b->e_steps.clear();
+  int task = 0;
+  for (const auto& parts : steps) {
+    auto step = std::make_unique<EdgeStep>();
+    int cb = 0, fb = 0;
+    std::vector<SceneTask> sva[kBuckets];
+    for (const Part& pt : parts) {
+      const int s = pt.s;
+      const gprm_ctx* c = b->ctx[s];
+      int* ctl = b->e_ctl.as<int>() + 3 * task++;
+      EdgeTask t{};
+      t.ea = edge_args(b, sg, s);
+      t.rb = c->rb;
+      t.dq = c->dq;
+      t.e0 = pt.e0;
+      t.e1 = pt.e1;
+      t.lo = pt.lo;
+      t.hi = pt.hi;
+      t.item_edge = items + ibase[s];
+      t.item_pos = items + icap + ibase[s];
+      t.nitems = ctl;
+      t.cap = pt.cap;
+      t.frames = b->e_frames.as<float>() + (size_t)12 * nd * ibase[s];
+      t.block0 = cb;
+      t.fblock0 = fb;
+      cb += (pt.e1 - pt.e0 + 255) / 256;
+      fb += (pt.cap + kFkBlock - 1) / kFkBlock;
+      step->tasks.push_back(t);
+      step->scene.push_back(s);
+      const Bucket& bk = b->buckets[bucket_of[s]];
+      SceneTask q{};
+      q.a.img = c->img;
+      q.a.ndof = nd;
+      q.a.frames = t.frames;
+      q.a.fstride = pt.cap;
+      q.a.hist = b->hist.as<float4>();
+      q.a.nitems = -1;
+      q.a.nitems_dev = ctl;
+      q.a.counter = reinterpret_cast<unsigned*>(ctl + 1);
+      q.a.pairs_per_unit = bk.shape.ppu;
+      q.a.units_per_item = (c->img.M + bk.shape.ppu - 1) / bk.shape.ppu;
+      q.a.item_edge = t.item_edge;
+      q.a.edge_res = t.ea.edge_res;
+      q.done = reinterpret_cast<unsigned*>(ctl + 2);
+      q.t_done = b->e_clock.as<unsigned long long>() + s;
+      sva[bucket_of[s]].push_back(q);
+    }
+    step->cblocks = cb;
+    step->fblocks = fb;
+    BHIP(b, step->d_tasks.ensure(sizeof(EdgeTask) * step->tasks.size()), "hipMalloc tasks");
+    BHIP(b, hipMemcpy(step->d_tasks.p, step->tasks.data(), sizeof(EdgeTask) * step->tasks.size(),
+                      hipMemcpyHostToDevice),
+         "copy tasks");
+    for (int i = 0; i < kBuckets; ++i) {
+      step->nsva[i] = (int)sva[i].size();
+      if (sva[i].empty()) continue;
+      BHIP(b, step->sva[i].ensure(sizeof(SceneTask) * sva[i].size()), "hipMalloc tasks");
+      BHIP(b, hipMemcpy(step->sva[i].p, sva[i].data(), sizeof(SceneTask) * sva[i].size(), hipMemcpyHostToDevice),
+           "copy tasks");
+    }
+    b->e_steps.push_back(std::move(step));
+  }
+  b->e_ntasks = task;
+  return GPRM_OK;
+}
+
+// The launch steps on b->stream (capturable: kernels only).
+int edge_enqueue(gprm_batch* b) {
+  const int S = (int)b->ctx.size();
+  hipStream_t st = b->stream;
+  unsigned long long* clk = b->e_clock.as<unsigned long long>();
+  BHIP(b, launch_fill_u32(b->e_ctl.as<unsigned>(), 0u, 3LL * std::max(1, b->e_ntasks), st), "fill");
+  BHIP(b, launch_fill_u32(reinterpret_cast<unsigned*>(clk), 0u, 2LL * S, st), "fill");
+  BHIP(b, launch_fill_u32(reinterpret_cast<unsigned*>(clk + S), 0xffffffffu, 2, st), "fill");
+  for (const auto& step : b->e_steps) {
+    const int nt = (int)step->tasks.size();
+    BHIP(b, launch_edge_compact_multi(step->d_tasks.as<EdgeTask>(), nt, step->cblocks, st), "compaction launch");
+    BHIP(b, launch_edge_fk_multi(step->d_tasks.as<EdgeTask>(), nt, step->fblocks, b->ndof, st),
+         "kinematics launch");
+    for (int i = 0; i < kBuckets; ++i) {
+      if (step->nsva[i] == 0) continue;
+      const Bucket& bk = b->buckets[i];
+      BHIP(b, launch_sva_multi(step->sva[i].as<SceneTask>(), step->nsva[i], 1, bk.shape, bk.lds, clk + S, nullptr,
+                               st),
+           "separating-vector launch");
+    }
+  }
+  return GPRM_OK;
+}
+
+int edge_run(gprm_batch* b, const EdgeSignature& sg, bool use_graph, float* done_ms, long long* samples) {
+  BHIP(b, hipSetDevice(b->device), "hipSetDevice");
+  const int S = (int)b->ctx.size();
+  if (!(b->have_esig && b->esig == sg)) {
+    drop_edge_plan(b);
+    b->have_esig = false;
+    int rc = edge_prepare(b, sg);
+    if (rc != GPRM_OK) return rc;
+    b->esig = sg;
+    b->have_esig = true;
+  }
+  // prologue (edge results zeroed, testnum, order, largest testnum per scene)
+  hipStream_t st = b->stream;
+  BHIP(b, launch_fill_u32(b->e_max.as<unsigned>(), 0u, S, st), "fill");
+  BHIP(b, launch_fill_u32(b->e_total.as<unsigned>(), 0u, 2LL * S, st), "fill");
+  BHIP(b, launch_edge_prologue_multi(b->e_pro.as<EdgeTask>(), b->e_pro_tasks, b->e_pro_blocks, st),
+       "edge prologue launch");
+  std::vector<int> maxT(S);
+  std::vector<unsigned long long> tot(S);
+  BHIP(b, hipMemcpyAsync(maxT.data(), b->e_max.p, sizeof(int) * S, hipMemcpyDeviceToHost, st), "copy");
+  BHIP(b, hipMemcpyAsync(tot.data(), b->e_total.p, sizeof(unsigned long long) * S, hipMemcpyDeviceToHost, st),
+       "copy");
+  BHIP(b, hipStreamSynchronize(st), "sync");
+  if (samples) {
+    *samples = 0;
+    for (int s = 0; s < S; ++s) *samples += (long long)tot[s];
+  }
+  if (!(b->have_eplan && b->e_plan_key == maxT)) {
+    drop_edge_plan(b);
+    int rc = edge_plan(b, sg, maxT);
+    if (rc != GPRM_OK) return rc;
+    b->e_plan_key = maxT;
+    b->have_eplan = true;
+  }
+  if (use_graph) {
+    if (!b->eexec) {
+      BHIP(b, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+      int rc = edge_enqueue(b);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(st, &g);
+      if (rc != GPRM_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return bhip(b, e, "end capture");
+      b->egraph = g;
+      BHIP(b, hipGraphInstantiate(&b->eexec, b->egraph, nullptr, nullptr, 0), "instantiate graph");
+    }
+    BHIP(b, hipGraphLaunch(b->eexec, st), "graph launch");
+  } else {
+    int rc = edge_enqueue(b);
+    if (rc != GPRM_OK) return rc;
+  }
+  BHIP(b, hipStreamSynchronize(st), "sync");
+  if (done_ms) {
+    std::vector<unsigned long long> td(S + 1);
+    BHIP(b, hipMemcpy(td.data(), b->e_clock.p, sizeof(unsigned long long) * (S + 1), hipMemcpyDeviceToHost),
+         "copy clocks");
+    const unsigned long long t0 = td[S];
+    for (int s = 0; s < S; ++s)
+      done_ms[s] = (sg.n[s] == 0 || td[s] == 0 || td[s] < t0) ? 0.0f
+                                                               : (float)((double)(td[s] - t0) / b->clock_khz);
+  }
+  return GPRM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -304,9 +657,10 @@ int gprm_batch_create(gprm_ctx* const* ctxs, int nscenes, gprm_batch** out) {
   }
   for (auto& bk : b->buckets) {
     if (bk.scenes.empty()) continue;
-    // default: four scene copies per CU when LDS holds them, else one
+    // default: five or four scene copies per CU when LDS holds them, else one
     const size_t cap = (size_t)sva_max_lds_bytes() - 16;
-    if (4 * (bk.lds + sva_ring_bytes(256)) <= cap) { bk.shape.block = 256; bk.shape.waves = 4; }
+    if (5 * (bk.lds + sva_ring_bytes(256)) <= cap) { bk.shape.block = 256; bk.shape.waves = 5; }
+    else if (4 * (bk.lds + sva_ring_bytes(256)) <= cap) { bk.shape.block = 256; bk.shape.waves = 4; }
     else { bk.shape.block = 1024; bk.shape.waves = 4; }
     bk.shape.ppu = 64;
     e = sva_multi_shape_finish(bk.lds, &bk.shape, dev);
@@ -399,6 +753,16 @@ int gprm_batch_autotune(gprm_batch* b, const float* const* d_q, const long long*
     if (rc != GPRM_OK) return rc;
   }
   return GPRM_OK;
+}
+
+int gprm_batch_indicator2_dev(gprm_batch* b, const float* const* d_qs, const float* const* d_qe,
+                              const long long* offset, const int* n, int* const* d_res, int use_graph, float* done_ms,
+                              long long* samples_total) {
+  if (!b) return bfail(nullptr, GPRM_E_INVALID, "null batch");
+  EdgeSignature sg;
+  int rc = edge_signature(b, d_qs, d_qe, offset, n, d_res, &sg);
+  if (rc != GPRM_OK) return rc;
+  return edge_run(b, sg, use_graph != 0, done_ms, samples_total);
 }
 
 const char* gprm_batch_last_error(gprm_batch* b) { return b ? b->err.c_str() : g_batch_error.c_str(); }

@@ -16,6 +16,10 @@
 
 namespace gprm {
 
+// Order-position windows of the edge rounds for a largest testnum maxT:
+// [0, w0), then doubling widths (defined in gprm_api.cpp).
+std::vector<std::pair<int, int>> edge_windows(int maxT);
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;

@@ -39,6 +39,20 @@ constexpr int kFirstWindow = 16;
 thread_local std::string g_create_error;
 
 }  // namespace
+
+// Sample-window rounds: positions [0,w0), then doubling widths.
+std::vector<std::pair<int, int>> edge_windows(int maxT) {
+  std::vector<std::pair<int, int>> w;
+  int lo = 0, width = kFirstWindow;
+  while (lo < maxT) {
+    int hi = (int)std::min<long long>((long long)lo + width, maxT);
+    w.push_back({lo, hi});
+    lo = hi;
+    if (width < (1 << 20)) width *= 2;
+  }
+  return w;
+}
+
 }  // namespace gprm
 
 
@@ -337,18 +351,6 @@ int run_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n,
   return GPRM_OK;
 }
 
-// Sample-window rounds: positions [0,w0), then doubling widths.
-std::vector<std::pair<int, int>> windows(int maxT) {
-  std::vector<std::pair<int, int>> w;
-  int lo = 0, width = kFirstWindow;
-  while (lo < maxT) {
-    int hi = (int)std::min<long long>((long long)lo + width, maxT);
-    w.push_back({lo, hi});
-    lo = hi;
-    if (width < (1 << 20)) width *= 2;
-  }
-  return w;
-}
 
 // Edge checks on the device.  testnum/stride/res are computed by the
 // prologue kernel; maxT (host-known or read back) sizes the rounds.
@@ -377,7 +379,7 @@ int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long 
     GPRM_HIP(hipStreamSynchronize(st), "sync");
     maxT = hmax;
   }
-  std::vector<std::pair<int, int>> win = windows(maxT);
+  std::vector<std::pair<int, int>> win = edge_windows(maxT);
   // chunking: at most kItemBudget items per launch triple
   struct Launch { int e0, e1, lo, hi; };
   std::vector<Launch> plan;

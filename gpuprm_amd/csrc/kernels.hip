@@ -200,6 +200,33 @@ __global__ void finalize_points_kernel(const int* __restrict__ word, int* __rest
   codes[i] = w == kWordFree ? 0 : (w & 15) - 1;
 }
 
+// Frames of item i of an edge round: sample (pos * stride) mod T of its
+// edge, interpolated as robotconfigspace.cpp:406-420 does (c1 = s/(T-1),
+// c2 = 1 - c1, q = c1 qs + c2 qe), then the kinematics.
+template <int NDOF>
+__device__ __forceinline__ void fk_edge_item(const DevRobot& rb, const EdgeArgs& ea, const int* __restrict__ item_edge,
+                                             const int* __restrict__ item_pos, int i, float* __restrict__ frames,
+                                             long long fstride) {
+  const int nd = NDOF > 0 ? NDOF : rb.ndof;
+  const int e = item_edge[i];
+  const int pos = item_pos[i];
+  const int T = ea.testnum[e];
+  const int s = (int)(((long long)pos * ea.stride[e]) % T);
+  float c1, c2;
+  if (T > 0) {
+    c1 = (float)s / (float)(T - 1);
+    c2 = 1.0f - c1;
+  } else {
+    c1 = 1.0f;
+    c2 = 0.0f;
+  }
+  float qq[kMaxDofDev];
+#pragma unroll
+  for (int j = 0; j < (NDOF > 0 ? NDOF : kMaxDofDev); ++j)
+    if (NDOF > 0 || j < nd) qq[j] = c1 * ea.qs[e + ea.offset * j] + c2 * ea.qe[e + ea.offset * j];
+  fk_store<NDOF>(rb, qq, frames, i, fstride);
+}
+
 // q of the sample at order position pos of edge e (robotconfigspace.cpp:409-420).
 template <int NDOF>
 __global__ void __launch_bounds__(kFkBlock)
@@ -208,26 +235,8 @@ __global__ void __launch_bounds__(kFkBlock)
                     float* __restrict__ frames, long long fstride) {
   int n = *nitems_dev;
   if (n > cap) n = cap;
-  const int nd = NDOF > 0 ? NDOF : rb.ndof;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    int e = item_edge[i];
-    int pos = item_pos[i];
-    int T = ea.testnum[e];
-    int s = (int)(((long long)pos * ea.stride[e]) % T);
-    float c1, c2;
-    if (T > 0) {
-      c1 = (float)s / (float)(T - 1);
-      c2 = 1.0f - c1;
-    } else {
-      c1 = 1.0f;
-      c2 = 0.0f;
-    }
-    float qq[kMaxDofDev];
-#pragma unroll
-    for (int j = 0; j < (NDOF > 0 ? NDOF : kMaxDofDev); ++j)
-      if (NDOF > 0 || j < nd) qq[j] = c1 * ea.qs[e + ea.offset * j] + c2 * ea.qe[e + ea.offset * j];
-    fk_store<NDOF>(rb, qq, frames, i, fstride);
-  }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    fk_edge_item<NDOF>(rb, ea, item_edge, item_pos, i, frames, fstride);
 }
 
 __device__ int gcd_i(int a, int b) {
@@ -242,11 +251,10 @@ __device__ int gcd_i(int a, int b) {
 // testnum (robotconfigspace.cpp:660-673) and the sample order stride: a
 // golden-ratio step coprime with testnum, so that positions 0,1,2,... visit
 // every sample once, spread along the edge.
-__global__ void __launch_bounds__(256)
-    edge_prologue_kernel(int ndof, float dq, EdgeArgs ea, int* __restrict__ testnum,
-                         int* __restrict__ stride, int* __restrict__ max_testnum,
-                         unsigned long long* __restrict__ total) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void edge_prologue_one(int ndof, float dq, const EdgeArgs& ea, int k,
+                                                  int* __restrict__ testnum, int* __restrict__ stride,
+                                                  int* __restrict__ max_testnum,
+                                                  unsigned long long* __restrict__ total) {
   int T = 0;
   if (k < ea.nedges) {
     float norm = 0.0f;
@@ -270,12 +278,20 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+__global__ void __launch_bounds__(256)
+    edge_prologue_kernel(int ndof, float dq, EdgeArgs ea, int* __restrict__ testnum,
+                         int* __restrict__ stride, int* __restrict__ max_testnum,
+                         unsigned long long* __restrict__ total) {
+  edge_prologue_one(ndof, dq, ea, blockIdx.x * blockDim.x + threadIdx.x, testnum, stride, max_testnum, total);
+}
+
 // Items for order positions [lo, hi) of every open edge in [e0, e1).
 // One atomic per wave: lanes reserve contiguous slots by prefix count.
-__global__ void __launch_bounds__(256)
-    edge_compact_kernel(EdgeArgs ea, int e0, int e1, int lo, int hi, int* __restrict__ item_edge,
-                        int* __restrict__ item_pos, int* __restrict__ nitems) {
-  int e = e0 + blockIdx.x * blockDim.x + threadIdx.x;
+// Every lane of the wave calls it (the prefix sum is wave-wide); lanes with
+// e >= e1 contribute nothing.
+__device__ __forceinline__ void edge_compact_one(const EdgeArgs& ea, int e, int e1, int lo, int hi,
+                                                 int* __restrict__ item_edge, int* __restrict__ item_pos,
+                                                 int* __restrict__ nitems) {
   int cnt = 0;
   if (e < e1) {
     int T = ea.testnum[e];
@@ -298,6 +314,49 @@ __global__ void __launch_bounds__(256)
     item_edge[o + t] = e;
     item_pos[o + t] = lo + t;
   }
+}
+
+__global__ void __launch_bounds__(256)
+    edge_compact_kernel(EdgeArgs ea, int e0, int e1, int lo, int hi, int* __restrict__ item_edge,
+                        int* __restrict__ item_pos, int* __restrict__ nitems) {
+  edge_compact_one(ea, e0 + blockIdx.x * blockDim.x + threadIdx.x, e1, lo, hi, item_edge, item_pos, nitems);
+}
+
+// Multi-scene edge calls: one task per scene (prologue) or per scene part of
+// a launch step (compaction, kinematics); block b of a grid belongs to the
+// last task whose first block (block0 for the prologue and the compaction,
+// fblock0 for the kinematics) is <= b.
+template <bool FK>
+__device__ __forceinline__ const EdgeTask& edge_task_of(const EdgeTask* __restrict__ tasks, int ntasks) {
+  int lo = 0, hi = ntasks - 1;
+  const int b = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((FK ? tasks[mid].fblock0 : tasks[mid].block0) <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return tasks[lo];
+}
+
+__global__ void __launch_bounds__(256) edge_prologue_multi_kernel(const EdgeTask* __restrict__ tasks, int ntasks) {
+  const EdgeTask& T = edge_task_of<false>(tasks, ntasks);
+  edge_prologue_one(T.rb.ndof, T.dq, T.ea, (blockIdx.x - T.block0) * 256 + threadIdx.x, T.testnum, T.stride,
+                    T.max_testnum, T.total);
+}
+
+__global__ void __launch_bounds__(256) edge_compact_multi_kernel(const EdgeTask* __restrict__ tasks, int ntasks) {
+  const EdgeTask& T = edge_task_of<false>(tasks, ntasks);
+  edge_compact_one(T.ea, T.e0 + (blockIdx.x - T.block0) * 256 + threadIdx.x, T.e1, T.lo, T.hi, T.item_edge,
+                   T.item_pos, T.nitems);
+}
+
+template <int NDOF>
+__global__ void __launch_bounds__(kFkBlock) fk_edges_multi_kernel(const EdgeTask* __restrict__ tasks, int ntasks) {
+  const EdgeTask& T = edge_task_of<true>(tasks, ntasks);
+  const int i = (blockIdx.x - T.fblock0) * kFkBlock + threadIdx.x;
+  int n = *T.nitems;
+  if (n > T.cap) n = T.cap;
+  if (i < n) fk_edge_item<NDOF>(T.rb, T.ea, T.item_edge, T.item_pos, i, T.frames, (long long)T.cap);
 }
 
 // Radius neighbours (get_near_vertices, prm/src/lib/prm/prmsolver.cpp:
@@ -1303,6 +1362,33 @@ hipError_t launch_edge_compact(EdgeArgs ea, int e0, int e1, int lo, int hi, int*
   if (e1 <= e0) return hipSuccess;
   dim3 grid((e1 - e0 + 255) / 256), block(256);
   hipLaunchKernelGGL(edge_compact_kernel, grid, block, 0, st, ea, e0, e1, lo, hi, item_edge, item_pos, nitems);
+  return hipGetLastError();
+}
+
+hipError_t launch_edge_prologue_multi(const EdgeTask* d_tasks, int ntasks, int nblocks, hipStream_t st) {
+  if (ntasks <= 0 || nblocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(edge_prologue_multi_kernel, dim3(nblocks), dim3(256), 0, st, d_tasks, ntasks);
+  return hipGetLastError();
+}
+
+hipError_t launch_edge_compact_multi(const EdgeTask* d_tasks, int ntasks, int nblocks, hipStream_t st) {
+  if (ntasks <= 0 || nblocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(edge_compact_multi_kernel, dim3(nblocks), dim3(256), 0, st, d_tasks, ntasks);
+  return hipGetLastError();
+}
+
+hipError_t launch_edge_fk_multi(const EdgeTask* d_tasks, int ntasks, int nblocks, int ndof, hipStream_t st) {
+  if (ntasks <= 0 || nblocks <= 0) return hipSuccess;
+  dim3 grid(nblocks), block(kFkBlock);
+  switch (ndof) {
+    case 2: hipLaunchKernelGGL(fk_edges_multi_kernel<2>, grid, block, 0, st, d_tasks, ntasks); break;
+    case 3: hipLaunchKernelGGL(fk_edges_multi_kernel<3>, grid, block, 0, st, d_tasks, ntasks); break;
+    case 4: hipLaunchKernelGGL(fk_edges_multi_kernel<4>, grid, block, 0, st, d_tasks, ntasks); break;
+    case 5: hipLaunchKernelGGL(fk_edges_multi_kernel<5>, grid, block, 0, st, d_tasks, ntasks); break;
+    case 6: hipLaunchKernelGGL(fk_edges_multi_kernel<6>, grid, block, 0, st, d_tasks, ntasks); break;
+    case 7: hipLaunchKernelGGL(fk_edges_multi_kernel<7>, grid, block, 0, st, d_tasks, ntasks); break;
+    default: hipLaunchKernelGGL(fk_edges_multi_kernel<0>, grid, block, 0, st, d_tasks, ntasks); break;
+  }
   return hipGetLastError();
 }
 

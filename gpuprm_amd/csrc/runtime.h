@@ -113,6 +113,33 @@ struct EdgeArgs {
   int* edge_res;           // [edge]
 };
 
+// One scene's part of a multi-scene edge call (device memory).  Prologue
+// tasks (one per scene) cover edges [0, ea.nedges) and write testnum,
+// stride, the scene's largest testnum and its sample total; launch-step
+// tasks cover edges [e0, e1) and order positions [lo, hi): the compaction
+// appends their items (item_edge, item_pos, *nitems, at most cap) and the
+// kinematics writes their frames (link-major, item stride cap).
+struct EdgeTask {
+  EdgeArgs ea;
+  DevRobot rb;
+  float dq;
+  int* testnum;
+  int* stride;
+  int* max_testnum;
+  unsigned long long* total;
+  int e0, e1, lo, hi;
+  int* item_edge;
+  int* item_pos;
+  int* nitems;
+  int cap;
+  float* frames;
+  int block0;    // first block of the task in the prologue / compaction grid (tasks sorted)
+  int fblock0;   // first block of the task in the kinematics grid (tasks sorted)
+};
+hipError_t launch_edge_prologue_multi(const EdgeTask* d_tasks, int ntasks, int nblocks, hipStream_t st);
+hipError_t launch_edge_compact_multi(const EdgeTask* d_tasks, int ntasks, int nblocks, hipStream_t st);
+hipError_t launch_edge_fk_multi(const EdgeTask* d_tasks, int ntasks, int nblocks, int ndof, hipStream_t st);
+
 // Deferral rings of the separating-vector kernel: per wave, a ring of
 // sva_ring_cap(block) entries of 12 bytes (item, pair, edge).  Larger rings
 // start the full tests in larger batches, which shortens their tails; the

@@ -259,6 +259,20 @@ int gprm_batch_set_bucket_shape(gprm_batch* batch, int bucket, const gprm_launch
  * (device clock).  Returns when the results are complete. */
 int gprm_batch_indicator_dev(gprm_batch* batch, const float* const* d_q, const long long* offset, const int* n,
                              int* const* d_res, int use_graph, float* done_ms);
+/* Edge checks of many scenes: scene s has n[s] (<= 2^22) edges, start and
+ * end configurations at d_qs[s], d_qe[s] (device, SoA with offset[s] >= n[s])
+ * -> d_res[s] (device), as gprm_indicator2_dev computes them (nonzero when a
+ * sample collides; edges shorter than dq are blocked, robotconfigspace.cpp:
+ * 411-413).  One prologue launch over all scenes' edges, read back once, then
+ * launch steps over order-position windows: one compaction and one
+ * kinematics launch over every scene per step, one separating-vector launch
+ * per bucket.  use_graph: the steps are captured into a hipGraph and
+ * replayed while the arrays, sizes and the scenes' largest testnums stay the
+ * same.  done_ms as for gprm_batch_indicator_dev; samples_total (may be
+ * NULL): the sum of testnum over all edges, the reference's sample count. */
+int gprm_batch_indicator2_dev(gprm_batch* batch, const float* const* d_qs, const float* const* d_qe,
+                              const long long* offset, const int* n, int* const* d_res, int use_graph, float* done_ms,
+                              long long* samples_total);
 /* Autotuner: for every bucket in use, times its separating-vector launch
  * (HIP events; reps runs after one warm-up, best kept) on the given workload
  * with every available variant x pairs_per_unit in {32, 64, 128, 256} and keeps
