@@ -243,12 +243,57 @@ def leg_multi_scene(reps):
     dt = float(np.median(times))
     L = np.median(np.stack(lat), axis=0)
     info = b.bucket_info()
-    return {"workload": "64 scenes: 7-DOF arm, 8..128 hulls of 8..24 points, 1k-64k configurations each",
-            "configs": int(sum(sizes)), "checks_per_s": sum(sizes) / dt, "ms_per_batch": 1000 * dt,
-            "scene_done_ms_p50": float(np.percentile(L, 50)), "scene_done_ms_p99": float(np.percentile(L, 99)),
-            "buckets": [{"vertices": [x["min_vertices"], x["max_vertices"]], "scenes": x["scenes"],
-                         "shape": x["shape"]} for x in info if x["scenes"]],
-            "tuning": table, "mismatches_vs_single_scene": mism}
+    out = {"workload": "64 scenes: 7-DOF arm, 8..128 hulls of 8..24 points, 1k-64k configurations each",
+           "configs": int(sum(sizes)), "checks_per_s": sum(sizes) / dt, "ms_per_batch": 1000 * dt,
+           "scene_done_ms_p50": float(np.percentile(L, 50)), "scene_done_ms_p99": float(np.percentile(L, 99)),
+           "buckets": [{"vertices": [x["min_vertices"], x["max_vertices"]], "scenes": x["scenes"],
+                        "shape": x["shape"]} for x in info if x["scenes"]],
+           "tuning": table, "mismatches_vs_single_scene": mism}
+    # edge sets: per scene the k=10 nearest-neighbour edges among its first
+    # 2048 configurations, all scenes in one graph-replayed edge batch
+    QS, QE = [], []
+    for s in range(64):
+        V = qs[s][:, :2048].cpu().numpy()
+        I, J = knn_edges(V, 10)
+        QS.append(torch.from_numpy(V[:, I].copy()).cuda())
+        QE.append(torch.from_numpy(V[:, J].copy()).cuda())
+    er = [torch.zeros(x.shape[1], dtype=torch.int32, device="cuda") for x in QS]
+    b.indicator2_dev(QS, QE, er, use_graph=True)  # plan and capture
+    torch.cuda.synchronize()
+    times, lat = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        done, samples = b.indicator2_dev(QS, QE, er, use_graph=True)
+        times.append(time.perf_counter() - t0)
+        lat.append(done)
+    emism = 0
+    for s in (0, 21, 42, 63):
+        r1 = torch.zeros(QS[s].shape[1], dtype=torch.int32, device="cuda")
+        spaces[s].indicator2_dev(QS[s], QE[s], r1)
+        emism += int(((r1 != 0) != (er[s] != 0)).sum().item())
+    # the same edges scene by scene (64 single-scene calls), for comparison
+    r1 = [torch.zeros(x.shape[1], dtype=torch.int32, device="cuda") for x in QS]
+    for s in range(64):
+        spaces[s].indicator2_dev(QS[s], QE[s], r1[s])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(64):
+        spaces[s].indicator2_dev(QS[s], QE[s], r1[s])
+    torch.cuda.synchronize()
+    single_ms = 1000 * (time.perf_counter() - t0)
+    dt = float(np.median(times))
+    L = np.median(np.stack(lat), axis=0)
+    nedges = int(sum(x.shape[1] for x in QS))
+    out["edges"] = {"workload": "k=10 nearest-neighbour edges among each scene's first 2048 configurations, "
+                                "dq=0.01 (reference L2 sample rule)",
+                    "edges": nedges, "samples": int(samples), "ms_per_batch": 1000 * dt,
+                    "checks_per_s": samples / dt, "edges_per_s": nedges / dt,
+                    "blocked_edges": int(sum(int((x != 0).sum().item()) for x in er)),
+                    "scene_done_ms_p50": float(np.percentile(L, 50)),
+                    "scene_done_ms_p99": float(np.percentile(L, 99)),
+                    "decision_mismatches_vs_single_scene": emism,
+                    "scene_by_scene_ms": single_ms}
+    return out
 
 
 def _tasks(o, nd, n, seed):
