@@ -114,9 +114,9 @@ __device__ __forceinline__ T12 t_lapply(const T12& self, const T12& T) {
 }
 
 // Kinematics<ndof>::calculate (kinematics.h:27-37); frames 1..ndof of item
-// to the link-major frame array: the 12 floats of link frame i of item at
-// out[(i stride + item) 12], so one link's frames of consecutive items are
-// contiguous 48-byte records.
+// to the frame array, link-major and quad-planar: float4 j (0..2) of link
+// frame i of item at out4[(3 i + j) stride + item], so each quad of one link
+// is a contiguous array over the items.
 template <int NDOF>
 __device__ __forceinline__ void fk_store(const DevRobot& rb, const float* q, float* __restrict__ out, int item,
                                          long long stride) {
@@ -137,17 +137,21 @@ __device__ __forceinline__ void fk_store(const DevRobot& rb, const float* q, flo
     else dset = q[i];
     T12 A = t_set_cs(rb.a[i], rb.ca[i], rb.sa[i], qset, dset);
     T = t_lapply(A, T);
-    float4* o = reinterpret_cast<float4*>(out + ((size_t)i * stride + item) * 12);
+    // quad-planar: the lanes of a wave store each quad of 64 consecutive
+    // items as one contiguous 1 KB span
+    float4* o = reinterpret_cast<float4*>(out) + (size_t)i * 3 * stride + item;
     o[0] = make_float4(T.m[0], T.m[1], T.m[2], T.m[3]);
-    o[1] = make_float4(T.m[4], T.m[5], T.m[6], T.m[7]);
-    o[2] = make_float4(T.m[8], T.m[9], T.m[10], T.m[11]);
+    o[stride] = make_float4(T.m[4], T.m[5], T.m[6], T.m[7]);
+    o[2 * stride] = make_float4(T.m[8], T.m[9], T.m[10], T.m[11]);
   }
 }
 
-// Frame of link sys (1..ndof) of item from the link-major frame array.
+// Frame of link sys (1..ndof) of item from the frame array: link-major and
+// quad-planar, float4 j (0..2) of link s of item i at
+// frames4[((s - 1) 3 + j) stride + i].
 __device__ __forceinline__ T12 load_frame(const float* __restrict__ frames, int item, long long stride, int sys) {
-  const float4* f = reinterpret_cast<const float4*>(frames + ((size_t)(sys - 1) * stride + item) * 12);
-  const float4 a = f[0], b = f[1], c = f[2];
+  const float4* f = reinterpret_cast<const float4*>(frames) + (size_t)(sys - 1) * 3 * stride + item;
+  const float4 a = f[0], b = f[stride], c = f[2 * stride];
   T12 t;
   t.m[0] = a.x; t.m[1] = a.y; t.m[2] = a.z;  t.m[3] = a.w;
   t.m[4] = b.x; t.m[5] = b.y; t.m[6] = b.z;  t.m[7] = b.w;

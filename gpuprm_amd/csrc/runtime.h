@@ -47,8 +47,8 @@ struct SceneImg {
 struct SvaArgs {
   SceneImg img;
   int ndof;
-  const float* frames;     // link frames, link-major: the 12 floats of link s (1..ndof)
-                           // of item i at frames[((s-1) fstride + i) 12]
+  const float* frames;     // link frames, link-major and quad-planar: float4 j (0..2) of
+                           // link s (1..ndof) of item i at frames4[((s-1) 3 + j) fstride + i]
   long long fstride;       // item stride of frames (n of the kinematics launch, or its cap)
   float4* hist;            // [kMaxIter][grid*block] per-lane iteration history
   int nitems;              // item count, or -1: read *nitems_dev
