@@ -38,6 +38,11 @@ struct DevBuf {
 // one stream per outstanding request, robotconfigspace.cpp:627-650).
 struct Slot {
   hipStream_t stream = nullptr;
+  // edge calls read the prologue's largest testnum back on a side stream
+  // (into pinned memory) while the first round runs on the call's stream
+  hipStream_t side = nullptr;
+  hipEvent_t ev_prologue = nullptr;
+  void* pinned = nullptr;  // 32 bytes: largest testnum (int), sample total (u64 at +8)
   DevBuf frames, status, item_edge, item_pos, counters, qs, qe, res, testnum, stride, misc, tally, hist;
   int* host_res = nullptr;
   int N = 0;
@@ -50,6 +55,10 @@ struct Slot {
   ~Slot() {
     for (auto e : ev_start) (void)hipEventDestroy(e);
     for (auto e : ev_end) (void)hipEventDestroy(e);
+    if (side) (void)hipStreamSynchronize(side);
+    if (ev_prologue) (void)hipEventDestroy(ev_prologue);
+    if (pinned) (void)hipHostFree(pinned);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };

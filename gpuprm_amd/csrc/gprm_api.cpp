@@ -370,71 +370,110 @@ int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long 
   ea.testnum = sl.testnum.as<int>(); ea.stride = sl.stride.as<int>(); ea.edge_res = d_res;
   GPRM_HIP(launch_edge_prologue(c->rb, c->dq, ea, sl.testnum.as<int>(), sl.stride.as<int>(), d_max, d_total, st),
            "edge prologue launch");
-  int maxT = host_maxT;
-  unsigned long long total = 0;
-  if (maxT < 0 || stats) {
-    int hmax = 0;
-    GPRM_HIP(hipMemcpyAsync(&hmax, d_max, 4, hipMemcpyDeviceToHost, st), "copy max");
-    GPRM_HIP(hipMemcpyAsync(&total, d_total, 8, hipMemcpyDeviceToHost, st), "copy total");
-    GPRM_HIP(hipStreamSynchronize(st), "sync");
-    maxT = hmax;
+  // The first window [0, kFirstWindow) needs no largest testnum (edges with
+  // fewer samples emit fewer items), so it is enqueued at once; the largest
+  // testnum, which sizes the later windows, is read back on the side stream
+  // meanwhile.
+  const bool need_max = host_maxT < 0 || stats;
+  if (need_max) {
+    if (!sl.side) GPRM_HIP(hipStreamCreateWithFlags(&sl.side, hipStreamNonBlocking), "hipStreamCreate");
+    if (!sl.ev_prologue) GPRM_HIP(hipEventCreateWithFlags(&sl.ev_prologue, hipEventDisableTiming), "hipEventCreate");
+    if (!sl.pinned) GPRM_HIP(hipHostMalloc(&sl.pinned, 32, hipHostMallocDefault), "hipHostMalloc");
+    GPRM_HIP(hipEventRecord(sl.ev_prologue, st), "event");
+    GPRM_HIP(hipStreamWaitEvent(sl.side, sl.ev_prologue, 0), "stream wait");
+    GPRM_HIP(hipMemcpyAsync(sl.pinned, sl.misc.p, 24, hipMemcpyDeviceToHost, sl.side), "copy max");
   }
-  std::vector<std::pair<int, int>> win = edge_windows(maxT);
-  // chunking: at most kItemBudget items per launch triple
+  // launches: edges [e0, e1) x order positions [lo, hi), at most item_budget
+  // items each
   struct Launch { int e0, e1, lo, hi; };
   std::vector<Launch> plan;
-  for (auto& w : win) {
-    int width = w.second - w.first;
-    int per = std::max(1, item_budget(c) / width);
-    for (int e0 = 0; e0 < N; e0 += per) plan.push_back({e0, std::min(N, e0 + per), w.first, w.second});
+  std::vector<int> early;  // item counts of launches whose counters were replaced
+  auto add_window = [&](int lo, int hi) {
+    const int per = std::max(1, item_budget(c) / (hi - lo));
+    for (int e0 = 0; e0 < N; e0 += per) plan.push_back({e0, std::min(N, e0 + per), lo, hi});
+  };
+  auto enqueue = [&](size_t from) -> int {
+    int cap = 0;
+    for (size_t i = from; i < plan.size(); ++i)
+      cap = std::max(cap, (plan[i].e1 - plan[i].e0) * (plan[i].hi - plan[i].lo));
+    const size_t fbytes = sizeof(float) * 12 * nd * (size_t)std::max(cap, 1);
+    const size_t ibytes = sizeof(int) * (size_t)std::max(cap, 1);
+    // per launch: [0] item count, [1] work counter
+    const size_t cbytes = sizeof(int) * 2 * std::max<size_t>(plan.size(), 32);
+    if (from > 0 && (fbytes > sl.frames.bytes || ibytes > sl.item_edge.bytes || cbytes > sl.counters.bytes)) {
+      // earlier launches still use the buffers to be replaced; their item
+      // counts are kept on the host
+      GPRM_HIP(hipStreamSynchronize(st), "sync");
+      early.resize(2 * from);
+      GPRM_HIP(hipMemcpy(early.data(), sl.counters.p, sizeof(int) * early.size(), hipMemcpyDeviceToHost), "copy");
+    }
+    GPRM_HIP(sl.frames.ensure(fbytes), "hipMalloc frames");
+    GPRM_HIP(sl.item_edge.ensure(ibytes), "hipMalloc items");
+    GPRM_HIP(sl.item_pos.ensure(ibytes), "hipMalloc items");
+    GPRM_HIP(sl.counters.ensure(cbytes), "hipMalloc counters");
+    GPRM_HIP(sl.hist.ensure(sizeof(float) * 4 * kMaxIter * (size_t)c->shape.grid * c->shape.block),
+             "hipMalloc history");
+    if (plan.size() > from)
+      GPRM_HIP(launch_fill_u32(sl.counters.as<unsigned>() + 2 * from, 0u, 2 * (long long)(plan.size() - from), st),
+               "fill");
+    for (size_t i = from; i < plan.size(); ++i) {
+      const Launch& p = plan[i];
+      int* cnt = sl.counters.as<int>() + 2 * i;
+      GPRM_HIP(launch_edge_compact(ea, p.e0, p.e1, p.lo, p.hi, sl.item_edge.as<int>(), sl.item_pos.as<int>(), cnt,
+                                   st),
+               "edge compaction launch");
+      int pcap = (p.e1 - p.e0) * (p.hi - p.lo);
+      size_t ev = 0;
+      GPRM_HIP(prof_begin(c, sl, 1, st, &ev), "event");
+      GPRM_HIP(launch_edge_fk(c->rb, ea, sl.item_edge.as<int>(), sl.item_pos.as<int>(), cnt, pcap,
+                              sl.frames.as<float>(), st),
+               "edge kinematics launch");
+      GPRM_HIP(prof_end(c, sl, ev, st), "event");
+      SvaArgs a{};
+      a.img = c->img;
+      a.ndof = nd;
+      a.frames = sl.frames.as<float>();
+      a.fstride = pcap;
+      a.hist = sl.hist.as<float4>();
+      a.nitems = -1;
+      a.nitems_dev = cnt;
+      a.counter = reinterpret_cast<unsigned*>(cnt + 1);
+      a.item_edge = sl.item_edge.as<int>();
+      a.edge_res = d_res;
+      a.pairs_per_unit = unit_pairs(c, pcap);
+      a.units_per_item = (c->img.M + a.pairs_per_unit - 1) / a.pairs_per_unit;
+      a.tally = tally_ptr(c, sl);
+      a.static_first = 1;
+      GPRM_HIP(prof_begin(c, sl, 0, st, &ev), "event");
+      GPRM_HIP(launch_sva(a, 1, launch_shape(c, (long long)((pcap + 63) / 64) * a.units_per_item), st),
+               "separating-vector launch");
+      GPRM_HIP(prof_end(c, sl, ev, st), "event");
+    }
+    return GPRM_OK;
+  };
+  int rc = GPRM_OK;
+  int maxT = host_maxT;
+  if (need_max || maxT > 0) {
+    add_window(0, kFirstWindow);
+    if ((rc = enqueue(0)) != GPRM_OK) return rc;
   }
-  int cap = 0;
-  for (auto& p : plan) cap = std::max(cap, (p.e1 - p.e0) * (p.hi - p.lo));
-  GPRM_HIP(sl.frames.ensure(sizeof(float) * 12 * nd * (size_t)std::max(cap, 1)), "hipMalloc frames");
-  GPRM_HIP(sl.item_edge.ensure(sizeof(int) * (size_t)std::max(cap, 1)), "hipMalloc items");
-  GPRM_HIP(sl.item_pos.ensure(sizeof(int) * (size_t)std::max(cap, 1)), "hipMalloc items");
-  GPRM_HIP(sl.hist.ensure(sizeof(float) * 4 * kMaxIter * (size_t)c->shape.grid * c->shape.block), "hipMalloc history");
-  // per launch: [0] item count, [1] work counter
-  GPRM_HIP(sl.counters.ensure(sizeof(int) * 2 * std::max<size_t>(plan.size(), 1)), "hipMalloc counters");
-  GPRM_HIP(launch_fill_u32(sl.counters.as<unsigned>(), 0u, 2 * (long long)plan.size(), st), "fill");
-  for (size_t i = 0; i < plan.size(); ++i) {
-    const Launch& p = plan[i];
-    int* cnt = sl.counters.as<int>() + 2 * i;
-    GPRM_HIP(launch_edge_compact(ea, p.e0, p.e1, p.lo, p.hi, sl.item_edge.as<int>(), sl.item_pos.as<int>(), cnt, st),
-             "edge compaction launch");
-    int pcap = (p.e1 - p.e0) * (p.hi - p.lo);
-    size_t ev = 0;
-    GPRM_HIP(prof_begin(c, sl, 1, st, &ev), "event");
-    GPRM_HIP(launch_edge_fk(c->rb, ea, sl.item_edge.as<int>(), sl.item_pos.as<int>(), cnt, pcap,
-                            sl.frames.as<float>(), st),
-             "edge kinematics launch");
-    GPRM_HIP(prof_end(c, sl, ev, st), "event");
-    SvaArgs a{};
-    a.img = c->img;
-    a.ndof = nd;
-    a.frames = sl.frames.as<float>();
-    a.fstride = pcap;
-    a.hist = sl.hist.as<float4>();
-    a.nitems = -1;
-    a.nitems_dev = cnt;
-    a.counter = reinterpret_cast<unsigned*>(cnt + 1);
-    a.item_edge = sl.item_edge.as<int>();
-    a.edge_res = d_res;
-    a.pairs_per_unit = unit_pairs(c, pcap);
-    a.units_per_item = (c->img.M + a.pairs_per_unit - 1) / a.pairs_per_unit;
-    a.tally = tally_ptr(c, sl);
-    a.static_first = 1;
-    GPRM_HIP(prof_begin(c, sl, 0, st, &ev), "event");
-    GPRM_HIP(launch_sva(a, 1, launch_shape(c, (long long)((pcap + 63) / 64) * a.units_per_item), st),
-             "separating-vector launch");
-    GPRM_HIP(prof_end(c, sl, ev, st), "event");
+  unsigned long long total = 0;
+  if (need_max) {
+    GPRM_HIP(hipStreamSynchronize(sl.side), "sync");
+    std::memcpy(&maxT, sl.pinned, 4);
+    std::memcpy(&total, static_cast<char*>(sl.pinned) + 16, 8);
   }
+  const size_t first = plan.size();
+  for (const auto& w : edge_windows(maxT))
+    if (w.first >= kFirstWindow) add_window(w.first, w.second);
+  if ((rc = enqueue(first)) != GPRM_OK) return rc;
   if (stats) {
     std::vector<int> h(2 * plan.size());
     if (!plan.empty()) {
       GPRM_HIP(hipMemcpyAsync(h.data(), sl.counters.p, sizeof(int) * h.size(), hipMemcpyDeviceToHost, st), "copy");
       GPRM_HIP(hipStreamSynchronize(st), "sync");
     }
+    for (size_t i = 0; i < early.size(); ++i) h[i] = early[i];
     long long sched = 0;
     for (size_t i = 0; i < plan.size(); ++i) sched += h[2 * i];
     stats->samples_total = (long long)total;
