@@ -550,7 +550,7 @@ __global__ void fill_u32_kernel(unsigned int* p, unsigned int v, long long n) {
 struct Lds {
   const float4* verts;          // (x, y, z, info bits): info = row offset << 8 | degree
   const int4* pairs;            // (vbP | vbQ << 16, sysP | sysQ << 8, CRS index, kP | kQ << 16)
-  const float4* spheres;        // per polytope, own frame: (centre, sphere radius), (box half extents, 0)
+  const float4* spheres;        // per polytope, own frame: (centre, sphere radius), (box half extents, |c|_1 + r)
   const unsigned short* adj;    // rows of 4k neighbour ids, padded with the row's own vertex
 };
 

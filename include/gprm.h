@@ -138,7 +138,7 @@ typedef struct {
   long long units_done;        /* work units run to their end */
   long long iterations;        /* separating-vector iterations run */
   long long bound_separated;   /* pair tests whose first iteration is decided
-                                  separated by a bounding-sphere margin test */
+                                  separated by a bounding-box margin test */
 } gprm_profile_stats;
 int gprm_profile_enable(gprm_ctx* ctx, int enable);
 int gprm_profile_read(gprm_ctx* ctx, gprm_profile_stats* out);
