@@ -266,10 +266,14 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
   std::vector<int> cand_tree(batch), kept, ccode(batch);
   std::vector<float> qnew((size_t)nd * num);
   std::vector<float> prob;
+  std::vector<double> sorted_u;
+  std::vector<int> slot;
+  std::vector<size_t> bucket;
   DevArr d_cand, d_code, d_new, d_nq, d_nn, d_nd2, d_cnt, d_qs, d_qe, d_er, d_mq, d_mp;
   if (!d_cand.ensure(sizeof(float) * nd * batch) || !d_code.ensure(sizeof(int) * batch) ||
-      !d_new.ensure(sizeof(float) * nd * num) || !d_nq.ensure(sizeof(int) * prm->nbuf) ||
-      !d_nn.ensure(sizeof(int) * prm->nbuf) || !d_nd2.ensure(sizeof(float) * prm->nbuf) || !d_cnt.ensure(8))
+      !d_new.ensure(sizeof(float) * nd * num) || !d_nq.ensure(sizeof(int) * 2 * (size_t)prm->nbuf) ||
+      !d_nn.ensure(sizeof(int) * 2 * (size_t)prm->nbuf) || !d_nd2.ensure(sizeof(float) * 2 * (size_t)prm->nbuf) ||
+      !d_cnt.ensure(8))
     return GPRM_E_HIP;
   std::vector<int> hq, hn, her;
   std::vector<float> hd2, hqs, hqe, scoreT, score;
@@ -298,15 +302,36 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
     }
     while (filled < num) {
       if (++guard > 1000) break;
-      for (int k = 0; k < batch; ++k) {
-        const int g = k & 1;  // alternate trees; surplus of one tree is discarded
+      // Candidates alternate between the trees (k & 1; a tree's surplus is
+      // discarded).  A tree's base nodes are drawn as sorted variates of its
+      // weight distribution: sorted uniforms from exponential spacings, then
+      // one merge pass over the cumulative weights (the inverse-CDF draw of
+      // the reference's rule, without a search per draw).  Their candidates
+      // go to a random permutation of the tree's slots, so that the order of
+      // the batch carries no bias.
+      for (int g = 0; g < 2; ++g) {
         const Tree& t = T[g];
-        const double u = rng.uniform() * cum[g].back();
-        int m = (int)(std::upper_bound(cum[g].begin(), cum[g].end(), u) - cum[g].begin());
-        if (m >= t.n) m = t.n - 1;
-        cand_tree[k] = g;
-        for (int i = 0; i < nd; ++i)
-          candT[(size_t)batch * i + k] = t.q[(size_t)nd * m + i] - D + 2 * D * (float)rng.uniform();
+        const int kg = (batch - g + 1) / 2;  // slots k = g, g + 2, ...
+        sorted_u.resize(kg);
+        double acc = 0.0;
+        for (int j = 0; j < kg; ++j) {
+          acc -= std::log1p(-rng.uniform());
+          sorted_u[j] = acc;
+        }
+        acc -= std::log1p(-rng.uniform());
+        const double scale = cum[g].back() / acc;
+        slot.resize(kg);
+        for (int j = 0; j < kg; ++j) slot[j] = g + 2 * j;
+        for (int j = kg - 1; j > 0; --j) std::swap(slot[j], slot[rng.integer(0, j)]);
+        int m = 0;
+        for (int j = 0; j < kg; ++j) {
+          const double u = sorted_u[j] * scale;
+          while (m < t.n - 1 && cum[g][m] <= u) ++m;
+          const int k = slot[j];
+          cand_tree[k] = g;
+          for (int i = 0; i < nd; ++i)
+            candT[(size_t)batch * i + k] = t.q[(size_t)nd * m + i] - D + 2 * D * (float)rng.uniform();
+        }
       }
       // Sampling bias: every second pair of candidates must be predicted
       // free; candidates it drops are never checked.  The others, and all
@@ -358,31 +383,55 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
       return GPRM_E_HIP;
     // candidate edges (new node j, tree g, stored node, distance) in the
     // reference's buffer order: by new node, left tree before right tree
-    // (find_neighbours), at most nbuf per step
+    // (find_neighbours), then by node; at most nbuf per tree and per step.
+    // The two trees' searches run back to back into the two halves of the
+    // output arrays and are read back together.
     struct Cand { int j, g, node; float d2; };
     std::vector<Cand> cands;
-    for (int g = 0; g < 2; ++g) {
-      int zero[2] = {0, 0};
+    const size_t half = (size_t)prm->nbuf;
+    int cnt[2] = {0, 0};
+    {
+      const int zero[2] = {0, 0};
       if (hipMemcpyAsync(d_cnt.p, zero, 8, hipMemcpyHostToDevice, st) != hipSuccess) return GPRM_E_HIP;
-      if (launch_near_grid(nd, d_new.as<float>(), num, num, grid, T[g].starts.as<int>(), T[g].counts.as<int>(),
-                           T[g].sorted.as<float>(), T[g].sid.as<int>(), T[g].ndev, D2, d_nq.as<int>(), d_nn.as<int>(),
-                           d_nd2.as<float>(), d_cnt.as<int>(), prm->nbuf, st) != hipSuccess)
-        return GPRM_E_HIP;
-      int cnt = 0;
-      if (hipMemcpyAsync(&cnt, d_cnt.p, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      for (int g = 0; g < 2; ++g)
+        if (launch_near_grid(nd, d_new.as<float>(), num, num, grid, T[g].starts.as<int>(), T[g].counts.as<int>(),
+                             T[g].sorted.as<float>(), T[g].sid.as<int>(), T[g].ndev, D2, d_nq.as<int>() + g * half,
+                             d_nn.as<int>() + g * half, d_nd2.as<float>() + g * half, d_cnt.as<int>() + g,
+                             prm->nbuf, st) != hipSuccess)
+          return GPRM_E_HIP;
+      if (hipMemcpyAsync(cnt, d_cnt.p, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
           hipStreamSynchronize(st) != hipSuccess)
         return GPRM_E_HIP;
-      cnt = std::min(cnt, prm->nbuf);
-      hq.resize(cnt); hn.resize(cnt); hd2.resize(cnt);
-      if (cnt > 0 && (hipMemcpy(hq.data(), d_nq.p, 4 * (size_t)cnt, hipMemcpyDeviceToHost) != hipSuccess ||
-                      hipMemcpy(hn.data(), d_nn.p, 4 * (size_t)cnt, hipMemcpyDeviceToHost) != hipSuccess ||
-                      hipMemcpy(hd2.data(), d_nd2.p, 4 * (size_t)cnt, hipMemcpyDeviceToHost) != hipSuccess))
-        return GPRM_E_HIP;
-      for (int i = 0; i < cnt; ++i) cands.push_back({hq[i], g, hn[i], hd2[i]});
+      hq.resize(2 * half); hn.resize(2 * half); hd2.resize(2 * half);
+      for (int g = 0; g < 2; ++g) {
+        cnt[g] = std::min(cnt[g], prm->nbuf);
+        if (cnt[g] == 0) continue;
+        if (hipMemcpyAsync(hq.data() + g * half, d_nq.as<int>() + g * half, 4 * (size_t)cnt[g],
+                           hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(hn.data() + g * half, d_nn.as<int>() + g * half, 4 * (size_t)cnt[g],
+                           hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(hd2.data() + g * half, d_nd2.as<float>() + g * half, 4 * (size_t)cnt[g],
+                           hipMemcpyDeviceToHost, st) != hipSuccess)
+          return GPRM_E_HIP;
+      }
+      if (hipStreamSynchronize(st) != hipSuccess) return GPRM_E_HIP;
     }
-    std::sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) {
-      return a.j != b.j ? a.j < b.j : (a.g != b.g ? a.g < b.g : a.node < b.node);
-    });
+    // counting sort by (j, g), then each (small) group by node
+    bucket.assign(2 * (size_t)num + 1, 0);
+    for (int g = 0; g < 2; ++g)
+      for (int i = 0; i < cnt[g]; ++i) ++bucket[2 * (size_t)hq[g * half + i] + g + 1];
+    for (size_t b = 1; b < bucket.size(); ++b) bucket[b] += bucket[b - 1];
+    cands.resize((size_t)cnt[0] + cnt[1]);
+    for (int g = 0; g < 2; ++g)
+      for (int i = 0; i < cnt[g]; ++i) {
+        const size_t x = g * half + i;
+        cands[bucket[2 * (size_t)hq[x] + g]++] = Cand{hq[x], g, hn[x], hd2[x]};
+      }
+    for (size_t a = 0, b = 0; a < cands.size(); a = b) {
+      b = a + 1;
+      while (b < cands.size() && cands[b].j == cands[a].j && cands[b].g == cands[a].g) ++b;
+      std::sort(cands.begin() + a, cands.begin() + b, [](const Cand& x, const Cand& y) { return x.node < y.node; });
+    }
     if ((long long)cands.size() > prm->nbuf) cands.resize(prm->nbuf);
     res->ms_near += ms_since(tn);
 
