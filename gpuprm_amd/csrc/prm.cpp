@@ -77,7 +77,10 @@ struct Tree {
   int ndev = 0;                         // nodes [0, ndev) are on the device
   std::vector<float> q;                 // AoS ndof*n
   std::vector<int> surrnum;             // neighbours per node
-  std::vector<std::vector<std::pair<int, float>>> adj;
+  // adjacency as directed entries in insertion order (both directions of an
+  // edge); csr() groups them by node, keeping each node's order
+  std::vector<int> ef, et;
+  std::vector<float> ew;
   DevArr dq;                            // SoA ndof x cap on the device
   std::vector<float> stage;             // host staging of one upload
   // cell index of the device nodes for the radius search (rebuilt on upload)
@@ -89,8 +92,29 @@ int insert(Tree& t, const float* q) {
   int pos = t.n++;
   t.q.insert(t.q.end(), q, q + t.ndof);
   t.surrnum.push_back(0);
-  t.adj.emplace_back();
   return pos;
+}
+
+// Adjacency of a tree in CSR form: node l's entries (neighbour, weight) at
+// [off[l], off[l+1]), in the order they were added.
+struct Csr {
+  std::vector<int> off, to;
+  std::vector<float> w;
+};
+Csr csr(const Tree& t) {
+  Csr c;
+  c.off.assign((size_t)t.n + 1, 0);
+  for (int f : t.ef) ++c.off[(size_t)f + 1];
+  for (size_t l = 1; l < c.off.size(); ++l) c.off[l] += c.off[l - 1];
+  std::vector<int> cur(c.off.begin(), c.off.end() - 1);
+  c.to.resize(t.ef.size());
+  c.w.resize(t.ef.size());
+  for (size_t k = 0; k < t.ef.size(); ++k) {
+    const int x = cur[t.ef[k]]++;
+    c.to[x] = t.et[k];
+    c.w[x] = t.ew[k];
+  }
+  return c;
 }
 
 // Upload nodes [ndev, n) to the device copy, one copy per component.
@@ -122,7 +146,7 @@ bool index_nodes(Tree& t, const CellGrid& g, hipStream_t st) {
 }
 
 // Dijkstra on one tree (do_dijkstra, prmsolver.cpp:2804-2846).
-bool dijkstra(const Tree& t, int from, int to, std::vector<int>* path) {
+bool dijkstra(const Tree& t, const Csr& g, int from, int to, std::vector<int>* path) {
   std::vector<float> dist(t.n, INFINITY);
   std::vector<int> parent(t.n, -1);
   typedef std::pair<float, int> QE;
@@ -142,12 +166,13 @@ bool dijkstra(const Tree& t, int from, int to, std::vector<int>* path) {
       }
       return true;
     }
-    for (auto& nb : t.adj[e.second]) {
-      float nd = e.first + nb.second;
-      if (nd < dist[nb.first]) {
-        dist[nb.first] = nd;
-        parent[nb.first] = e.second;
-        pq.push({nd, nb.first});
+    for (int x = g.off[e.second]; x < g.off[e.second + 1]; ++x) {
+      const int nb = g.to[x];
+      float nd = e.first + g.w[x];
+      if (nd < dist[nb]) {
+        dist[nb] = nd;
+        parent[nb] = e.second;
+        pq.push({nd, nb});
       }
     }
   }
@@ -170,16 +195,17 @@ bool write_bin(const std::string& path, const T* v, size_t n) {
 // std::vector<int> edgesweight stores them.  blocknum.bin ends up holding the
 // block count (the reference writes it twice; the second write wins): one
 // block holding every node.
-bool store_tree(const std::string& dir, const Tree& t, const std::vector<int>* path, int start, int end) {
+bool store_tree(const std::string& dir, const Tree& t, const Csr& g, const std::vector<int>* path, int start,
+                int end) {
   std::error_code ec;
   std::filesystem::create_directories(dir, ec);
   if (ec) return false;
   std::vector<int> from, to, weight;
   for (int l = 0; l < t.n; ++l)
-    for (auto& e : t.adj[l]) {
+    for (int x = g.off[l]; x < g.off[l + 1]; ++x) {
       from.push_back(l);
-      to.push_back(e.first);
-      weight.push_back((int)e.second);
+      to.push_back(g.to[x]);
+      weight.push_back((int)g.w[x]);
     }
   const int n = t.n, zero = 0, one = 1;
   bool ok = write_bin(dir + "/qstorage.bin", t.q.data(), (size_t)t.ndof * n) &&
@@ -548,8 +574,13 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
         for (size_t k = e0; k < e1; ++k) {
           if (eg[k] != g || her[k] != 0) continue;
           const int other = en[k];
-          T[g].adj[pos[g]].push_back({other, ed[k]});
-          T[g].adj[other].push_back({pos[g], ed[k]});
+          Tree& tg = T[g];
+          tg.ef.push_back(pos[g]);
+          tg.et.push_back(other);
+          tg.ew.push_back(ed[k]);
+          tg.ef.push_back(other);
+          tg.et.push_back(pos[g]);
+          tg.ew.push_back(ed[k]);
           ++T[g].surrnum[other];
           ++T[g].surrnum[pos[g]];
           ++res->edges_free;
@@ -571,8 +602,11 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
   res->nodes_right = T[1].n;
   std::vector<int> pl, pr;
   bool connected = false;
+  Csr G[2];
+  if (conn_left >= 0 || prm->output_dir)
+    for (int g = 0; g < 2; ++g) G[g] = csr(T[g]);
   if (conn_left >= 0) {
-    connected = dijkstra(T[0], 0, conn_left, &pl) && dijkstra(T[1], conn_right, 0, &pr);
+    connected = dijkstra(T[0], G[0], 0, conn_left, &pl) && dijkstra(T[1], G[1], conn_right, 0, &pr);
     if (connected) {
       // pl = [conn_left, ..., start], pr = [goal, ..., conn_right]; conn_left
       // and conn_right hold the same configuration
@@ -594,8 +628,8 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
     const int conn = connected ? 1 : 0;
     std::error_code ec;
     std::filesystem::create_directories(root, ec);
-    if (ec || !store_tree(root + "/graphl", T[0], connected ? &pl : nullptr, 0, std::max(conn_left, 0)) ||
-        !store_tree(root + "/graphr", T[1], connected ? &pr : nullptr, std::max(conn_right, 0), 0) ||
+    if (ec || !store_tree(root + "/graphl", T[0], G[0], connected ? &pl : nullptr, 0, std::max(conn_left, 0)) ||
+        !store_tree(root + "/graphr", T[1], G[1], connected ? &pr : nullptr, std::max(conn_right, 0), 0) ||
         !write_bin(root + "/connection.bin", &conn, 1))
       return GPRM_E_INVALID;
   }
