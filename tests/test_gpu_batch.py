@@ -217,3 +217,32 @@ def test_batch_edges_every_shape(scenes):
         b.indicator2_dev(qs, qe, r, use_graph=True)
         for s in range(NSCENES):
             assert np.array_equal(r[s].cpu().numpy() != 0, want[s] != 0), (block, waves, ppu, s)
+
+
+def test_batch_edges_several_steps_per_window(scenes):
+    """Long edges on large scenes: a window's samples exceed a scene's
+    per-step budget (2^19), so the window runs as several steps over ranges
+    of the scene's edges; decisions still equal the single-scene path's."""
+    import gpuprm_amd
+    import torch
+    dirs, spaces = scenes
+    rng = np.random.default_rng(14)
+    sub = [spaces[0], spaces[1], spaces[2]]
+    sizes = [20000, 0, 15000]
+    QS, QE = [], []
+    for n in sizes:
+        a = rng.uniform(-3.0, 3.0, size=(7, n)).astype(np.float32)
+        d = rng.normal(size=(7, n))
+        d /= np.linalg.norm(d, axis=0, keepdims=True) + 1e-12
+        QS.append(a)
+        QE.append((a + d * rng.uniform(1.5, 2.5, size=n)).astype(np.float32))
+    qs = [torch.from_numpy(x).cuda() for x in QS]
+    qe = [torch.from_numpy(x).cuda() for x in QE]
+    r = [torch.full((n,), 99, dtype=torch.int32, device="cuda") for n in sizes]
+    b = gpuprm_amd.SceneBatch(sub)
+    done, total = b.indicator2_dev(qs, qe, r, use_graph=True)
+    want = _single_edges(sub, qs, qe)
+    for s in range(3):
+        assert np.array_equal(r[s].cpu().numpy() != 0, want[s] != 0), s
+    assert done[1] == 0 and done[0] > 0 and done[2] > 0
+    assert total == sum(int(OracleScene(dirs[s]).edge_testnum(QS[s], QE[s])[1]) for s in (0, 2))
