@@ -17,7 +17,7 @@
 namespace gprm {
 
 // Order-position windows of the edge rounds for a largest testnum maxT:
-// [0, w0), then doubling widths (defined in gprm_api.cpp).
+// [0, 16), then widths growing threefold (defined in gprm_api.cpp).
 std::vector<std::pair<int, int>> edge_windows(int maxT);
 
 struct DevBuf {

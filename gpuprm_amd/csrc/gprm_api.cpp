@@ -29,18 +29,22 @@ namespace gprm {
 namespace {
 
 constexpr int kItemBudget = 1 << 23;   // configurations per kinematics/SVA chunk
-// Sample positions in the first edge round; later rounds double.  The
-// golden-ratio order spreads the first positions along the edge, so blocked
-// edges are mostly found in the first round; on the 6-DOF edge benchmark a
-// first window of 2/4/8/16/32/64/128 gave 6.80/6.70/6.53/6.37-6.42/6.45/
-// 6.57/6.72 ms per step.
+// Sample positions in the first edge round; each later round is kWindowGrowth
+// times wider.  The golden-ratio order spreads the first positions along the
+// edge, so blocked edges are mostly found in the first round.  On the 6-DOF
+// edge benchmark, with doubling windows, a first window of 2/4/8/16/32/64/128
+// gave 6.80/6.70/6.53/6.37-6.42/6.45/6.57/6.72 ms per step; later, (first
+// window, growth) of (16, 2)/(16, 3)/(16, 4)/(16, 5)/(8, 3)/(12, 3)/(24, 3)/
+// (32, 3)/(24, 4) gave 5.76/5.56-5.61/5.61-5.62/5.61/5.77/5.90/5.62/5.69/
+// 5.82 ms.
 constexpr int kFirstWindow = 16;
+constexpr int kWindowGrowth = 3;
 
 thread_local std::string g_create_error;
 
 }  // namespace
 
-// Sample-window rounds: positions [0,w0), then doubling widths.
+// Sample-window rounds: positions [0,w0), then widths growing kWindowGrowth-fold.
 std::vector<std::pair<int, int>> edge_windows(int maxT) {
   std::vector<std::pair<int, int>> w;
   int lo = 0, width = kFirstWindow;
@@ -48,7 +52,7 @@ std::vector<std::pair<int, int>> edge_windows(int maxT) {
     int hi = (int)std::min<long long>((long long)lo + width, maxT);
     w.push_back({lo, hi});
     lo = hi;
-    if (width < (1 << 20)) width *= 2;
+    if (width < (1 << 20)) width *= kWindowGrowth;
   }
   return w;
 }
