@@ -313,10 +313,19 @@ __device__ __forceinline__ void edge_compact_one(const EdgeArgs& ea, int e, int 
   int base = 0;
   if (lane == 63 && wave_total > 0) base = atomicAdd(nitems, wave_total);
   base = __shfl(base, 63, 64);
-  int o = base + incl - cnt;
-  for (int t = 0; t < cnt; ++t) {
-    item_edge[o + t] = e;
-    item_pos[o + t] = lo + t;
+  // The wave's runs are consecutive: lane l's items start at base + its
+  // exclusive prefix.  They are written run by run with all 64 lanes, so
+  // every store instruction covers consecutive slots.
+  const int o = base + incl - cnt;
+  unsigned long long runs = __ballot(cnt > 0);
+  while (runs) {
+    const int src = __builtin_ctzll(runs);
+    runs &= runs - 1;
+    const int c = __shfl(cnt, src, 64), os = __shfl(o, src, 64), es = __shfl(e, src, 64);
+    for (int t = lane; t < c; t += 64) {
+      item_edge[os + t] = es;
+      item_pos[os + t] = lo + t;
+    }
   }
 }
 
