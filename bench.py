@@ -335,7 +335,12 @@ def leg_prm():
     for i, (a, b) in enumerate(_tasks(o3, 7, 3, SEED + 7)):
         r, path = sp3.prm_solve(a, b, num=8192, D=0.5, max_nodes=1 << 20, max_steps=400, seed=i)
         runs.append({"found": r.found, "valid": _path_ok(o3, path), "ms": r.ms_total, "steps": r.steps,
-                     "nodes": r.nodes_left + r.nodes_right, "edge_samples": r.edge_samples})
+                     "nodes": r.nodes_left + r.nodes_right, "edge_samples": r.edge_samples,
+                     "point_checks": r.point_checks,
+                     # reference-equivalent checks (edge samples + point checks) per second of the whole run
+                     "checks_per_s": (r.edge_samples + r.point_checks) / max(1e-9, r.ms_total / 1000),
+                     "ms_phases": {"sampling": r.ms_sampling, "near": r.ms_near, "edges": r.ms_edges,
+                                   "host": r.ms_host}})
     out["7dof_time_to_path"] = {"workload": "7-DOF arm, 32 hulls of 16 points, D=0.5, 8192 new nodes/step",
                                 "ms_median": float(np.median([x["ms"] for x in runs])), "runs": runs}
 
