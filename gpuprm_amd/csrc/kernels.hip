@@ -372,6 +372,23 @@ __global__ void __launch_bounds__(kFkBlock) fk_edges_multi_kernel(const EdgeTask
   if (i < n) fk_edge_item<NDOF>(T.rb, T.ea, T.item_edge, T.item_pos, i, T.frames, (long long)T.cap);
 }
 
+// Planner edges: edge e joins the step's new node enew[e] (SoA qnew, stride
+// nnew) and node enode[e] of tree etree[e] (SoA tree0 / tree1, stride cap);
+// both endpoints are copied into the SoA edge arrays qs, qe (stride E).
+__global__ void __launch_bounds__(256)
+    gather_edges_kernel(int ndof, int E, const int* __restrict__ eidx, const float* __restrict__ qnew, int nnew,
+                        const float* __restrict__ tree0, const float* __restrict__ tree1, long long cap,
+                        float* __restrict__ qs, float* __restrict__ qe) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const int j = eidx[e], g = eidx[E + e], m = eidx[2 * (size_t)E + e];
+  const float* t = g ? tree1 : tree0;
+  for (int i = 0; i < ndof; ++i) {
+    qs[e + (size_t)E * i] = qnew[j + (size_t)nnew * i];
+    qe[e + (size_t)E * i] = t[m + cap * i];
+  }
+}
+
 // Radius neighbours (get_near_vertices, prm/src/lib/prm/prmsolver.cpp:
 // 2532-2574): for query j and stored node l, |q_l - query_j|^2 < D^2 in float
 // with the reference's summation order.  One block per query; matches are
@@ -1402,6 +1419,14 @@ hipError_t launch_edge_fk_multi(const EdgeTask* d_tasks, int ntasks, int nblocks
     case 7: hipLaunchKernelGGL(fk_edges_multi_kernel<7>, grid, block, 0, st, d_tasks, ntasks); break;
     default: hipLaunchKernelGGL(fk_edges_multi_kernel<0>, grid, block, 0, st, d_tasks, ntasks); break;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_edges(int ndof, int E, const int* eidx, const float* qnew, int nnew, const float* tree0,
+                               const float* tree1, long long cap, float* qs, float* qe, hipStream_t st) {
+  if (E <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_edges_kernel, dim3((E + 255) / 256), dim3(256), 0, st, ndof, E, eidx, qnew, nnew, tree0,
+                     tree1, cap, qs, qe);
   return hipGetLastError();
 }
 

@@ -295,14 +295,15 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
   std::vector<double> sorted_u;
   std::vector<int> slot;
   std::vector<size_t> bucket;
-  DevArr d_cand, d_code, d_new, d_nq, d_nn, d_nd2, d_cnt, d_qs, d_qe, d_er, d_mq, d_mp;
+  DevArr d_cand, d_code, d_new, d_nq, d_nn, d_nd2, d_cnt, d_qs, d_qe, d_er, d_mq, d_mp, d_eidx;
+  std::vector<int> heidx;
   if (!d_cand.ensure(sizeof(float) * nd * batch) || !d_code.ensure(sizeof(int) * batch) ||
       !d_new.ensure(sizeof(float) * nd * num) || !d_nq.ensure(sizeof(int) * 2 * (size_t)prm->nbuf) ||
       !d_nn.ensure(sizeof(int) * 2 * (size_t)prm->nbuf) || !d_nd2.ensure(sizeof(float) * 2 * (size_t)prm->nbuf) ||
       !d_cnt.ensure(8))
     return GPRM_E_HIP;
   std::vector<int> hq, hn, her;
-  std::vector<float> hd2, hqs, hqe, scoreT, score;
+  std::vector<float> hd2, scoreT, score;
 
   int conn_left = -1, conn_right = -1;
   for (int step = 0; step < prm->max_steps && conn_left < 0; ++step) {
@@ -532,17 +533,18 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
     her.assign(E, 1);
     if (E > 0) {
       if (!d_qs.ensure(sizeof(float) * nd * (size_t)E) || !d_qe.ensure(sizeof(float) * nd * (size_t)E) ||
-          !d_er.ensure(sizeof(int) * (size_t)E))
+          !d_er.ensure(sizeof(int) * (size_t)E) || !d_eidx.ensure(sizeof(int) * 3 * (size_t)E))
         return GPRM_E_HIP;
-      hqs.resize((size_t)nd * E);
-      hqe.resize((size_t)nd * E);
-      for (int e = 0; e < E; ++e)
-        for (int i = 0; i < nd; ++i) {
-          hqs[(size_t)E * i + e] = qnew[(size_t)nd * eq[e] + i];
-          hqe[(size_t)E * i + e] = T[eg[e]].q[(size_t)nd * en[e] + i];
-        }
-      if (hipMemcpyAsync(d_qs.p, hqs.data(), sizeof(float) * nd * E, hipMemcpyHostToDevice, st) != hipSuccess ||
-          hipMemcpyAsync(d_qe.p, hqe.data(), sizeof(float) * nd * E, hipMemcpyHostToDevice, st) != hipSuccess)
+      // endpoints gathered on the device from the new nodes (d_new) and the
+      // trees' device copies; only the indices travel
+      heidx.resize(3 * (size_t)E);
+      std::copy(eq.begin(), eq.end(), heidx.begin());
+      std::copy(eg.begin(), eg.end(), heidx.begin() + E);
+      std::copy(en.begin(), en.end(), heidx.begin() + 2 * (size_t)E);
+      if (hipMemcpyAsync(d_eidx.p, heidx.data(), sizeof(int) * 3 * (size_t)E, hipMemcpyHostToDevice, st) !=
+              hipSuccess ||
+          launch_gather_edges(nd, E, d_eidx.as<int>(), d_new.as<float>(), num, T[0].dq.as<float>(),
+                              T[1].dq.as<float>(), T[0].cap, d_qs.as<float>(), d_qe.as<float>(), st) != hipSuccess)
         return GPRM_E_HIP;
       gprm_edge_stats es;
       rc = gprm_indicator2_dev(ctx, d_qs.as<float>(), d_qe.as<float>(), d_er.as<int>(), E, E, st, &es);

@@ -189,6 +189,11 @@ hipError_t launch_near(int ndof, const float* queries, int nq, long long qoff, c
                        long long noff, float D2, int* out_q, int* out_node, float* out_d2, int* count, int cap,
                        hipStream_t st);
 hipError_t launch_fill_u32(unsigned int* p, unsigned int v, long long n, hipStream_t st);
+// Planner edge endpoints: eidx holds E new-node indices, then E tree
+// indices (0/1), then E node indices; qnew is SoA with stride nnew, the trees
+// SoA with stride cap; qs, qe receive SoA with stride E.
+hipError_t launch_gather_edges(int ndof, int E, const int* eidx, const float* qnew, int nnew, const float* tree0,
+                               const float* tree1, long long cap, float* qs, float* qe, hipStream_t st);
 
 // Launch shape of the persistent separating-vector kernel: a kernel variant
 // (block size, occupancy target in waves per SIMD), the pairs of one work
