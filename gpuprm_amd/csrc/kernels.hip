@@ -487,27 +487,28 @@ __global__ void grid_scatter_kernel(int ndof, const float* __restrict__ nodes, i
 }
 
 // Radius search over the cell index: one block per query; the nodes of the
-// (up to 27) neighbouring cells form one index range that the block's
-// threads stride through.  Same squared distance (same summation order) and
-// same append as near_kernel.
+// (up to 3^kGridDims = 81) neighbouring cells form one index range that the
+// block's threads stride through.  Same squared distance (same summation
+// order) and same append as near_kernel.
 __global__ void __launch_bounds__(256)
     near_grid_kernel(int ndof, const float* __restrict__ queries, int nq, long long qoff, CellGrid g,
                      const int* __restrict__ starts, const int* __restrict__ counts, const float* __restrict__ sorted,
                      const int* __restrict__ sid, int n, float D2, int* __restrict__ out_q,
                      int* __restrict__ out_node, float* __restrict__ out_d2, int* __restrict__ count, int cap) {
-  __shared__ int s_start[27], s_pre[28];
+  constexpr int NK = 81;  // 3^kGridDims neighbour cells
+  __shared__ int s_start[NK], s_pre[NK + 1];
   const int j = blockIdx.x;
   if (j >= nq) return;
   float qr[kMaxDofDev];
   for (int i = 0; i < ndof; ++i) qr[i] = queries[j + qoff * i];
-  if (threadIdx.x < 27) {
+  if (threadIdx.x < NK) {
     // neighbour k: offset digit[d] - 1 in grid dimension d; a dimension
-    // without grid (ndof < 3) takes only offset 0
+    // without grid (d >= dims) takes only offset 0
     const int k = threadIdx.x;
-    const int digit[3] = {k % 3, (k / 3) % 3, k / 9};
+    const int digit[kGridDims] = {k % 3, (k / 3) % 3, (k / 9) % 3, k / 27};
     int c = 0;
     bool ok = true;
-    for (int d = 2; d >= 0; --d) {
+    for (int d = kGridDims - 1; d >= 0; --d) {
       if (d >= g.dims) {
         if (digit[d] != 1) ok = false;
         continue;
@@ -522,10 +523,10 @@ __global__ void __launch_bounds__(256)
   __syncthreads();
   if (threadIdx.x == 0) {
     s_pre[0] = 0;
-    for (int k = 0; k < 27; ++k) s_pre[k + 1] += s_pre[k];
+    for (int k = 0; k < NK; ++k) s_pre[k + 1] += s_pre[k];
   }
   __syncthreads();
-  const int total = s_pre[27];
+  const int total = s_pre[NK];
   const int lane = threadIdx.x & 63;
   for (int base = 0; base < total; base += blockDim.x) {
     const int idx = base + threadIdx.x;
@@ -533,8 +534,10 @@ __global__ void __launch_bounds__(256)
     bool hit = false;
     int node = 0;
     if (idx < total) {
+      // the neighbour cell holding idx: last k with s_pre[k] <= idx
       int k = 0;
-      while (s_pre[k + 1] <= idx) ++k;
+      for (int step = 64; step > 0; step >>= 1)
+        if (k + step < NK && s_pre[k + step] <= idx) k += step;
       const int pos = s_start[k] + (idx - s_pre[k]);
       for (int i = 0; i < ndof; ++i) {
         float diff = sorted[pos + (size_t)n * i] - qr[i];
@@ -1439,11 +1442,22 @@ hipError_t launch_near(int ndof, const float* queries, int nq, long long qoff, c
   return hipGetLastError();
 }
 
+CellGrid make_cell_grid_dims(int dims, const float* mins, const float* maxs, float D);
+
 CellGrid make_cell_grid(int ndof, const float* mins, const float* maxs, float D) {
+  // as many gridded coordinates as kGridMaxCells allows (at most kGridDims)
+  for (int dims = ndof < kGridDims ? ndof : kGridDims; dims >= 1; --dims) {
+    CellGrid g = make_cell_grid_dims(dims, mins, maxs, D);
+    if (g.ncells <= kGridMaxCells || dims == 1) return g;
+  }
+  return make_cell_grid_dims(1, mins, maxs, D);
+}
+
+CellGrid make_cell_grid_dims(int dims, const float* mins, const float* maxs, float D) {
   CellGrid g{};
-  g.dims = ndof < 3 ? ndof : 3;
+  g.dims = dims;
   g.ncells = 1;
-  for (int d = 0; d < 3; ++d) {
+  for (int d = 0; d < kGridDims; ++d) {
     g.cells[d] = 1;
     g.lo[d] = 0.0f;
     g.inv[d] = 0.0f;

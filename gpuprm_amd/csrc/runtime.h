@@ -96,11 +96,14 @@ struct FkTask {
 };
 
 // Cell index of the radius search: nodes binned by their first `dims`
-// coordinates (cells at least D wide, at most 64 per dimension).
+// coordinates (cells at least D wide, at most 64 per dimension; up to
+// kGridDims dimensions while the cells number at most kGridMaxCells).
+constexpr int kGridDims = 4;
+constexpr int kGridMaxCells = 1 << 20;
 struct CellGrid {
   int dims;
-  int cells[3];
-  float lo[3], inv[3];   // cell = clamp(floor((x - lo) * inv), 0, cells - 1)
+  int cells[kGridDims];
+  float lo[kGridDims], inv[kGridDims];   // cell = clamp(floor((x - lo) * inv), 0, cells - 1)
   int ncells;
 };
 
