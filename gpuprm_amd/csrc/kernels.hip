@@ -372,6 +372,42 @@ __global__ void __launch_bounds__(kFkBlock) fk_edges_multi_kernel(const EdgeTask
   if (i < n) fk_edge_item<NDOF>(T.rb, T.ea, T.item_edge, T.item_pos, i, T.frames, (long long)T.cap);
 }
 
+// Planner candidates: slot k of a batch (tree k & 1) is node base[k] of its
+// tree (SoA tree0 / tree1, stride cap) plus an offset uniform in [-D, D) per
+// coordinate, q - D + 2 D u.  The u come from a counter-based generator
+// (splitmix64 of key + k ndof + i, top 53 bits), so a batch is a function of
+// its key.
+__device__ __forceinline__ double counter_uniform(unsigned long long key, unsigned long long ctr) {
+  unsigned long long z = key + (ctr + 1ull) * 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * 0x1.0p-53;
+}
+__global__ void __launch_bounds__(256)
+    sample_candidates_kernel(int ndof, int batch, const int* __restrict__ base, const float* __restrict__ tree0,
+                             const float* __restrict__ tree1, long long cap, float D, unsigned long long key,
+                             float* __restrict__ cand) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= batch) return;
+  const float* t = (k & 1) ? tree1 : tree0;
+  const int m = base[k];
+  for (int i = 0; i < ndof; ++i) {
+    const float u = (float)counter_uniform(key, (unsigned long long)k * ndof + i);
+    cand[k + (size_t)batch * i] = t[m + cap * i] - D + 2 * D * u;
+  }
+}
+
+// Columns idx[0..n) of the SoA array src (stride ns) into dst (stride n).
+__global__ void __launch_bounds__(256)
+    gather_columns_kernel(int ndof, int n, const int* __restrict__ idx, const float* __restrict__ src, int ns,
+                          float* __restrict__ dst) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const int k = idx[a];
+  for (int i = 0; i < ndof; ++i) dst[a + (size_t)n * i] = src[k + (size_t)ns * i];
+}
+
 // Planner edges: edge e joins the step's new node enew[e] (SoA qnew, stride
 // nnew) and node enode[e] of tree etree[e] (SoA tree0 / tree1, stride cap);
 // both endpoints are copied into the SoA edge arrays qs, qe (stride E).
@@ -1422,6 +1458,21 @@ hipError_t launch_edge_fk_multi(const EdgeTask* d_tasks, int ntasks, int nblocks
     case 7: hipLaunchKernelGGL(fk_edges_multi_kernel<7>, grid, block, 0, st, d_tasks, ntasks); break;
     default: hipLaunchKernelGGL(fk_edges_multi_kernel<0>, grid, block, 0, st, d_tasks, ntasks); break;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_sample_candidates(int ndof, int batch, const int* base, const float* tree0, const float* tree1,
+                                    long long cap, float D, unsigned long long key, float* cand, hipStream_t st) {
+  if (batch <= 0) return hipSuccess;
+  hipLaunchKernelGGL(sample_candidates_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, ndof, batch, base, tree0,
+                     tree1, cap, D, key, cand);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_columns(int ndof, int n, const int* idx, const float* src, int ns, float* dst,
+                                 hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_columns_kernel, dim3((n + 255) / 256), dim3(256), 0, st, ndof, n, idx, src, ns, dst);
   return hipGetLastError();
 }
 

@@ -288,19 +288,22 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
   const int num = prm->num;
   const float D = prm->D, D2 = prm->D * prm->D;
   const int batch = 4 * num;  // candidates drawn per exact point-check batch
-  std::vector<float> candT((size_t)nd * batch), keptT;
+  std::vector<float> keptT;
   std::vector<int> cand_tree(batch), kept, ccode(batch);
   std::vector<float> qnew((size_t)nd * num);
   std::vector<float> prob;
   std::vector<double> sorted_u;
-  std::vector<int> slot;
+  std::vector<int> slot, base(batch);
   std::vector<size_t> bucket;
-  DevArr d_cand, d_code, d_new, d_nq, d_nn, d_nd2, d_cnt, d_qs, d_qe, d_er, d_mq, d_mp, d_eidx;
+  DevArr d_cand, d_code, d_new, d_nq, d_nn, d_nd2, d_cnt, d_qs, d_qe, d_er, d_mq, d_mp, d_eidx, d_base, d_kidx,
+      d_kept;
   std::vector<int> heidx;
   if (!d_cand.ensure(sizeof(float) * nd * batch) || !d_code.ensure(sizeof(int) * batch) ||
       !d_new.ensure(sizeof(float) * nd * num) || !d_nq.ensure(sizeof(int) * 2 * (size_t)prm->nbuf) ||
       !d_nn.ensure(sizeof(int) * 2 * (size_t)prm->nbuf) || !d_nd2.ensure(sizeof(float) * 2 * (size_t)prm->nbuf) ||
-      !d_cnt.ensure(8))
+      !d_cnt.ensure(8) || !d_base.ensure(sizeof(int) * batch) ||
+      (bias && (!d_mp.ensure(sizeof(float) * batch) || !d_kidx.ensure(sizeof(int) * batch) ||
+                !d_kept.ensure(sizeof(float) * nd * batch))))
     return GPRM_E_HIP;
   std::vector<int> hq, hn, her;
   std::vector<float> hd2, scoreT, score;
@@ -356,37 +359,51 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
           while (m < t.n - 1 && cum[g][m] <= u) ++m;
           const int k = slot[j];
           cand_tree[k] = g;
-          for (int i = 0; i < nd; ++i)
-            candT[(size_t)batch * i + k] = t.q[(size_t)nd * m + i] - D + 2 * D * (float)rng.uniform();
+          base[k] = m;
         }
       }
+      // the candidates themselves are formed on the device from the trees'
+      // device copies (offsets from a counter-based generator keyed per batch)
+      if (hipMemcpyAsync(d_base.p, base.data(), sizeof(int) * batch, hipMemcpyHostToDevice, st) != hipSuccess ||
+          launch_sample_candidates(nd, batch, d_base.as<int>(), T[0].dq.as<float>(), T[1].dq.as<float>(), T[0].cap,
+                                   D, rng.next(), d_cand.as<float>(), st) != hipSuccess)
+        return GPRM_E_HIP;
       // Sampling bias: every second pair of candidates must be predicted
       // free; candidates it drops are never checked.  The others, and all
       // candidates without a model, go to the exact check.
       kept.clear();
+      const float* d_chk = d_cand.as<float>();
       if (bias) {
         auto tm = Clock::now();
-        rc = predict_host(model, candT, nd, batch, d_mq, d_mp, &prob, st);
-        if (rc != GPRM_OK) return rc;
+        prob.resize(batch);
+        if (surrogate_predict(model, d_cand.as<float>(), batch, batch, d_mp.as<float>(), st) != 0 ||
+            hipMemcpyAsync(prob.data(), d_mp.p, sizeof(float) * batch, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+          return GPRM_E_HIP;
         res->model_predictions += batch;
         res->ms_model += ms_since(tm);
         for (int k = 0; k < batch; ++k)
           if (!((k & 2) && prob[k] > 0.5f)) kept.push_back(k);
+        const int nk = (int)kept.size();
+        if (nk < batch) {
+          if (hipMemcpyAsync(d_kidx.p, kept.data(), sizeof(int) * nk, hipMemcpyHostToDevice, st) != hipSuccess ||
+              launch_gather_columns(nd, nk, d_kidx.as<int>(), d_cand.as<float>(), batch, d_kept.as<float>(), st) !=
+                  hipSuccess)
+            return GPRM_E_HIP;
+          d_chk = d_kept.as<float>();
+        }
       } else {
         kept.resize(batch);
         std::iota(kept.begin(), kept.end(), 0);
       }
       const int nk = (int)kept.size();
       res->candidates_skipped += batch - nk;
-      keptT.resize((size_t)nd * nk);
-      for (int i = 0; i < nd; ++i)
-        for (int a = 0; a < nk; ++a) keptT[(size_t)nk * i + a] = candT[(size_t)batch * i + kept[a]];
       if (nk == 0) continue;
-      if (hipMemcpyAsync(d_cand.p, keptT.data(), sizeof(float) * nd * nk, hipMemcpyHostToDevice, st) != hipSuccess)
-        return GPRM_E_HIP;
-      rc = gprm_indicator_dev(ctx, d_cand.as<float>(), d_code.as<int>(), nk, nk, st);
+      rc = gprm_indicator_dev(ctx, d_chk, d_code.as<int>(), nk, nk, st);
       if (rc != GPRM_OK) return rc;
+      keptT.resize((size_t)nd * nk);
       if (hipMemcpyAsync(ccode.data(), d_code.p, sizeof(int) * nk, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipMemcpyAsync(keptT.data(), d_chk, sizeof(float) * nd * nk, hipMemcpyDeviceToHost, st) != hipSuccess ||
           hipStreamSynchronize(st) != hipSuccess)
         return GPRM_E_HIP;
       res->point_checks += nk;

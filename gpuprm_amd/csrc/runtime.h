@@ -192,6 +192,14 @@ hipError_t launch_near(int ndof, const float* queries, int nq, long long qoff, c
                        long long noff, float D2, int* out_q, int* out_node, float* out_d2, int* count, int cap,
                        hipStream_t st);
 hipError_t launch_fill_u32(unsigned int* p, unsigned int v, long long n, hipStream_t st);
+// Planner candidates: slot k (tree k & 1) = node base[k] of its tree (SoA,
+// stride cap) plus a uniform offset in [-D, D) per coordinate from a
+// counter-based generator keyed by key; cand is SoA with stride batch.
+hipError_t launch_sample_candidates(int ndof, int batch, const int* base, const float* tree0, const float* tree1,
+                                    long long cap, float D, unsigned long long key, float* cand, hipStream_t st);
+// Columns idx[0..n) of the SoA array src (stride ns) into dst (stride n).
+hipError_t launch_gather_columns(int ndof, int n, const int* idx, const float* src, int ns, float* dst,
+                                 hipStream_t st);
 // Planner edge endpoints: eidx holds E new-node indices, then E tree
 // indices (0/1), then E node indices; qnew is SoA with stride nnew, the trees
 // SoA with stride cap; qs, qe receive SoA with stride E.
