@@ -398,6 +398,26 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Points scored along planner edges: for candidate a (new node idx[a], tree
+// idx[n + a], node idx[2n + a]) and s < S, the point qa + t (qb - qa) with
+// t = (s + 1) / (S + 1), into out (SoA, stride n S, column a S + s).
+__global__ void __launch_bounds__(256)
+    score_points_kernel(int ndof, int n, int S, const int* __restrict__ idx, const float* __restrict__ qnew, int nnew,
+                        const float* __restrict__ tree0, const float* __restrict__ tree1, long long cap,
+                        float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n * S) return;
+  const int a = x / S, s = x - a * S;
+  const int j = idx[a], g = idx[n + a], m = idx[2 * (size_t)n + a];
+  const float* t = g ? tree1 : tree0;
+  const float w = (float)(s + 1) / (float)(S + 1);
+  const size_t np = (size_t)n * S;
+  for (int i = 0; i < ndof; ++i) {
+    const float qa = qnew[j + (size_t)nnew * i], qb = t[m + cap * i];
+    out[x + np * i] = qa + w * (qb - qa);
+  }
+}
+
 // Columns idx[0..n) of the SoA array src (stride ns) into dst (stride n).
 __global__ void __launch_bounds__(256)
     gather_columns_kernel(int ndof, int n, const int* __restrict__ idx, const float* __restrict__ src, int ns,
@@ -1466,6 +1486,15 @@ hipError_t launch_sample_candidates(int ndof, int batch, const int* base, const 
   if (batch <= 0) return hipSuccess;
   hipLaunchKernelGGL(sample_candidates_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, ndof, batch, base, tree0,
                      tree1, cap, D, key, cand);
+  return hipGetLastError();
+}
+
+hipError_t launch_score_points(int ndof, int n, int S, const int* idx, const float* qnew, int nnew, const float* tree0,
+                               const float* tree1, long long cap, float* out, hipStream_t st) {
+  if (n <= 0 || S <= 0) return hipSuccess;
+  const long long total = (long long)n * S;
+  hipLaunchKernelGGL(score_points_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, ndof, n, S, idx, qnew,
+                     nnew, tree0, tree1, cap, out);
   return hipGetLastError();
 }
 

@@ -225,20 +225,6 @@ bool store_tree(const std::string& dir, const Tree& t, const Csr& g, const std::
   return ok;
 }
 
-// Model probabilities of n host configurations (SoA with stride n).
-int predict_host(Surrogate* model, const std::vector<float>& qT, int nd, int n, DevArr& d_q, DevArr& d_p,
-                 std::vector<float>* prob, hipStream_t st) {
-  prob->resize(n);
-  if (n == 0) return GPRM_OK;
-  if (!d_q.ensure(sizeof(float) * nd * (size_t)n) || !d_p.ensure(sizeof(float) * (size_t)n)) return GPRM_E_HIP;
-  if (hipMemcpyAsync(d_q.p, qT.data(), sizeof(float) * nd * (size_t)n, hipMemcpyHostToDevice, st) != hipSuccess ||
-      surrogate_predict(model, d_q.as<float>(), n, n, d_p.as<float>(), st) != 0 ||
-      hipMemcpyAsync(prob->data(), d_p.p, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, st) != hipSuccess ||
-      hipStreamSynchronize(st) != hipSuccess)
-    return GPRM_E_HIP;
-  return GPRM_OK;
-}
-
 }  // namespace
 }  // namespace gprm
 
@@ -306,7 +292,7 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
                 !d_kept.ensure(sizeof(float) * nd * batch))))
     return GPRM_E_HIP;
   std::vector<int> hq, hn, her;
-  std::vector<float> hd2, scoreT, score;
+  std::vector<float> hd2, score;
 
   int conn_left = -1, conn_right = -1;
   for (int step = 0; step < prm->max_steps && conn_left < 0; ++step) {
@@ -495,20 +481,32 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
         for (size_t a = 0; a + 1 < gstart.size(); ++a)
           if (gstart[a + 1] - gstart[a] > (size_t)B)
             for (size_t i = gstart[a]; i < gstart[a + 1]; ++i) need.push_back(i);
-        const int np = (int)(need.size() * kScorePoints);
-        scoreT.resize((size_t)nd * np);
-        for (size_t a = 0; a < need.size(); ++a) {
+        const int nn = (int)need.size();
+        const int np = nn * kScorePoints;
+        // the scored points are formed on the device from the new nodes and
+        // the trees' device copies; only the indices travel
+        heidx.resize(3 * (size_t)nn);
+        for (int a = 0; a < nn; ++a) {
           const Cand& c = cands[need[a]];
-          const float* qa = &qnew[(size_t)nd * c.j];
-          const float* qb = &T[c.g].q[(size_t)nd * c.node];
-          for (int s = 0; s < kScorePoints; ++s) {
-            const float t = (float)(s + 1) / (float)(kScorePoints + 1);
-            for (int i = 0; i < nd; ++i)
-              scoreT[(size_t)np * i + a * kScorePoints + s] = qa[i] + t * (qb[i] - qa[i]);
-          }
+          heidx[a] = c.j;
+          heidx[nn + a] = c.g;
+          heidx[2 * (size_t)nn + a] = c.node;
         }
-        rc = predict_host(model, scoreT, nd, np, d_mq, d_mp, &prob, st);
-        if (rc != GPRM_OK) return rc;
+        prob.resize(np);
+        if (np > 0) {
+          if (!d_eidx.ensure(sizeof(int) * 3 * (size_t)nn) || !d_mq.ensure(sizeof(float) * nd * (size_t)np) ||
+              !d_mp.ensure(sizeof(float) * (size_t)np))
+            return GPRM_E_HIP;
+          if (hipMemcpyAsync(d_eidx.p, heidx.data(), sizeof(int) * 3 * (size_t)nn, hipMemcpyHostToDevice, st) !=
+                  hipSuccess ||
+              launch_score_points(nd, nn, kScorePoints, d_eidx.as<int>(), d_new.as<float>(), num, T[0].dq.as<float>(),
+                                  T[1].dq.as<float>(), T[0].cap, d_mq.as<float>(), st) != hipSuccess ||
+              surrogate_predict(model, d_mq.as<float>(), np, np, d_mp.as<float>(), st) != 0 ||
+              hipMemcpyAsync(prob.data(), d_mp.p, sizeof(float) * (size_t)np, hipMemcpyDeviceToHost, st) !=
+                  hipSuccess ||
+              hipStreamSynchronize(st) != hipSuccess)
+            return GPRM_E_HIP;
+        }
         res->model_predictions += np;
         for (size_t a = 0; a < need.size(); ++a) {
           float m = 0.0f;

@@ -197,6 +197,11 @@ hipError_t launch_fill_u32(unsigned int* p, unsigned int v, long long n, hipStre
 // counter-based generator keyed by key; cand is SoA with stride batch.
 hipError_t launch_sample_candidates(int ndof, int batch, const int* base, const float* tree0, const float* tree1,
                                     long long cap, float D, unsigned long long key, float* cand, hipStream_t st);
+// Points scored along planner edges: candidate a (new node idx[a], tree
+// idx[n + a], node idx[2n + a]), s < S: qa + t (qb - qa), t = (s+1)/(S+1),
+// into out (SoA, stride n S, column a S + s).
+hipError_t launch_score_points(int ndof, int n, int S, const int* idx, const float* qnew, int nnew, const float* tree0,
+                               const float* tree1, long long cap, float* out, hipStream_t st);
 // Columns idx[0..n) of the SoA array src (stride ns) into dst (stride n).
 hipError_t launch_gather_columns(int ndof, int n, const int* idx, const float* src, int ns, float* dst,
                                  hipStream_t st);
