@@ -325,12 +325,12 @@ int run_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n,
   const int nchunks = (n + chunk - 1) / chunk;
   GPRM_HIP(sl.hist.ensure(sizeof(float) * 4 * kMaxIter * (size_t)c->shape.grid * c->shape.block), "hipMalloc history");
   GPRM_HIP(sl.counters.ensure(sizeof(unsigned) * (size_t)nchunks), "hipMalloc counters");
-  GPRM_HIP(launch_fill_u32(sl.counters.as<unsigned>(), 0u, nchunks, st), "fill");
   for (int ci = 0; ci < nchunks; ++ci) {
     int c0 = ci * chunk, cn = std::min(chunk, n - c0);
     size_t ev = 0;
     GPRM_HIP(prof_begin(c, sl, 1, st, &ev), "event");
-    GPRM_HIP(launch_fk_points(c->rb, d_q + c0, offset, cn, sl.frames.as<float>(), sl.status.as<int>(), 1, st),
+    GPRM_HIP(launch_fk_points(c->rb, d_q + c0, offset, cn, sl.frames.as<float>(), sl.status.as<int>(), 1,
+                              sl.counters.as<unsigned>() + ci, st),
              "kinematics launch");
     GPRM_HIP(prof_end(c, sl, ev, st), "event");
     SvaArgs a{};

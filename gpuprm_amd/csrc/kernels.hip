@@ -176,8 +176,11 @@ constexpr int kWordOutOfBounds = 2;
 template <int NDOF>
 __global__ void __launch_bounds__(kFkBlock)
     fk_points_kernel(DevRobot rb, const float* __restrict__ q, long long offset, int n,
-                     float* __restrict__ frames, long long fstride, int* __restrict__ word, int check_bounds) {
+                     float* __restrict__ frames, long long fstride, int* __restrict__ word, int check_bounds,
+                     unsigned* __restrict__ zero_counter) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // the separating-vector launch that follows claims units from this counter
+  if (i == 0 && zero_counter != nullptr) *zero_counter = 0u;
   if (i >= n) return;
   const int nd = NDOF > 0 ? NDOF : rb.ndof;
   float qq[kMaxDofDev];
@@ -1289,11 +1292,11 @@ int sva_max_lds_bytes() { return 160 * 1024; }
   }
 
 hipError_t launch_fk_points(const DevRobot& rb, const float* q, long long offset, int n, float* frames,
-                            int* word, int check_bounds, hipStream_t st) {
+                            int* word, int check_bounds, unsigned* zero_counter, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   dim3 grid((n + kFkBlock - 1) / kFkBlock), block(kFkBlock);
   GPRM_FK_DISPATCH(fk_points_kernel, grid, block, st, rb, q, offset, n, frames, (long long)n, word,
-                   check_bounds);
+                   check_bounds, zero_counter);
   return hipGetLastError();
 }
 

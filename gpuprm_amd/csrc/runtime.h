@@ -154,8 +154,10 @@ size_t sva_lds_bytes(const SceneImg& img, int block);
 int sva_max_lds_bytes();
 
 // Kinematics of n configurations; frames link-major with item stride n.
+// zero_counter (may be null) is set to zero by the launch (the work counter
+// of the separating-vector launch that follows).
 hipError_t launch_fk_points(const DevRobot& rb, const float* q, long long offset, int n, float* frames,
-                            int* word, int check_bounds, hipStream_t st);
+                            int* word, int check_bounds, unsigned* zero_counter, hipStream_t st);
 hipError_t launch_finalize_points(const int* word, int* codes, int n, hipStream_t st);
 struct SvaShape;
 hipError_t launch_sva(const SvaArgs& a, int edge_mode, const SvaShape& sh, hipStream_t st);
