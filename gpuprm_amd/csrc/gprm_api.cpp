@@ -28,7 +28,11 @@
 namespace gprm {
 namespace {
 
-constexpr int kItemBudget = 1 << 23;   // configurations per kinematics/SVA chunk
+// Configurations per kinematics / separating-vector launch: 2^24 (frames of
+// up to 5.4 GB for a 7-DOF arm; HBM holds 288 GB).  On the 6-DOF edge
+// benchmark 2^24 runs each window in one launch and measured 5.38 against
+// 5.47 ms per step for 2^23.
+constexpr int kItemBudget = 1 << 24;
 // Sample positions in the first edge round; each later round is kWindowGrowth
 // times wider.  The golden-ratio order spreads the first positions along the
 // edge, so blocked edges are mostly found in the first round.  On the 6-DOF
@@ -58,7 +62,6 @@ std::vector<std::pair<int, int>> edge_windows(int maxT) {
 }
 
 }  // namespace gprm
-
 
 namespace gprm {
 namespace {
