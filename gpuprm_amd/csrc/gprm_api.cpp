@@ -54,6 +54,10 @@ std::vector<std::pair<int, int>> edge_windows(int maxT) {
   int lo = 0, width = kFirstWindow;
   while (lo < maxT) {
     int hi = (int)std::min<long long>((long long)lo + width, maxT);
+    // A remainder under a quarter of this window joins it: a round of its
+    // own costs about 50 us (four launches and their gaps) however few
+    // samples it holds.
+    if ((long long)maxT - hi < width / 4) hi = maxT;
     w.push_back({lo, hi});
     lo = hi;
     if (width < (1 << 20)) width *= kWindowGrowth;

@@ -258,6 +258,7 @@ __device__ int gcd_i(int a, int b) {
 // testnum (robotconfigspace.cpp:660-673) and the sample order stride: a
 // golden-ratio step coprime with testnum, so that positions 0,1,2,... visit
 // every sample once, spread along the edge.
+template <int BLOCK>
 __device__ __forceinline__ void edge_prologue_one(int ndof, float dq, const EdgeArgs& ea, int k,
                                                   int* __restrict__ testnum, int* __restrict__ stride,
                                                   int* __restrict__ max_testnum,
@@ -280,22 +281,49 @@ __device__ __forceinline__ void edge_prologue_one(int ndof, float dq, const Edge
     testnum[k] = T;
     stride[k] = g;
     ea.edge_res[k] = 0;
-    atomicMax(max_testnum, T);
-    atomicAdd(total, (unsigned long long)T);
+  }
+  // One atomic pair per block: same-address atomics from all CUs serialise
+  // at the memory side (about 15 ns each), and one pair per wave took 45 us
+  // for 100k edges.  Every thread of the block calls this function (threads
+  // past the edges bring T = 0).
+  __shared__ int s_max[BLOCK / 64];
+  __shared__ unsigned long long s_sum[BLOCK / 64];
+  int tmax = T;
+  unsigned long long tsum = (unsigned long long)T;
+  for (int d = 32; d >= 1; d >>= 1) {
+    tmax = max(tmax, __shfl_xor(tmax, d, 64));
+    tsum += __shfl_xor(tsum, d, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    s_max[threadIdx.x >> 6] = tmax;
+    s_sum[threadIdx.x >> 6] = tsum;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < BLOCK / 64; ++w) {
+      tmax = max(tmax, s_max[w]);
+      tsum += s_sum[w];
+    }
+    if (tmax > 0) {
+      atomicMax(max_testnum, tmax);
+      atomicAdd(total, tsum);
+    }
   }
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
     edge_prologue_kernel(int ndof, float dq, EdgeArgs ea, int* __restrict__ testnum,
                          int* __restrict__ stride, int* __restrict__ max_testnum,
                          unsigned long long* __restrict__ total) {
-  edge_prologue_one(ndof, dq, ea, blockIdx.x * blockDim.x + threadIdx.x, testnum, stride, max_testnum, total);
+  edge_prologue_one<1024>(ndof, dq, ea, blockIdx.x * 1024 + threadIdx.x, testnum, stride, max_testnum, total);
 }
 
 // Items for order positions [lo, hi) of every open edge in [e0, e1).
-// One atomic per wave: lanes reserve contiguous slots by prefix count.
-// Every lane of the wave calls it (the prefix sum is wave-wide); lanes with
-// e >= e1 contribute nothing.
+// One atomic per block reserves the block's slots (same-address atomics
+// serialise at the memory side; one per wave took 23 us for 100k edges);
+// each lane's slots follow by prefix count.  Every thread of the block calls
+// it; threads with e >= e1 contribute nothing.
+template <int BLOCK>
 __device__ __forceinline__ void edge_compact_one(const EdgeArgs& ea, int e, int e1, int lo, int hi,
                                                  int* __restrict__ item_edge, int* __restrict__ item_pos,
                                                  int* __restrict__ nitems) {
@@ -312,14 +340,25 @@ __device__ __forceinline__ void edge_compact_one(const EdgeArgs& ea, int e, int 
     int v = __shfl_up(incl, d, 64);
     if (lane >= d) incl += v;
   }
-  int wave_total = __shfl(incl, 63, 64);
-  int base = 0;
-  if (lane == 63 && wave_total > 0) base = atomicAdd(nitems, wave_total);
-  base = __shfl(base, 63, 64);
-  // The wave's runs are consecutive: lane l's items start at base + its
-  // exclusive prefix.  They are written run by run with all 64 lanes, so
-  // every store instruction covers consecutive slots.
-  const int o = base + incl - cnt;
+  __shared__ int s_wave[BLOCK / 64];
+  __shared__ int s_base;
+  const int w = threadIdx.x >> 6;
+  if (lane == 63) s_wave[w] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int sum = 0;
+    for (int i = 0; i < BLOCK / 64; ++i) {
+      const int v = s_wave[i];
+      s_wave[i] = sum;
+      sum += v;
+    }
+    s_base = sum > 0 ? atomicAdd(nitems, sum) : 0;
+  }
+  __syncthreads();
+  // The wave's runs are consecutive: lane l's items start at the wave's base
+  // + its exclusive prefix.  They are written run by run with all 64 lanes,
+  // so every store instruction covers consecutive slots.
+  const int o = s_base + s_wave[w] + incl - cnt;
   unsigned long long runs = __ballot(cnt > 0);
   while (runs) {
     const int src = __builtin_ctzll(runs);
@@ -332,10 +371,10 @@ __device__ __forceinline__ void edge_compact_one(const EdgeArgs& ea, int e, int 
   }
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
     edge_compact_kernel(EdgeArgs ea, int e0, int e1, int lo, int hi, int* __restrict__ item_edge,
                         int* __restrict__ item_pos, int* __restrict__ nitems) {
-  edge_compact_one(ea, e0 + blockIdx.x * blockDim.x + threadIdx.x, e1, lo, hi, item_edge, item_pos, nitems);
+  edge_compact_one<1024>(ea, e0 + blockIdx.x * 1024 + threadIdx.x, e1, lo, hi, item_edge, item_pos, nitems);
 }
 
 // Multi-scene edge calls: one task per scene (prologue) or per scene part of
@@ -356,14 +395,14 @@ __device__ __forceinline__ const EdgeTask& edge_task_of(const EdgeTask* __restri
 
 __global__ void __launch_bounds__(256) edge_prologue_multi_kernel(const EdgeTask* __restrict__ tasks, int ntasks) {
   const EdgeTask& T = edge_task_of<false>(tasks, ntasks);
-  edge_prologue_one(T.rb.ndof, T.dq, T.ea, (blockIdx.x - T.block0) * 256 + threadIdx.x, T.testnum, T.stride,
+  edge_prologue_one<256>(T.rb.ndof, T.dq, T.ea, (blockIdx.x - T.block0) * 256 + threadIdx.x, T.testnum, T.stride,
                     T.max_testnum, T.total);
 }
 
 __global__ void __launch_bounds__(256) edge_compact_multi_kernel(const EdgeTask* __restrict__ tasks, int ntasks) {
   const EdgeTask& T = edge_task_of<false>(tasks, ntasks);
-  edge_compact_one(T.ea, T.e0 + (blockIdx.x - T.block0) * 256 + threadIdx.x, T.e1, T.lo, T.hi, T.item_edge,
-                   T.item_pos, T.nitems);
+  edge_compact_one<256>(T.ea, T.e0 + (blockIdx.x - T.block0) * 256 + threadIdx.x, T.e1, T.lo, T.hi, T.item_edge,
+                        T.item_pos, T.nitems);
 }
 
 template <int NDOF>
@@ -1443,7 +1482,7 @@ hipError_t launch_edge_prologue(const DevRobot& rb, float dq, EdgeArgs ea, int* 
                                 int* stride_out, int* max_testnum, unsigned long long* total,
                                 hipStream_t st) {
   if (ea.nedges <= 0) return hipSuccess;
-  dim3 grid((ea.nedges + 255) / 256), block(256);
+  dim3 grid((ea.nedges + 1023) / 1024), block(1024);
   hipLaunchKernelGGL(edge_prologue_kernel, grid, block, 0, st, rb.ndof, dq, ea, testnum_out, stride_out,
                      max_testnum, total);
   return hipGetLastError();
@@ -1452,7 +1491,7 @@ hipError_t launch_edge_prologue(const DevRobot& rb, float dq, EdgeArgs ea, int* 
 hipError_t launch_edge_compact(EdgeArgs ea, int e0, int e1, int lo, int hi, int* item_edge,
                                int* item_pos, int* nitems, hipStream_t st) {
   if (e1 <= e0) return hipSuccess;
-  dim3 grid((e1 - e0 + 255) / 256), block(256);
+  dim3 grid((e1 - e0 + 1023) / 1024), block(1024);
   hipLaunchKernelGGL(edge_compact_kernel, grid, block, 0, st, ea, e0, e1, lo, hi, item_edge, item_pos, nitems);
   return hipGetLastError();
 }
