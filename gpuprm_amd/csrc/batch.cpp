@@ -185,8 +185,9 @@ int prepare(gprm_batch* b, const Signature& sg) {
   const int S = (int)b->ctx.size();
   const int nd = b->ndof;
   std::vector<long long> base(S + 1, 0);
-  for (int s = 0; s < S; ++s) base[s + 1] = base[s] + sg.n[s];
-  b->total_items = base[S];
+  for (int s = 0; s < S; ++s) base[s + 1] = base[s] + frame_stride(sg.n[s]);
+  b->total_items = 0;
+  for (int s = 0; s < S; ++s) b->total_items += sg.n[s];
   BHIP(b, b->frames.ensure(sizeof(float) * 12 * nd * (size_t)std::max(1LL, base[S])), "hipMalloc frames");
   BHIP(b, b->words.ensure(sizeof(int) * (size_t)std::max(1LL, base[S])), "hipMalloc words");
   BHIP(b, b->ctl.ensure(sizeof(unsigned) * ctl_words(S)), "hipMalloc control words");
@@ -205,6 +206,7 @@ int prepare(gprm_batch* b, const Signature& sg) {
     t.offset = sg.off[s];
     t.n = sg.n[s];
     t.block0 = blocks;
+    t.fstride = frame_stride(sg.n[s]);
     t.frames = b->frames.as<float>() + (size_t)12 * nd * base[s];
     t.word = b->words.as<int>() + base[s];
     t.codes = static_cast<int*>(const_cast<void*>(sg.res[s]));
@@ -224,7 +226,7 @@ int prepare(gprm_batch* b, const Signature& sg) {
       t.a.img = c->img;
       t.a.ndof = nd;
       t.a.frames = fk[s].frames;
-      t.a.fstride = sg.n[s];
+      t.a.fstride = fk[s].fstride;
       t.a.hist = b->hist.as<float4>();
       t.a.nitems = sg.n[s];
       t.a.counter = ctl_counter(b, s);
@@ -426,7 +428,7 @@ int edge_plan(gprm_batch* b, const EdgeSignature& sg, const std::vector<int>& ma
         if (j >= nch[s]) continue;
         const int width = std::min(w.second, maxT[s]) - w.first;
         const int e0 = j * per[s], e1 = std::min(sg.n[s], e0 + per[s]);
-        const long long c = (long long)(e1 - e0) * width;
+        const long long c = frame_stride((long long)(e1 - e0) * width);
         parts.push_back({s, e0, e1, w.first, w.second, (int)c});
         cap[s] = std::max(cap[s], (int)c);
       }

@@ -327,7 +327,7 @@ int run_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n,
   if (n <= 0) return GPRM_OK;
   const int nd = c->scene.ndof;
   const int chunk = std::min(n, item_budget(c));
-  GPRM_HIP(sl.frames.ensure(sizeof(float) * 12 * nd * (size_t)chunk), "hipMalloc frames");
+  GPRM_HIP(sl.frames.ensure(sizeof(float) * 12 * nd * (size_t)frame_stride(chunk)), "hipMalloc frames");
   GPRM_HIP(sl.status.ensure(sizeof(int) * (size_t)chunk), "hipMalloc words");
   const int nchunks = (n + chunk - 1) / chunk;
   GPRM_HIP(sl.hist.ensure(sizeof(float) * 4 * kMaxIter * (size_t)c->shape.grid * c->shape.block), "hipMalloc history");
@@ -336,7 +336,8 @@ int run_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n,
     int c0 = ci * chunk, cn = std::min(chunk, n - c0);
     size_t ev = 0;
     GPRM_HIP(prof_begin(c, sl, 1, st, &ev), "event");
-    GPRM_HIP(launch_fk_points(c->rb, d_q + c0, offset, cn, sl.frames.as<float>(), sl.status.as<int>(), 1,
+    GPRM_HIP(launch_fk_points(c->rb, d_q + c0, offset, cn, sl.frames.as<float>(), frame_stride(cn),
+                              sl.status.as<int>(), 1,
                               sl.counters.as<unsigned>() + ci, st),
              "kinematics launch");
     GPRM_HIP(prof_end(c, sl, ev, st), "event");
@@ -344,7 +345,7 @@ int run_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n,
     a.img = c->img;
     a.ndof = nd;
     a.frames = sl.frames.as<float>();
-    a.fstride = cn;
+    a.fstride = frame_stride(cn);
     a.hist = sl.hist.as<float4>();
     a.nitems = cn;
     a.counter = sl.counters.as<unsigned>() + ci;
@@ -406,7 +407,7 @@ int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long 
   auto enqueue = [&](size_t from) -> int {
     int cap = 0;
     for (size_t i = from; i < plan.size(); ++i)
-      cap = std::max(cap, (plan[i].e1 - plan[i].e0) * (plan[i].hi - plan[i].lo));
+      cap = std::max(cap, (int)frame_stride((long long)(plan[i].e1 - plan[i].e0) * (plan[i].hi - plan[i].lo)));
     const size_t fbytes = sizeof(float) * 12 * nd * (size_t)std::max(cap, 1);
     const size_t ibytes = sizeof(int) * (size_t)std::max(cap, 1);
     // per launch: [0] item count, [1] work counter
@@ -433,7 +434,8 @@ int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long 
       GPRM_HIP(launch_edge_compact(ea, p.e0, p.e1, p.lo, p.hi, sl.item_edge.as<int>(), sl.item_pos.as<int>(), cnt,
                                    st),
                "edge compaction launch");
-      int pcap = (p.e1 - p.e0) * (p.hi - p.lo);
+      // item capacity and frame plane stride
+      const int pcap = (int)frame_stride((long long)(p.e1 - p.e0) * (p.hi - p.lo));
       size_t ev = 0;
       GPRM_HIP(prof_begin(c, sl, 1, st, &ev), "event");
       GPRM_HIP(launch_edge_fk(c->rb, ea, sl.item_edge.as<int>(), sl.item_pos.as<int>(), cnt, pcap,

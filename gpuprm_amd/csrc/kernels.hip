@@ -1287,7 +1287,7 @@ __global__ void __launch_bounds__(kFkBlock) fk_points_multi_kernel(const FkTask*
   for (int j = 0; j < nd; ++j)
     if (qq[j] < T.rb.mins[j] || qq[j] > T.rb.maxs[j]) { st = 1; break; }
   T.word[i] = st ? kWordOutOfBounds : kWordFree;
-  fk_store<NDOF>(T.rb, qq, T.frames, i, (long long)T.n);
+  fk_store<NDOF>(T.rb, qq, T.frames, i, T.fstride);
 }
 
 // codes of every task from its result words (same block map).
@@ -1331,10 +1331,11 @@ int sva_max_lds_bytes() { return 160 * 1024; }
   }
 
 hipError_t launch_fk_points(const DevRobot& rb, const float* q, long long offset, int n, float* frames,
-                            int* word, int check_bounds, unsigned* zero_counter, hipStream_t st) {
+                            long long fstride, int* word, int check_bounds, unsigned* zero_counter,
+                            hipStream_t st) {
   if (n <= 0) return hipSuccess;
   dim3 grid((n + kFkBlock - 1) / kFkBlock), block(kFkBlock);
-  GPRM_FK_DISPATCH(fk_points_kernel, grid, block, st, rb, q, offset, n, frames, (long long)n, word,
+  GPRM_FK_DISPATCH(fk_points_kernel, grid, block, st, rb, q, offset, n, frames, fstride, word,
                    check_bounds, zero_counter);
   return hipGetLastError();
 }

@@ -84,13 +84,20 @@ struct SceneTask {
 };
 
 // One scene of a multi-scene kinematics / finalize launch (device memory).
+// Plane stride of the quad-planar frames for n items: n rounded up to 64,
+// so every plane starts 1 KB-aligned and a wave's float4 store (64 lanes,
+// 1 KB) covers whole 128-byte lines.  With an odd stride the edge benchmark's
+// third kinematics launch ran 20% longer.
+inline long long frame_stride(long long n) { return (n + 63) / 64 * 64; }
+
 struct FkTask {
   DevRobot rb;
   const float* q;               // SoA, component j of item i at q[i + offset*j]
   long long offset;
   int n;
   int block0;                   // first block of this task (tasks sorted by block0)
-  float* frames;                // link-major, item stride n
+  long long fstride;            // frame plane stride, frame_stride(n)
+  float* frames;                // link-major, item stride fstride
   int* word;                    // [n] result words
   int* codes;                   // [n] codes written by the finalize kernel
 };
@@ -134,7 +141,7 @@ struct EdgeTask {
   int* item_edge;
   int* item_pos;
   int* nitems;
-  int cap;
+  int cap;                      // item capacity, a multiple of 64 (also the frame plane stride)
   float* frames;
   int block0;    // first block of the task in the prologue / compaction grid (tasks sorted)
   int fblock0;   // first block of the task in the kinematics grid (tasks sorted)
@@ -157,7 +164,7 @@ int sva_max_lds_bytes();
 // zero_counter (may be null) is set to zero by the launch (the work counter
 // of the separating-vector launch that follows).
 hipError_t launch_fk_points(const DevRobot& rb, const float* q, long long offset, int n, float* frames,
-                            int* word, int check_bounds, unsigned* zero_counter, hipStream_t st);
+                            long long fstride, int* word, int check_bounds, unsigned* zero_counter, hipStream_t st);
 hipError_t launch_finalize_points(const int* word, int* codes, int n, hipStream_t st);
 struct SvaShape;
 hipError_t launch_sva(const SvaArgs& a, int edge_mode, const SvaShape& sh, hipStream_t st);
@@ -175,7 +182,7 @@ hipError_t launch_edge_compact(EdgeArgs ea, int e0, int e1, int lo, int hi, int*
 // Kinematics of the *nitems (<= cap) compacted samples; frames link-major
 // with item stride cap.
 hipError_t launch_edge_fk(const DevRobot& rb, EdgeArgs ea, const int* item_edge, const int* item_pos,
-                          const int* nitems, int cap, float* frames, hipStream_t st);
+                          const int* nitems, int cap, float* frames, hipStream_t st);  // cap: plane stride
 hipError_t launch_fill_i32(int* p, int v, long long n, hipStream_t st);
 CellGrid make_cell_grid(int ndof, const float* mins, const float* maxs, float D);
 // Counting sort of n nodes (SoA, stride noff) into the cell index: counts and
