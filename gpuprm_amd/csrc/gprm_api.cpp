@@ -40,9 +40,12 @@ constexpr int kItemBudget = 1 << 24;
 // gave 6.80/6.70/6.53/6.37-6.42/6.45/6.57/6.72 ms per step; later, (first
 // window, growth) of (16, 2)/(16, 3)/(16, 4)/(16, 5)/(8, 3)/(12, 3)/(24, 3)/
 // (32, 3)/(24, 4) gave 5.76/5.56-5.61/5.61-5.62/5.61/5.77/5.90/5.62/5.69/
-// 5.82 ms.
+// 5.82 ms.  Once a short remainder joined the last window and the frame
+// planes were 1 KB-aligned, (16, 3)/(16, 4)/(16, 5)/(16, 6)/(8, 4)/(12, 4)/
+// (20, 4)/(24, 3)/(32, 3) gave 5.31/5.20/5.27/5.28/5.45/5.26-5.29/5.28/
+// 5.27-5.29/5.29 ms: windows [0,16), [16,80), [80,209) for its longest edges.
 constexpr int kFirstWindow = 16;
-constexpr int kWindowGrowth = 3;
+constexpr int kWindowGrowth = 4;
 
 thread_local std::string g_create_error;
 
