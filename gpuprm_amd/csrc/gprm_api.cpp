@@ -259,16 +259,18 @@ hipError_t prof_end(gprm_ctx* c, Slot& sl, size_t idx, hipStream_t st) {
 unsigned long long* tally_ptr(gprm_ctx* c, Slot& sl) {
   return c->profile ? sl.tally.as<unsigned long long>() : nullptr;
 }
+// The slot's tally counters are cleared once and then accumulate over calls;
+// gprm_profile_read folds them into the context totals, so a profiled call
+// pays no extra copy or synchronisation for them.
 int prof_prepare(gprm_ctx* c, Slot& sl, hipStream_t st) {
   sl.ev_used = 0;
-  sl.tally_pending = false;
-  if (!c->profile) return GPRM_OK;
+  if (!c->profile || sl.tally_pending) return GPRM_OK;
   GPRM_HIP(sl.tally.ensure(sizeof(unsigned long long) * kTallyCount * kTallySlots), "hipMalloc tally");
   GPRM_HIP(launch_fill_u32(sl.tally.as<unsigned>(), 0u, 2 * kTallyCount * kTallySlots, st), "fill");
   sl.tally_pending = true;
   return GPRM_OK;
 }
-// After the slot's stream has completed: fold event times and counters into
+// After the slot's stream has completed: fold the launches' event times into
 // the context totals.
 int prof_harvest(gprm_ctx* c, Slot& sl) {
   for (size_t i = 0; i < sl.ev_used; ++i) {
@@ -278,6 +280,11 @@ int prof_harvest(gprm_ctx* c, Slot& sl) {
     else { c->prof.fk_launches++; c->prof.fk_ms += ms; }
   }
   sl.ev_used = 0;
+  return GPRM_OK;
+}
+// Fold the accumulated tally counters of an idle slot (its stream has
+// completed) into the context totals.
+int prof_fold_tally(gprm_ctx* c, Slot& sl) {
   if (sl.tally_pending) {
     std::vector<unsigned long long> slots((size_t)kTallyCount * kTallySlots);
     GPRM_HIP(hipMemcpy(slots.data(), sl.tally.p, sizeof(unsigned long long) * slots.size(), hipMemcpyDeviceToHost),
@@ -798,12 +805,18 @@ int gprm_profile_enable(gprm_ctx* c, int enable) {
   std::lock_guard<std::mutex> lk(c->mu);
   c->profile = enable != 0;
   c->prof = gprm_profile_stats{};
+  for (auto& s : c->slots) s->tally_pending = false;  // the next profiled call clears them
   return GPRM_OK;
 }
 
 int gprm_profile_read(gprm_ctx* c, gprm_profile_stats* out) {
   if (!c || !out) return GPRM_E_INVALID;
   std::lock_guard<std::mutex> lk(c->mu);
+  for (auto& s : c->slots) {
+    if (s->busy) continue;  // an asynchronous request in flight: folded by a later read
+    int rc = prof_fold_tally(c, *s);
+    if (rc) return rc;
+  }
   *out = c->prof;
   return GPRM_OK;
 }
