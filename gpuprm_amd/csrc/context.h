@@ -17,7 +17,7 @@
 namespace gprm {
 
 // Order-position windows of the edge rounds for a largest testnum maxT:
-// [0, 16), then widths growing threefold (defined in gprm_api.cpp).
+// [0, 16), then widths growing fourfold (defined in gprm_api.cpp).
 std::vector<std::pair<int, int>> edge_windows(int maxT);
 
 struct DevBuf {
@@ -43,7 +43,16 @@ struct Slot {
   hipStream_t side = nullptr;
   hipEvent_t ev_prologue = nullptr;
   void* pinned = nullptr;  // 32 bytes: largest testnum (int), sample total (u64 at +8)
+  // Edge launches after the first compact their items and compute their
+  // frames on the prep stream, into the buffer set of the launch's parity,
+  // while the previous launch's separating-vector kernel runs: ev_first
+  // follows the first launch's kinematics, ev_fk[i] launch i's kinematics,
+  // ev_sva[i] its separating-vector kernel.
+  hipStream_t prep = nullptr;
+  hipEvent_t ev_first = nullptr;
+  std::vector<hipEvent_t> ev_fk, ev_sva;
   DevBuf frames, status, item_edge, item_pos, counters, qs, qe, res, testnum, stride, misc, tally, hist;
+  DevBuf frames_odd, item_edge_odd, item_pos_odd;  // buffer set of the odd edge launches
   int* host_res = nullptr;
   int N = 0;
   bool busy = false;
@@ -56,6 +65,11 @@ struct Slot {
     for (auto e : ev_start) (void)hipEventDestroy(e);
     for (auto e : ev_end) (void)hipEventDestroy(e);
     if (side) (void)hipStreamSynchronize(side);
+    if (prep) (void)hipStreamSynchronize(prep);
+    for (auto e : ev_fk) (void)hipEventDestroy(e);
+    for (auto e : ev_sva) (void)hipEventDestroy(e);
+    if (ev_first) (void)hipEventDestroy(ev_first);
+    if (prep) (void)hipStreamDestroy(prep);
     if (ev_prologue) (void)hipEventDestroy(ev_prologue);
     if (pinned) (void)hipHostFree(pinned);
     if (side) (void)hipStreamDestroy(side);

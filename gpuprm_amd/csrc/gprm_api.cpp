@@ -414,54 +414,102 @@ int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long 
     const int per = std::max(1, item_budget(c) / (hi - lo));
     for (int e0 = 0; e0 < N; e0 += per) plan.push_back({e0, std::min(N, e0 + per), lo, hi});
   };
+  // Launch i uses buffer set i & 1.  Launch 0 runs on the call's stream;
+  // launches i >= 1 compact and compute frames on the prep stream (after
+  // launch 0's kinematics, and once launch i - 2's separating-vector kernel
+  // has released their buffer set), so that this work overlaps launch
+  // i - 1's separating-vector kernel.  Their compaction reads the open edges
+  // while launch i - 1 may still block some: it can schedule samples of
+  // edges that end up blocked, which the separating-vector kernel skips
+  // (its units test the edge's result first) or tests without changing the
+  // decision.
+  auto set_frames = [&](size_t i) -> DevBuf& { return (i & 1) ? sl.frames_odd : sl.frames; };
+  auto set_edge = [&](size_t i) -> DevBuf& { return (i & 1) ? sl.item_edge_odd : sl.item_edge; };
+  auto set_pos = [&](size_t i) -> DevBuf& { return (i & 1) ? sl.item_pos_odd : sl.item_pos; };
   auto enqueue = [&](size_t from) -> int {
-    int cap = 0;
+    int cap[2] = {0, 0};
     for (size_t i = from; i < plan.size(); ++i)
-      cap = std::max(cap, (int)frame_stride((long long)(plan[i].e1 - plan[i].e0) * (plan[i].hi - plan[i].lo)));
-    const size_t fbytes = sizeof(float) * 12 * nd * (size_t)std::max(cap, 1);
-    const size_t ibytes = sizeof(int) * (size_t)std::max(cap, 1);
+      cap[i & 1] = std::max(cap[i & 1],
+                            (int)frame_stride((long long)(plan[i].e1 - plan[i].e0) * (plan[i].hi - plan[i].lo)));
+    size_t fbytes[2], ibytes[2];
+    for (int k = 0; k < 2; ++k) {
+      fbytes[k] = sizeof(float) * 12 * nd * (size_t)std::max(cap[k], 1);
+      ibytes[k] = sizeof(int) * (size_t)std::max(cap[k], 1);
+    }
     // per launch: [0] item count, [1] work counter
     const size_t cbytes = sizeof(int) * 2 * std::max<size_t>(plan.size(), 32);
-    if (from > 0 && (fbytes > sl.frames.bytes || ibytes > sl.item_edge.bytes || cbytes > sl.counters.bytes)) {
+    bool grow = cbytes > sl.counters.bytes;
+    for (int k = 0; k < 2; ++k)
+      grow = grow || (cap[k] > 0 && (fbytes[k] > set_frames(k).bytes || ibytes[k] > set_edge(k).bytes));
+    if (from > 0 && grow) {
       // earlier launches still use the buffers to be replaced; their item
       // counts are kept on the host
       GPRM_HIP(hipStreamSynchronize(st), "sync");
       early.resize(2 * from);
       GPRM_HIP(hipMemcpy(early.data(), sl.counters.p, sizeof(int) * early.size(), hipMemcpyDeviceToHost), "copy");
     }
-    GPRM_HIP(sl.frames.ensure(fbytes), "hipMalloc frames");
-    GPRM_HIP(sl.item_edge.ensure(ibytes), "hipMalloc items");
-    GPRM_HIP(sl.item_pos.ensure(ibytes), "hipMalloc items");
+    for (int k = 0; k < 2; ++k) {
+      if (cap[k] == 0) continue;
+      GPRM_HIP(set_frames(k).ensure(fbytes[k]), "hipMalloc frames");
+      GPRM_HIP(set_edge(k).ensure(ibytes[k]), "hipMalloc items");
+      GPRM_HIP(set_pos(k).ensure(ibytes[k]), "hipMalloc items");
+    }
     GPRM_HIP(sl.counters.ensure(cbytes), "hipMalloc counters");
     GPRM_HIP(sl.hist.ensure(sizeof(float) * 4 * kMaxIter * (size_t)c->shape.grid * c->shape.block),
              "hipMalloc history");
-    if (plan.size() > from)
-      GPRM_HIP(launch_fill_u32(sl.counters.as<unsigned>() + 2 * from, 0u, 2 * (long long)(plan.size() - from), st),
-               "fill");
+    if (plan.size() > 1 && !sl.prep) GPRM_HIP(hipStreamCreateWithFlags(&sl.prep, hipStreamNonBlocking), "hipStreamCreate");
+    while (sl.ev_fk.size() < plan.size()) {
+      hipEvent_t a, b;
+      GPRM_HIP(hipEventCreateWithFlags(&a, hipEventDisableTiming), "hipEventCreate");
+      sl.ev_fk.push_back(a);
+      GPRM_HIP(hipEventCreateWithFlags(&b, hipEventDisableTiming), "hipEventCreate");
+      sl.ev_sva.push_back(b);
+    }
+    if (!sl.ev_first) GPRM_HIP(hipEventCreateWithFlags(&sl.ev_first, hipEventDisableTiming), "hipEventCreate");
+    bool prep_started = false;
     for (size_t i = from; i < plan.size(); ++i) {
       const Launch& p = plan[i];
       int* cnt = sl.counters.as<int>() + 2 * i;
-      GPRM_HIP(launch_edge_compact(ea, p.e0, p.e1, p.lo, p.hi, sl.item_edge.as<int>(), sl.item_pos.as<int>(), cnt,
-                                   st),
+      hipStream_t ps = st;
+      if (i == 0) {
+        GPRM_HIP(launch_fill_u32(sl.counters.as<unsigned>(), 0u, 2, st), "fill");
+      } else {
+        ps = sl.prep;
+        if (!prep_started) {
+          prep_started = true;
+          GPRM_HIP(hipStreamWaitEvent(ps, sl.ev_first, 0), "stream wait");
+          GPRM_HIP(launch_fill_u32(sl.counters.as<unsigned>() + 2 * i, 0u, 2 * (long long)(plan.size() - i), ps),
+                   "fill");
+        }
+        if (i >= 2) GPRM_HIP(hipStreamWaitEvent(ps, sl.ev_sva[i - 2], 0), "stream wait");
+      }
+      int* items_e = set_edge(i).as<int>();
+      int* items_p = set_pos(i).as<int>();
+      float* frames = set_frames(i).as<float>();
+      GPRM_HIP(launch_edge_compact(ea, p.e0, p.e1, p.lo, p.hi, items_e, items_p, cnt, ps),
                "edge compaction launch");
       // item capacity and frame plane stride
       const int pcap = (int)frame_stride((long long)(p.e1 - p.e0) * (p.hi - p.lo));
       size_t ev = 0;
-      GPRM_HIP(prof_begin(c, sl, 1, st, &ev), "event");
-      GPRM_HIP(launch_edge_fk(c->rb, ea, sl.item_edge.as<int>(), sl.item_pos.as<int>(), cnt, pcap,
-                              sl.frames.as<float>(), st),
-               "edge kinematics launch");
-      GPRM_HIP(prof_end(c, sl, ev, st), "event");
+      GPRM_HIP(prof_begin(c, sl, 1, ps, &ev), "event");
+      GPRM_HIP(launch_edge_fk(c->rb, ea, items_e, items_p, cnt, pcap, frames, ps), "edge kinematics launch");
+      GPRM_HIP(prof_end(c, sl, ev, ps), "event");
+      if (i == 0) {
+        GPRM_HIP(hipEventRecord(sl.ev_first, st), "event");
+      } else {
+        GPRM_HIP(hipEventRecord(sl.ev_fk[i], ps), "event");
+        GPRM_HIP(hipStreamWaitEvent(st, sl.ev_fk[i], 0), "stream wait");
+      }
       SvaArgs a{};
       a.img = c->img;
       a.ndof = nd;
-      a.frames = sl.frames.as<float>();
+      a.frames = frames;
       a.fstride = pcap;
       a.hist = sl.hist.as<float4>();
       a.nitems = -1;
       a.nitems_dev = cnt;
       a.counter = reinterpret_cast<unsigned*>(cnt + 1);
-      a.item_edge = sl.item_edge.as<int>();
+      a.item_edge = items_e;
       a.edge_res = d_res;
       a.pairs_per_unit = unit_pairs(c, pcap);
       a.units_per_item = (c->img.M + a.pairs_per_unit - 1) / a.pairs_per_unit;
@@ -471,6 +519,7 @@ int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long 
       GPRM_HIP(launch_sva(a, 1, launch_shape(c, (long long)((pcap + 63) / 64) * a.units_per_item), st),
                "separating-vector launch");
       GPRM_HIP(prof_end(c, sl, ev, st), "event");
+      GPRM_HIP(hipEventRecord(sl.ev_sva[i], st), "event");
     }
     return GPRM_OK;
   };
