@@ -115,7 +115,9 @@ int gprm_indicator_dev(gprm_ctx* ctx, const float* d_q, int* d_res, int N, long 
                        void* stream);
 typedef struct {
   long long samples_total;     /* sum of testnum: the reference's work for the call */
-  long long samples_scheduled; /* samples handed to the kernels after early exit */
+  long long samples_scheduled; /* samples handed to the kernels after early exit (a
+                                  round scheduled while the previous one runs
+                                  may include samples of edges it blocks) */
   int max_testnum;
   int rounds;                  /* compaction rounds launched */
 } gprm_edge_stats;
