@@ -177,6 +177,35 @@ def test_device_variants(workload_dir):
     assert np.array_equal(re.cpu().numpy() != 0, want != 0)
 
 
+def test_edge_rounds_overlapped_repeatable(workload_dir):
+    """Long edges (several hundred samples: four rounds, the later ones
+    compacted while the previous round still blocks edges) mixed with short
+    ones: every repeated call gives the oracle's decisions, and the scheduled
+    samples never exceed the reference's."""
+    import torch
+    d = workload_dir(2)
+    o = OracleScene(d)
+    rng = np.random.default_rng(4242)
+    A = rng.uniform(-np.pi, np.pi, size=(6, 400)).astype(np.float32)
+    B = rng.uniform(-np.pi, np.pi, size=(6, 400)).astype(np.float32)
+    V = rng.uniform(-np.pi, np.pi, size=(6, 300)).astype(np.float32)
+    KS, KE = _knn_edges(V, 10, rng, 800)
+    QS = np.ascontiguousarray(np.concatenate([A, KS], axis=1))
+    QE = np.ascontiguousarray(np.concatenate([B, KE], axis=1))
+    want, tot = o.indicator2(QS, QE, nthreads=8)
+    assert 0 < (want != 0).sum() < QS.shape[1]
+    s = _space(d)
+    qs, qe = torch.from_numpy(QS).cuda(), torch.from_numpy(QE).cuda()
+    for rep in range(4):
+        re = torch.full((QS.shape[1],), 7, dtype=torch.int32, device="cuda")
+        st = s.indicator2_dev(qs, qe, re)
+        assert st.samples_total == tot
+        assert st.rounds >= 4
+        assert 0 < st.samples_scheduled <= tot
+        got = re.cpu().numpy()
+        assert np.array_equal(got != 0, want != 0), (rep, np.flatnonzero((got != 0) != (want != 0))[:10])
+
+
 @pytest.mark.parametrize("block,waves", [(256, 5), (256, 4), (1024, 4)])
 def test_launch_shapes_agree(workload_dir, block, waves):
     """Every kernel variant and work-unit size gives the oracle's codes and
