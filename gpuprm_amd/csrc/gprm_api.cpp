@@ -402,8 +402,6 @@ int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long 
     if (!sl.ev_prologue) GPRM_HIP(hipEventCreateWithFlags(&sl.ev_prologue, hipEventDisableTiming), "hipEventCreate");
     if (!sl.pinned) GPRM_HIP(hipHostMalloc(&sl.pinned, 32, hipHostMallocDefault), "hipHostMalloc");
     GPRM_HIP(hipEventRecord(sl.ev_prologue, st), "event");
-    GPRM_HIP(hipStreamWaitEvent(sl.side, sl.ev_prologue, 0), "stream wait");
-    GPRM_HIP(hipMemcpyAsync(sl.pinned, sl.misc.p, 24, hipMemcpyDeviceToHost, sl.side), "copy max");
   }
   // launches: edges [e0, e1) x order positions [lo, hi), at most item_budget
   // items each
@@ -531,6 +529,10 @@ int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long 
   }
   unsigned long long total = 0;
   if (need_max) {
+    // issued after the first round so that the device gets that round's
+    // launches first
+    GPRM_HIP(hipStreamWaitEvent(sl.side, sl.ev_prologue, 0), "stream wait");
+    GPRM_HIP(hipMemcpyAsync(sl.pinned, sl.misc.p, 24, hipMemcpyDeviceToHost, sl.side), "copy max");
     GPRM_HIP(hipStreamSynchronize(sl.side), "sync");
     std::memcpy(&maxT, sl.pinned, 4);
     std::memcpy(&total, static_cast<char*>(sl.pinned) + 16, 8);
