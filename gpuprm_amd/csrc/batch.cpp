@@ -22,7 +22,7 @@
 // order, largest testnum per scene, read back once), then launch steps, each
 // ONE compaction and ONE kinematics launch over every scene's part of the
 // step and one persistent separating-vector launch per bucket.  Steps walk
-// the order-position windows [0,16), [16,64), ... shared by all scenes; a
+// the order-position windows [0,12), [12,60), ... shared by all scenes; a
 // window whose items exceed a scene's per-step budget takes several steps,
 // each over a range of that scene's edges.  The steps after the prologue
 // are what a hipGraph captures; the graph is kept while the arrays, sizes

@@ -17,7 +17,7 @@
 namespace gprm {
 
 // Order-position windows of the edge rounds for a largest testnum maxT:
-// [0, 16), then widths growing fourfold (defined in gprm_api.cpp).
+// [0, 12), then widths growing fourfold (defined in gprm_api.cpp).
 std::vector<std::pair<int, int>> edge_windows(int maxT);
 
 struct DevBuf {

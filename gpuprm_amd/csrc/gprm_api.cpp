@@ -43,8 +43,12 @@ constexpr int kItemBudget = 1 << 24;
 // 5.82 ms.  Once a short remainder joined the last window and the frame
 // planes were 1 KB-aligned, (16, 3)/(16, 4)/(16, 5)/(16, 6)/(8, 4)/(12, 4)/
 // (20, 4)/(24, 3)/(32, 3) gave 5.31/5.20/5.27/5.28/5.45/5.26-5.29/5.28/
-// 5.27-5.29/5.29 ms: windows [0,16), [16,80), [80,209) for its longest edges.
-constexpr int kFirstWindow = 16;
+// 5.27-5.29/5.29 ms.  Once the later rounds' kinematics overlapped the
+// previous round, (12, 4)/(16, 4)/(10, 4)/(8, 4)/(16, 3)/(12, 3)/(8, 3)/
+// (16, 5)/(24, 4) gave 4.84-4.88/4.91-4.92/4.93-4.94/5.11/4.89/5.09-5.11/
+// 5.09-5.10/4.93-4.95/5.01 ms: windows [0,12), [12,60), [60,209) for its
+// longest edges.
+constexpr int kFirstWindow = 12;
 constexpr int kWindowGrowth = 4;
 
 thread_local std::string g_create_error;
