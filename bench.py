@@ -36,7 +36,7 @@ M_PAIRS = 0
 # HBM bytes per sva_kernel<1> launch of this workload (FETCH_SIZE + WRITE_SIZE,
 # each scaled by the same counter's reading for a known-size copy kernel),
 # from profiles/sva_kernel_edges_wl2_traffic.md
-TRAFFIC_BYTES_PER_LAUNCH = 3.472e9
+TRAFFIC_BYTES_PER_LAUNCH = 3.463e9
 
 
 def log(msg):
