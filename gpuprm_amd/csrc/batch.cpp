@@ -738,14 +738,18 @@ int gprm_batch_autotune(gprm_batch* b, const float* const* d_q, const long long*
         gprm_launch_shape sh = v;
         sh.pairs_per_unit = ppu;
         if (gprm_batch_set_bucket_shape(b, i, &sh) != GPRM_OK) continue;  // does not fit this bucket
-        float ms = INFINITY;
-        for (int r = 0; r <= reps; ++r) {  // the first run warms up
+        // the median of reps timed runs after one warm-up: a single run's
+        // noise picked different shapes from run to run
+        std::vector<float> ts;
+        for (int r = 0; r <= reps; ++r) {
           rc = run(b, sg, false, true, nullptr);
           if (rc != GPRM_OK) return rc;
           float t = 0.0f;
           BHIP(b, hipEventElapsedTime(&t, bk.ev0, bk.ev1), "event time");
-          if (r > 0) ms = std::min(ms, t);
+          if (r > 0) ts.push_back(t);
         }
+        std::sort(ts.begin(), ts.end());
+        const float ms = ts[ts.size() / 2];
         if (ms < best_ms) {
           best_ms = ms;
           best = gprm_launch_shape{bk.shape.block, bk.shape.waves, bk.shape.ppu, bk.shape.grid};

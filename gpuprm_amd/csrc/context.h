@@ -34,6 +34,12 @@ struct DevBuf {
   template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// Layout of a slot's `misc` words written by the edge prologue and read back
+// into `pinned`: the largest testnum (int) and the sample total (u64).
+constexpr size_t kMiscMaxT = 0;
+constexpr size_t kMiscTotal = 16;
+constexpr size_t kMiscReadback = 24;
+
 // Buffers of one in-flight request (the reference keeps one buffer set and
 // one stream per outstanding request, robotconfigspace.cpp:627-650).
 struct Slot {
@@ -42,7 +48,7 @@ struct Slot {
   // (into pinned memory) while the first round runs on the call's stream
   hipStream_t side = nullptr;
   hipEvent_t ev_prologue = nullptr;
-  void* pinned = nullptr;  // 32 bytes: largest testnum (int), sample total (u64 at +8)
+  void* pinned = nullptr;  // 32 bytes: copy of misc[0, kMiscReadback)
   // Edge launches after the first compact their items and compute their
   // frames on the prep stream, into the buffer set of the launch's parity,
   // while the previous launch's separating-vector kernel runs: ev_first
@@ -53,6 +59,7 @@ struct Slot {
   std::vector<hipEvent_t> ev_fk, ev_sva;
   DevBuf frames, status, item_edge, item_pos, counters, qs, qe, res, testnum, stride, misc, tally, hist;
   DevBuf frames_odd, item_edge_odd, item_pos_odd;  // buffer set of the odd edge launches
+  int maxT_hint = 0;  // largest testnum of the slot's earlier edge calls (sizes buffers ahead)
   int* host_res = nullptr;
   int N = 0;
   bool busy = false;
@@ -77,6 +84,15 @@ struct Slot {
   }
 };
 
+}  // namespace gprm
+
+struct gprm_ctx;
+namespace gprm {
+// Pairs per work unit of a separating-vector launch over about n items, and
+// the context's launch shape cut to what a launch of `units` work units can
+// use (gprm_api.cpp).
+int unit_pairs(const gprm_ctx* c, long long n);
+SvaShape launch_shape(const gprm_ctx* c, long long units);
 }  // namespace gprm
 
 struct gprm_ctx {

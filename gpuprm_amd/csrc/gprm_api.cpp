@@ -307,6 +307,8 @@ int prof_fold_tally(gprm_ctx* c, Slot& sl) {
   return GPRM_OK;
 }
 
+}  // namespace
+
 // Pairs per work unit of a launch over about n items: the shape's, or fewer
 // so that there are at least two units (chunk x 64-item group) per resident
 // wave; never below 8.
@@ -327,6 +329,8 @@ SvaShape launch_shape(const gprm_ctx* c, long long units) {
   if (need < sh.grid) sh.grid = (int)need;
   return sh;
 }
+
+namespace {
 
 // Items of one launch: at most kItemBudget, and few enough that the unit
 // count stays below 2^31 even with the smallest units (8 pairs).
@@ -381,15 +385,15 @@ int run_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n,
 // Edge checks on the device.  testnum/stride/res are computed by the
 // prologue kernel; maxT (host-known or read back) sizes the rounds.
 int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long long offset, int N,
-              int* d_res, hipStream_t st, int host_maxT, gprm_edge_stats* stats) {
+              int* d_res, hipStream_t st, int host_maxT, gprm_edge_stats* stats, int maxT_hint = 0) {
   if (stats) std::memset(stats, 0, sizeof(*stats));
   if (N <= 0) return GPRM_OK;
   const int nd = c->scene.ndof;
   GPRM_HIP(sl.testnum.ensure(sizeof(int) * (size_t)N), "hipMalloc testnum");
   GPRM_HIP(sl.stride.ensure(sizeof(int) * (size_t)N), "hipMalloc stride");
   GPRM_HIP(sl.misc.ensure(64), "hipMalloc misc");
-  int* d_max = sl.misc.as<int>();
-  unsigned long long* d_total = reinterpret_cast<unsigned long long*>(sl.misc.as<char>() + 16);
+  int* d_max = reinterpret_cast<int*>(sl.misc.as<char>() + kMiscMaxT);
+  unsigned long long* d_total = reinterpret_cast<unsigned long long*>(sl.misc.as<char>() + kMiscTotal);
   GPRM_HIP(launch_fill_u32(sl.misc.as<unsigned>(), 0u, 16, st), "fill");
   EdgeArgs ea{};
   ea.qs = d_qs; ea.qe = d_qe; ea.offset = offset; ea.nedges = N;
@@ -428,18 +432,26 @@ int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long 
   auto set_frames = [&](size_t i) -> DevBuf& { return (i & 1) ? sl.frames_odd : sl.frames; };
   auto set_edge = [&](size_t i) -> DevBuf& { return (i & 1) ? sl.item_edge_odd : sl.item_edge; };
   auto set_pos = [&](size_t i) -> DevBuf& { return (i & 1) ? sl.item_pos_odd : sl.item_pos; };
+  // Launches expected after the first window when the largest testnum is not
+  // known yet (from the slot's earlier calls or the caller's bound): they only
+  // size the buffer sets, so that the later enqueue does not have to grow
+  // buffers that launches in flight still use.
+  std::vector<Launch> expected;
   auto enqueue = [&](size_t from) -> int {
     int cap[2] = {0, 0};
-    for (size_t i = from; i < plan.size(); ++i)
-      cap[i & 1] = std::max(cap[i & 1],
-                            (int)frame_stride((long long)(plan[i].e1 - plan[i].e0) * (plan[i].hi - plan[i].lo)));
+    auto size_launch = [&](size_t i, const Launch& p) {
+      cap[i & 1] = std::max(cap[i & 1], (int)frame_stride((long long)(p.e1 - p.e0) * (p.hi - p.lo)));
+    };
+    for (size_t i = from; i < plan.size(); ++i) size_launch(i, plan[i]);
+    for (size_t j = 0; j < expected.size(); ++j) size_launch(plan.size() + j, expected[j]);
     size_t fbytes[2], ibytes[2];
     for (int k = 0; k < 2; ++k) {
       fbytes[k] = sizeof(float) * 12 * nd * (size_t)std::max(cap[k], 1);
       ibytes[k] = sizeof(int) * (size_t)std::max(cap[k], 1);
     }
     // per launch: [0] item count, [1] work counter
-    const size_t cbytes = sizeof(int) * 2 * std::max<size_t>(plan.size(), 32);
+    const size_t cbytes = sizeof(int) * 2 * std::max<size_t>(plan.size() + expected.size(), 32);
+    expected.clear();
     bool grow = cbytes > sl.counters.bytes;
     for (int k = 0; k < 2; ++k)
       grow = grow || (cap[k] > 0 && (fbytes[k] > set_frames(k).bytes || ibytes[k] > set_edge(k).bytes));
@@ -527,24 +539,35 @@ int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long 
   };
   int rc = GPRM_OK;
   int maxT = host_maxT;
-  if (need_max || maxT > 0) {
+  if (host_maxT >= 0) {
+    // the whole plan is known: one enqueue sizes and launches it
+    for (const auto& w : edge_windows(host_maxT)) add_window(w.first, w.second);
+  } else {
     add_window(0, kFirstWindow);
-    if ((rc = enqueue(0)) != GPRM_OK) return rc;
+    const int hint = std::max(sl.maxT_hint, maxT_hint);
+    std::swap(plan, expected);
+    for (const auto& w : edge_windows(hint))
+      if (w.first >= kFirstWindow) add_window(w.first, w.second);
+    std::swap(plan, expected);
   }
+  if ((rc = enqueue(0)) != GPRM_OK) return rc;
   unsigned long long total = 0;
   if (need_max) {
     // issued after the first round so that the device gets that round's
     // launches first
     GPRM_HIP(hipStreamWaitEvent(sl.side, sl.ev_prologue, 0), "stream wait");
-    GPRM_HIP(hipMemcpyAsync(sl.pinned, sl.misc.p, 24, hipMemcpyDeviceToHost, sl.side), "copy max");
+    GPRM_HIP(hipMemcpyAsync(sl.pinned, sl.misc.p, kMiscReadback, hipMemcpyDeviceToHost, sl.side), "copy max");
     GPRM_HIP(hipStreamSynchronize(sl.side), "sync");
-    std::memcpy(&maxT, sl.pinned, 4);
-    std::memcpy(&total, static_cast<char*>(sl.pinned) + 16, 8);
+    std::memcpy(&maxT, static_cast<char*>(sl.pinned) + kMiscMaxT, 4);
+    std::memcpy(&total, static_cast<char*>(sl.pinned) + kMiscTotal, 8);
   }
-  const size_t first = plan.size();
-  for (const auto& w : edge_windows(maxT))
-    if (w.first >= kFirstWindow) add_window(w.first, w.second);
-  if ((rc = enqueue(first)) != GPRM_OK) return rc;
+  sl.maxT_hint = std::max(sl.maxT_hint, maxT);
+  if (host_maxT < 0) {
+    const size_t first = plan.size();
+    for (const auto& w : edge_windows(maxT))
+      if (w.first >= kFirstWindow) add_window(w.first, w.second);
+    if ((rc = enqueue(first)) != GPRM_OK) return rc;
+  }
   if (stats) {
     std::vector<int> h(2 * plan.size());
     if (!plan.empty()) {
@@ -921,8 +944,9 @@ int gprm_autotune(gprm_ctx* c, const float* d_q, const float* d_qe, int n, long 
       gprm_launch_shape sh = v;
       sh.pairs_per_unit = ppu;
       if (gprm_set_launch_shape(c, &sh) != GPRM_OK) continue;  // not available for this scene
-      double ms = 1e300;
-      for (int r = 0; r <= reps; ++r) {  // the first run warms up
+      // the median of reps timed runs after one warm-up
+      std::vector<double> ts;
+      for (int r = 0; r <= reps; ++r) {
         const auto t0 = std::chrono::steady_clock::now();
         rc = d_qe ? gprm_indicator2_dev(c, d_q, d_qe, d_res, n, offset, nullptr, nullptr)
                   : gprm_indicator_dev(c, d_q, d_res, n, offset, nullptr);
@@ -931,8 +955,10 @@ int gprm_autotune(gprm_ctx* c, const float* d_q, const float* d_qe, int n, long 
           (void)gprm_set_launch_shape(c, &keep);
           return rc;
         }
-        if (r > 0) ms = std::min(ms, t);
+        if (r > 0) ts.push_back(t);
       }
+      std::sort(ts.begin(), ts.end());
+      const double ms = ts[ts.size() / 2];
       if (ms < best_ms) {
         best_ms = ms;
         (void)gprm_get_launch_shape(c, &found);

@@ -229,8 +229,8 @@ int gprm_set_launch_shape(gprm_ctx* ctx, const gprm_launch_shape* in);
 /* Autotuner: runs the given device workload (point checks of n
  * configurations when d_qe is NULL, else n edges; SoA with offset >= n) with
  * every available variant x pairs_per_unit in {32, 64, 128, 256}, `reps` timed runs
- * each after one warm-up, keeps the fastest shape and returns it with its best
- * wall time in ms. */
+ * each after one warm-up, keeps the shape with the fastest median run and
+ * returns it with that median wall time in ms. */
 int gprm_autotune(gprm_ctx* ctx, const float* d_q, const float* d_qe, int n, long long offset, int reps,
                   gprm_launch_shape* best, float* ms_best);
 
@@ -276,9 +276,9 @@ int gprm_batch_indicator2_dev(gprm_batch* batch, const float* const* d_qs, const
                               const long long* offset, const int* n, int* const* d_res, int use_graph, float* done_ms,
                               long long* samples_total);
 /* Autotuner: for every bucket in use, times its separating-vector launch
- * (HIP events; reps runs after one warm-up, best kept) on the given workload
- * with every available variant x pairs_per_unit in {32, 64, 128, 256} and keeps
- * the fastest shape. */
+ * (HIP events; reps runs after one warm-up, their median kept) on the given
+ * workload with every available variant x pairs_per_unit in {32, 64, 128, 256}
+ * and keeps the shape with the fastest median. */
 int gprm_batch_autotune(gprm_batch* batch, const float* const* d_q, const long long* offset, const int* n,
                         int* const* d_res, int reps);
 const char* gprm_batch_last_error(gprm_batch* batch);  /* NULL: error of the last create */

@@ -61,7 +61,10 @@ class RobotConfigspaceGPU : public Configspace<ndof> {
     check_create(gprm_create(ndof, a, alpha, q, d, types, N, sys, n, m, vertices, dsp, cnt, dest, from, to, M, mins,
                              maxs, dq, nbuf, &ctx_));
   }
-  ~RobotConfigspaceGPU() override { gprm_destroy(ctx_); }
+  // Not marked override: the reference's Configspace<ndof> declares no
+  // virtual destructor (configspace.hpp:11-50), and this class must compile
+  // against that declaration too.  With the declaration below it is virtual.
+  ~RobotConfigspaceGPU() { gprm_destroy(ctx_); }
   RobotConfigspaceGPU(const RobotConfigspaceGPU&) = delete;
   RobotConfigspaceGPU& operator=(const RobotConfigspaceGPU&) = delete;
 

@@ -202,6 +202,62 @@ int main(int argc, char** argv) {
     assert out[2] == "missing %d" % GPRM_E_SCENE
 
 
+# The interface of the reference's Configspace<ndof> (prm/src/lib/prm/
+# configspace.hpp:11-50) restated with the property that matters here: it
+# declares no virtual destructor.
+_BASE_WITHOUT_VIRTUAL_DTOR = r'''
+template <int ndof> class Configspace {
+ public:
+  virtual int init(const int ressource_rank = 0, const int ressource_size = 1) = 0;
+  virtual int indicator(const float* q, int* res, const int N, const int offset) = 0;
+  virtual int indicator(const float* q) = 0;
+  virtual int indicator2(const float* qs, const float* qe, int* res, const int N, const int offset) = 0;
+  virtual int indicator2_async(const float* qs, const float* qe, int* res, const int N, const int offset,
+                               int& request) = 0;
+  virtual int indicator2_async_wait(int request) = 0;
+  virtual int indicator2(const float* qs, const int M, const float* qe, int* res, const int* posqe,
+                         const int* numqe, const int offset) = 0;
+  virtual int check_boundaries(const float* q, int* res, const int N, const int offset) = 0;
+  virtual int check_boundaries(const float* q) = 0;
+  virtual float min(int i) = 0;
+  virtual float max(int i) = 0;
+  virtual float deltaq() = 0;
+  virtual int dim() = 0;
+};
+'''
+
+_USE_DROPIN = r'''
+#define GPRM_HAVE_CONFIGSPACE
+#include "gprm_configspace.hpp"
+int use(Configspace<4>* s) { float q[4] = {0, 0, 0, 0}; return s->dim() + s->check_boundaries(q); }
+template class RobotConfigspaceGPU<4>;
+template class RobotConfigspaceGPU<7>;
+'''
+
+
+@pytest.mark.parametrize("std", ["c++11", "c++17"])
+def test_cpp_dropin_against_base_without_virtual_destructor(tmp_path, std):
+    """With GPRM_HAVE_CONFIGSPACE the drop-in derives from the caller's own
+    Configspace<ndof>, which (like the reference's) has no virtual
+    destructor; every member must still instantiate."""
+    src = tmp_path / "base.cpp"
+    src.write_text(_BASE_WITHOUT_VIRTUAL_DTOR + _USE_DROPIN)
+    subprocess.run(["g++", "-std=" + std, "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                    str(src)], check=True)
+
+
+REFERENCE_CONFIGSPACE = "/root/reference/prm/src/lib/prm/configspace.hpp"
+
+
+@pytest.mark.skipif(not os.path.exists(REFERENCE_CONFIGSPACE), reason="reference sources not present")
+def test_cpp_dropin_against_reference_header(tmp_path):
+    """The same against the reference's own header, where it is present."""
+    src = tmp_path / "ref.cpp"
+    src.write_text('#include "%s"\n' % REFERENCE_CONFIGSPACE + _USE_DROPIN)
+    subprocess.run(["g++", "-std=c++11", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                    str(src)], check=True)
+
+
 def test_tuning_table_file():
     """The committed per-bucket table names only shapes the kernel has and the
     library's bucket ranges."""
