@@ -308,9 +308,14 @@ def _tasks(o, nd, n, seed):
 
 
 def _path_ok(o, path):
+    """The path re-decided by the oracle: every node, and every edge in the
+    orientation the planner checked it (gpuprm_amd.canonical_edges)."""
+    import gpuprm_amd
+
     if path is None:
         return False
-    res, _ = o.indicator2(path[:-1].T.copy(), path[1:].T.copy(), nthreads=min(16, os.cpu_count() or 1))
+    S, E = gpuprm_amd.canonical_edges(path[:-1], path[1:])
+    res, _ = o.indicator2(S.T.copy(), E.T.copy(), nthreads=min(16, os.cpu_count() or 1))
     return bool(np.all(res == 0) and np.all(o.indicator_batch(path.T) == 0))
 
 

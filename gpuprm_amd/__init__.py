@@ -21,7 +21,7 @@ import numpy as np
 __all__ = ["RobotConfigspace", "Surrogate", "GprmError", "EdgeStats", "ProfileStats", "PrmParams", "PrmResult", "lib",
            "generate_scene", "LIB_PATH", "LaunchShape",
            "BucketInfo", "SceneBatch", "radius_neighbours",
-           "read_prm_output"]
+           "read_prm_output", "canonical_edges"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgprm.so")
 
@@ -82,6 +82,7 @@ class PrmParams(C.Structure):
         ("order_edges", C.c_int),
         ("edge_budget", C.c_int),
         ("output_dir", C.c_char_p),
+        ("workers", C.c_int),
     ]
 
 
@@ -125,8 +126,10 @@ class PrmResult(C.Structure):
         ("ms_sampling", C.c_double),
         ("ms_near", C.c_double),
         ("ms_edges", C.c_double),
-        ("ms_host", C.c_double),
+        ("ms_insert", C.c_double),
         ("ms_model", C.c_double),
+        ("ms_wait", C.c_double),
+        ("ms_final", C.c_double),
     ]
 
     def as_dict(self):
@@ -248,6 +251,23 @@ def radius_neighbours(nodes, queries, mins, maxs, D, cap=None):
     q, nn, d = oq[:k].cpu().numpy(), on[:k].cpu().numpy(), od[:k].cpu().numpy()
     o = np.lexsort((nn, q))
     return q[o], nn[o], d[o]
+
+
+def canonical_edges(A, B):
+    """Orient edges as the planner checks them: for rows A[k], B[k] (n, ndof)
+    the lexicographically smaller configuration (at the first differing
+    coordinate) becomes the start.  The samples of an edge depend on its
+    direction at the last bit, so a tree or path edge is re-checked exactly as
+    it was decided only in this orientation.  Returns (starts, ends)."""
+    A = np.asarray(A, np.float32)
+    B = np.asarray(B, np.float32)
+    ne = A != B
+    first = np.where(ne.any(axis=1), ne.argmax(axis=1), 0)
+    k = np.arange(len(A))
+    swap = B[k, first] < A[k, first]
+    S = np.where(swap[:, None], B, A)
+    E = np.where(swap[:, None], A, B)
+    return S, E
 
 
 def read_prm_output(path, ndof):
@@ -631,19 +651,20 @@ class RobotConfigspace:
 
     # -- single-process PRM ----------------------------------------------
     def prm_solve(self, qstart, qgoal, num=2048, D=0.5, max_nodes=1 << 18, max_steps=200, seed=1,
-                  nbuf=1 << 21, surrogate=None, bias_sampling=False, order_edges=False, edge_budget=0,
-                  path_cap=1 << 16, output_dir=None):
+                  nbuf=1 << 20, surrogate=None, bias_sampling=False, order_edges=False, edge_budget=0,
+                  path_cap=1 << 16, output_dir=None, workers=2):
         """Bidirectional PRM from qstart to qgoal (the reference's PRMSolver
         process, prm/src/lib/prm/prmsolver.cpp:1973-2187, on one GPU).
         edge_budget=0 checks every candidate edge, as the reference does; k > 0
         checks at most k per (new node, tree), ranked by the surrogate when
         order_edges is set, else shortest first.  output_dir: write both trees
-        in the reference's output layout (see read_prm_output).  Returns
+        in the reference's output layout (see read_prm_output).  workers: steps
+        in flight (2 as the reference's process 5, or 1).  Returns
         (PrmResult, path (n, ndof) or None)."""
         p = PrmParams(int(num), float(D), int(max_nodes), int(max_steps), int(seed), int(nbuf),
                       C.c_void_p(surrogate.handle if surrogate is not None else None),
                       int(bool(bias_sampling)), int(bool(order_edges)), int(edge_budget),
-                      output_dir.encode() if output_dir else None)
+                      output_dir.encode() if output_dir else None, int(workers))
         r = PrmResult()
         path = np.zeros((path_cap, self.ndof), np.float32)
         _check(lib().gprm_prm_solve(self._ctx, _f32(qstart).ctypes.data_as(_FP), _f32(qgoal).ctypes.data_as(_FP),

@@ -93,6 +93,23 @@ namespace gprm {
 // use (gprm_api.cpp).
 int unit_pairs(const gprm_ctx* c, long long n);
 SvaShape launch_shape(const gprm_ctx* c, long long units);
+// For the planner (gprm_api.cpp): a slot held across calls, and the point
+// and edge checks enqueued on `st` without synchronising.  enqueue_edges
+// plans its rounds from maxT_bound (at least the largest testnum of the
+// edges); skip (may be null) leaves edges unchecked.  edge_sample_total:
+// device address of the last edge call's sum of testnum on that slot.
+int slot_acquire(gprm_ctx* c, int* idx);
+void slot_release(gprm_ctx* c, int idx);
+int enqueue_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n, int* d_res, hipStream_t st,
+                   const unsigned char* skip);
+int enqueue_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long long offset, int N, int* d_res,
+                  hipStream_t st, int maxT_bound, const unsigned char* skip);
+const unsigned long long* edge_sample_total(Slot& sl);
+const int* edge_max_testnum(Slot& sl);  // the same call's largest testnum
+// Grow the slot's buffers (with headroom, after a device synchronisation)
+// for point checks of n_points items and edge checks of N edges planned
+// from maxT_bound, so that the enqueue calls above allocate nothing.
+int reserve_slot(gprm_ctx* c, Slot& sl, int n_points, int N, int maxT_bound);
 }  // namespace gprm
 
 struct gprm_ctx {

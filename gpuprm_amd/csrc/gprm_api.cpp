@@ -341,7 +341,7 @@ int item_budget(const gprm_ctx* c) {
 
 // Point checks of n items on the device (q SoA with offset, res[n]).
 int run_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n, int* d_res,
-               hipStream_t st) {
+               hipStream_t st, const unsigned char* skip = nullptr) {
   if (n <= 0) return GPRM_OK;
   const int nd = c->scene.ndof;
   const int chunk = std::min(n, item_budget(c));
@@ -356,7 +356,7 @@ int run_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n,
     GPRM_HIP(prof_begin(c, sl, 1, st, &ev), "event");
     GPRM_HIP(launch_fk_points(c->rb, d_q + c0, offset, cn, sl.frames.as<float>(), frame_stride(cn),
                               sl.status.as<int>(), 1,
-                              sl.counters.as<unsigned>() + ci, st),
+                              sl.counters.as<unsigned>() + ci, st, skip ? skip + c0 : nullptr),
              "kinematics launch");
     GPRM_HIP(prof_end(c, sl, ev, st), "event");
     SvaArgs a{};
@@ -385,7 +385,8 @@ int run_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n,
 // Edge checks on the device.  testnum/stride/res are computed by the
 // prologue kernel; maxT (host-known or read back) sizes the rounds.
 int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long long offset, int N,
-              int* d_res, hipStream_t st, int host_maxT, gprm_edge_stats* stats, int maxT_hint = 0) {
+              int* d_res, hipStream_t st, int host_maxT, gprm_edge_stats* stats, int maxT_hint = 0,
+              const unsigned char* skip = nullptr) {
   if (stats) std::memset(stats, 0, sizeof(*stats));
   if (N <= 0) return GPRM_OK;
   const int nd = c->scene.ndof;
@@ -397,6 +398,7 @@ int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long 
   GPRM_HIP(launch_fill_u32(sl.misc.as<unsigned>(), 0u, 16, st), "fill");
   EdgeArgs ea{};
   ea.qs = d_qs; ea.qe = d_qe; ea.offset = offset; ea.nedges = N;
+  ea.skip = skip;
   ea.testnum = sl.testnum.as<int>(); ea.stride = sl.stride.as<int>(); ea.edge_res = d_res;
   GPRM_HIP(launch_edge_prologue(c->rb, c->dq, ea, sl.testnum.as<int>(), sl.stride.as<int>(), d_max, d_total, st),
            "edge prologue launch");
@@ -626,6 +628,74 @@ int check_ready(gprm_ctx* c) {
 }
 
 }  // namespace
+
+// Entry points for the planner (prm.cpp), which keeps one slot per worker
+// and enqueues without synchronising.
+int slot_acquire(gprm_ctx* c, int* idx) { return acquire_slot(c, idx); }
+void slot_release(gprm_ctx* c, int idx) { c->slots[idx]->busy = false; }
+int enqueue_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n, int* d_res, hipStream_t st,
+                   const unsigned char* skip) {
+  return run_points(c, sl, d_q, offset, n, d_res, st, skip);
+}
+int enqueue_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long long offset, int N, int* d_res,
+                  hipStream_t st, int maxT_bound, const unsigned char* skip) {
+  return run_edges(c, sl, d_qs, d_qe, offset, N, d_res, st, maxT_bound, nullptr, 0, skip);
+}
+
+// Grow the slot's buffers for point checks of n items and edge checks of N
+// edges whose rounds are planned from maxT_bound (the sizes run_points and
+// run_edges ensure), with headroom.  Work already enqueued may still use the
+// buffers, so the device is synchronised before any of them is replaced.
+int reserve_slot(gprm_ctx* c, Slot& sl, int n_points, int N, int maxT_bound) {
+  const int nd = c->scene.ndof;
+  struct Need { DevBuf* b; size_t bytes; };
+  std::vector<Need> need;
+  const size_t hist = sizeof(float) * 4 * kMaxIter * (size_t)c->shape.grid * c->shape.block;
+  need.push_back({&sl.hist, hist});
+  if (n_points > 0) {
+    const int chunk = std::min(n_points, item_budget(c));
+    need.push_back({&sl.frames, sizeof(float) * 12 * nd * (size_t)frame_stride(chunk)});
+    need.push_back({&sl.status, sizeof(int) * (size_t)chunk});
+    need.push_back({&sl.counters, sizeof(unsigned) * (size_t)((n_points + chunk - 1) / chunk)});
+  }
+  if (N > 0) {
+    // the launches of run_edges's plan: windows split into edge ranges,
+    // buffer set i & 1 for launch i
+    long long cap[2] = {0, 0};
+    size_t launches = 0;
+    for (const auto& w : edge_windows(maxT_bound)) {
+      const int per = std::max(1, item_budget(c) / (w.second - w.first));
+      for (int e0 = 0; e0 < N; e0 += per, ++launches)
+        cap[launches & 1] = std::max(cap[launches & 1],
+                                     frame_stride((long long)(std::min(N, e0 + per) - e0) * (w.second - w.first)));
+    }
+    DevBuf* fr[2] = {&sl.frames, &sl.frames_odd};
+    DevBuf* ie[2] = {&sl.item_edge, &sl.item_edge_odd};
+    DevBuf* ip[2] = {&sl.item_pos, &sl.item_pos_odd};
+    for (int k = 0; k < 2; ++k) {
+      if (cap[k] == 0) continue;
+      need.push_back({fr[k], sizeof(float) * 12 * nd * (size_t)cap[k]});
+      need.push_back({ie[k], sizeof(int) * (size_t)cap[k]});
+      need.push_back({ip[k], sizeof(int) * (size_t)cap[k]});
+    }
+    need.push_back({&sl.counters, sizeof(int) * 2 * std::max<size_t>(launches, 32)});
+    need.push_back({&sl.testnum, sizeof(int) * (size_t)N});
+    need.push_back({&sl.stride, sizeof(int) * (size_t)N});
+    need.push_back({&sl.misc, 64});
+  }
+  bool grow = false;
+  for (const auto& x : need) grow = grow || x.bytes > x.b->bytes;
+  if (!grow) return GPRM_OK;
+  GPRM_HIP(hipDeviceSynchronize(), "sync before growing buffers");
+  for (const auto& x : need)
+    if (x.bytes > x.b->bytes) GPRM_HIP(x.b->ensure(x.bytes + x.bytes / 2), "hipMalloc planner buffers");
+  return GPRM_OK;
+}
+const unsigned long long* edge_sample_total(Slot& sl) {
+  return reinterpret_cast<const unsigned long long*>(sl.misc.as<char>() + kMiscTotal);
+}
+const int* edge_max_testnum(Slot& sl) { return reinterpret_cast<const int*>(sl.misc.as<char>() + kMiscMaxT); }
+
 }  // namespace gprm
 
 using namespace gprm;

@@ -177,7 +177,7 @@ template <int NDOF>
 __global__ void __launch_bounds__(kFkBlock)
     fk_points_kernel(DevRobot rb, const float* __restrict__ q, long long offset, int n,
                      float* __restrict__ frames, long long fstride, int* __restrict__ word, int check_bounds,
-                     unsigned* __restrict__ zero_counter) {
+                     unsigned* __restrict__ zero_counter, const unsigned char* __restrict__ skip) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   // the separating-vector launch that follows claims units from this counter
   if (i == 0 && zero_counter != nullptr) *zero_counter = 0u;
@@ -187,7 +187,9 @@ __global__ void __launch_bounds__(kFkBlock)
 #pragma unroll
   for (int j = 0; j < (NDOF > 0 ? NDOF : kMaxDofDev); ++j)
     if (NDOF > 0 || j < nd) qq[j] = q[i + offset * j];
-  if (check_bounds) {
+  if (skip != nullptr && skip[i]) {
+    word[i] = kWordOutOfBounds;
+  } else if (check_bounds) {
     // check_boundaries(q) (robotconfigspace.cpp:795-802)
     int st = 0;
     for (int j = 0; j < nd; ++j)
@@ -264,7 +266,11 @@ __device__ __forceinline__ void edge_prologue_one(int ndof, float dq, const Edge
                                                   int* __restrict__ max_testnum,
                                                   unsigned long long* __restrict__ total) {
   int T = 0;
-  if (k < ea.nedges) {
+  if (k < ea.nedges && ea.skip != nullptr && ea.skip[k]) {
+    testnum[k] = 0;
+    stride[k] = 1;
+    ea.edge_res[k] = kEdgeSkipped;
+  } else if (k < ea.nedges) {
     float norm = 0.0f;
     for (int i = 0; i < ndof; ++i) {
       float diff = ea.qe[k + ea.offset * i] - ea.qs[k + ea.offset * i];
@@ -584,8 +590,12 @@ __global__ void grid_scatter_kernel(int ndof, const float* __restrict__ nodes, i
   sid[pos] = l;
 }
 
+constexpr int pow3(int d) { return d == 0 ? 1 : 3 * pow3(d - 1); }
+constexpr int kNeighbourCells = pow3(kGridDims);
+static_assert(kNeighbourCells <= 256, "one thread per neighbour cell in a 256-thread block");
+
 // Radius search over the cell index: one block per query; the nodes of the
-// (up to 3^kGridDims = 81) neighbouring cells form one index range that the
+// (up to 3^kGridDims) neighbouring cells form one index range that the
 // block's threads stride through.  Same squared distance (same summation
 // order) and same append as near_kernel.
 __global__ void __launch_bounds__(256)
@@ -593,7 +603,7 @@ __global__ void __launch_bounds__(256)
                      const int* __restrict__ starts, const int* __restrict__ counts, const float* __restrict__ sorted,
                      const int* __restrict__ sid, int n, float D2, int* __restrict__ out_q,
                      int* __restrict__ out_node, float* __restrict__ out_d2, int* __restrict__ count, int cap) {
-  constexpr int NK = 81;  // 3^kGridDims neighbour cells
+  constexpr int NK = kNeighbourCells;
   __shared__ int s_start[NK], s_pre[NK + 1];
   const int j = blockIdx.x;
   if (j >= nq) return;
@@ -602,8 +612,8 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x < NK) {
     // neighbour k: offset digit[d] - 1 in grid dimension d; a dimension
     // without grid (d >= dims) takes only offset 0
-    const int k = threadIdx.x;
-    const int digit[kGridDims] = {k % 3, (k / 3) % 3, (k / 9) % 3, k / 27};
+    int digit[kGridDims];
+    for (int d = 0, rem = threadIdx.x; d < kGridDims; ++d, rem /= 3) digit[d] = rem % 3;
     int c = 0;
     bool ok = true;
     for (int d = kGridDims - 1; d >= 0; --d) {
@@ -634,7 +644,7 @@ __global__ void __launch_bounds__(256)
     if (idx < total) {
       // the neighbour cell holding idx: last k with s_pre[k] <= idx
       int k = 0;
-      for (int step = 64; step > 0; step >>= 1)
+      for (int step = 128; step > 0; step >>= 1)
         if (k + step < NK && s_pre[k + step] <= idx) k += step;
       const int pos = s_start[k] + (idx - s_pre[k]);
       for (int i = 0; i < ndof; ++i) {
@@ -1332,11 +1342,11 @@ int sva_max_lds_bytes() { return 160 * 1024; }
 
 hipError_t launch_fk_points(const DevRobot& rb, const float* q, long long offset, int n, float* frames,
                             long long fstride, int* word, int check_bounds, unsigned* zero_counter,
-                            hipStream_t st) {
+                            hipStream_t st, const unsigned char* skip) {
   if (n <= 0) return hipSuccess;
   dim3 grid((n + kFkBlock - 1) / kFkBlock), block(kFkBlock);
   GPRM_FK_DISPATCH(fk_points_kernel, grid, block, st, rb, q, offset, n, frames, fstride, word,
-                   check_bounds, zero_counter);
+                   check_bounds, zero_counter, skip);
   return hipGetLastError();
 }
 

@@ -1,30 +1,31 @@
 // prm.cpp -- single-process bidirectional PRM on one GPU.
 //
 // The planner of the reference (PRMSolver<ndof>, prm/src/lib/prm/prmsolver.
-// {h,cpp}) restated for one process and one device: its MPI rank split
-// (prmsolver.cpp:1765-1895) becomes one batched step per iteration.  Per
-// step (processing_step_part1a..3a, prmsolver.cpp:1973-2187):
-//   1. sample num/2 new nodes around random nodes of each tree, accepting a
-//      base node m with probability 1/(1+surrnum[m])^3 and drawing uniformly
-//      in the cube of half-width D around it (get_random_nodes, :2584-2624);
-//      candidates are decided by the exact point check in one batch instead
-//      of one host call each;
-//   2. list every stored node of either tree closer than D (L2) to each new
-//      node (find_neighbours / get_near_vertices, :2532-2574, :2701-2748) with
-//      a GPU radius search over a cell index of the first three joint
-//      coordinates, in place of the reference's block index over q[0];
-//   3. check the candidate edges with the exact edge check (indicator2): all
-//      of them, as the reference does, or at most edge_budget per (new node,
-//      tree), ranked by the collision-probability model when one is given and
-//      by length otherwise;
-//   4. insert each new node into every tree it has a free edge to, with all
-//      its free checked edges (calc_conn, part3a); a node joining both trees
-//      connects them, and Dijkstra on each tree (do_dijkstra, :2804-2846)
-//      gives the path.
+// {h,cpp}) restated for one process and one device.  Its process 5
+// (prmsolver.cpp:1765-1895) runs two workers: while one worker's edge checks
+// are in flight, the host inserts the other worker's results and prepares
+// its next step.  Here a worker is a HIP stream with its own buffers, and a
+// step is device work from end to end (planner.hip):
+//   part 1  weights, sampling around weighted base nodes with the exact
+//           point check, selection of the new nodes, radius search grouped
+//           by (new node, tree) (processing_step_part1a, :1973-2040);
+//           its counts are read back (the only read-back of a step) to size
+//   part 2  the edge checks: [edge budget ranking], endpoint gather, exact
+//           edge checks (indicator2_async, :2015; wait, :2071);
+//   part 3  insertion: every new node with a free edge joins that tree with
+//           all its free edges; the first node joining both trees connects
+//           them (calc_conn + part3a, :2076-2187); cell indices rebuilt.
+// The trees live on the device.  Parts 1 and 3, which read and write them,
+// run in the order the host issues them (one event chains them across the
+// workers' streams), so a worker sees every node inserted before its part 1
+// was issued, as in the reference's single-threaded host loop; part 2 of one
+// worker runs beside the other worker's parts 1 and 3.  The host reads the
+// trees back once, at the end, and runs Dijkstra on each tree
+// (do_dijkstra, :2804-2846).
 // The optional model (surrogate.hip) ranks; it never decides.  Sampling bias
 // drops candidates before their exact check, edge ranking chooses which
 // edges are checked; every node and edge that enters a tree has been decided
-// free by the exact checks above.
+// free by the exact checks.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,12 +34,16 @@
 #include <cstring>
 #include <filesystem>
 #include <fstream>
+#include <memory>
+#include <mutex>
 #include <numeric>
 #include <queue>
 #include <string>
 #include <vector>
 
 #include "../../include/gprm.h"
+#include "context.h"
+#include "planner.h"
 #include "runtime.h"
 #include "scene.h"
 #include "surrogate.h"
@@ -53,6 +58,8 @@ double ms_since(Clock::time_point t) {
 
 // Points scored along an edge by the model: interior points at t = s/(S+1).
 constexpr int kScorePoints = 8;
+// Undirected edges a tree may store, per node of its capacity.
+constexpr long long kEdgesPerNode = 16;
 
 struct DevArr {
   void* p = nullptr;
@@ -70,30 +77,17 @@ struct DevArr {
   template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
-// One tree of the bidirectional search (the reference's `graph`,
+// One tree on the host after the search (the reference's `graph`,
 // prmsolver.h:40-58, without the block index).
 struct Tree {
-  int ndof = 0, cap = 0, n = 0;
-  int ndev = 0;                         // nodes [0, ndev) are on the device
+  int ndof = 0, n = 0;
   std::vector<float> q;                 // AoS ndof*n
   std::vector<int> surrnum;             // neighbours per node
-  // adjacency as directed entries in insertion order (both directions of an
-  // edge); csr() groups them by node, keeping each node's order
+  // adjacency as directed entries (both directions of an edge); csr()
+  // groups them by node, keeping each node's order
   std::vector<int> ef, et;
   std::vector<float> ew;
-  DevArr dq;                            // SoA ndof x cap on the device
-  std::vector<float> stage;             // host staging of one upload
-  // cell index of the device nodes for the radius search (rebuilt on upload)
-  DevArr cell_id, counts, starts, cursor, sorted, sid;
 };
-
-int insert(Tree& t, const float* q) {
-  if (t.n >= t.cap) return -1;
-  int pos = t.n++;
-  t.q.insert(t.q.end(), q, q + t.ndof);
-  t.surrnum.push_back(0);
-  return pos;
-}
 
 // Adjacency of a tree in CSR form: node l's entries (neighbour, weight) at
 // [off[l], off[l+1]), in the order they were added.
@@ -115,34 +109,6 @@ Csr csr(const Tree& t) {
     c.w[x] = t.ew[k];
   }
   return c;
-}
-
-// Upload nodes [ndev, n) to the device copy, one copy per component.
-bool flush(Tree& t, hipStream_t st) {
-  const int cnt = t.n - t.ndev;
-  if (cnt <= 0) return true;
-  // the previous upload may still read the staging buffer
-  if (hipStreamSynchronize(st) != hipSuccess) return false;
-  t.stage.resize((size_t)t.ndof * cnt);
-  for (int m = 0; m < cnt; ++m)
-    for (int i = 0; i < t.ndof; ++i) t.stage[(size_t)cnt * i + m] = t.q[(size_t)t.ndof * (t.ndev + m) + i];
-  for (int i = 0; i < t.ndof; ++i)
-    if (hipMemcpyAsync(t.dq.as<float>() + (size_t)t.cap * i + t.ndev, t.stage.data() + (size_t)cnt * i,
-                       sizeof(float) * cnt, hipMemcpyHostToDevice, st) != hipSuccess)
-      return false;
-  t.ndev = t.n;
-  return true;
-}
-
-// Rebuild the tree's cell index over its nodes [0, ndev).
-bool index_nodes(Tree& t, const CellGrid& g, hipStream_t st) {
-  if (!t.counts.ensure(sizeof(int) * (size_t)g.ncells) || !t.starts.ensure(sizeof(int) * (size_t)g.ncells) ||
-      !t.cursor.ensure(sizeof(int) * (size_t)g.ncells) || !t.cell_id.ensure(sizeof(int) * (size_t)t.cap) ||
-      !t.sid.ensure(sizeof(int) * (size_t)t.cap) || !t.sorted.ensure(sizeof(float) * t.ndof * (size_t)t.cap))
-    return false;
-  return launch_grid_build(g, t.ndof, t.dq.as<float>(), t.ndev, t.cap, t.cell_id.as<int>(), t.counts.as<int>(),
-                           t.starts.as<int>(), t.cursor.as<int>(), t.sorted.as<float>(), t.sid.as<int>(),
-                           st) == hipSuccess;
 }
 
 // Dijkstra on one tree (do_dijkstra, prmsolver.cpp:2804-2846).
@@ -225,6 +191,38 @@ bool store_tree(const std::string& dir, const Tree& t, const Csr& g, const std::
   return ok;
 }
 
+// Device state of one tree (PlanTree's arrays).
+struct DevTree {
+  DevArr q, surr, cum, bsum, efrom, eto, ew, cell_id, counts, starts, cursor, cbsum, sorted, sid;
+};
+
+// What part 1 reads back: the step's counts and the planner's totals.
+struct Readback {
+  StepInfo info;
+  PlanState state;
+};
+
+// HIP events of one timed interval on a worker stream, folded into a total
+// once the solve has finished.
+struct Interval {
+  hipEvent_t a, b;
+  double* total;
+};
+
+// One worker of the process-5 scheme: a stream (its slot's), the buffers of
+// one step, and the step's state between its parts.
+struct Worker {
+  int slot = -1;
+  Slot* sl = nullptr;
+  hipStream_t st = nullptr;
+  DevArr cand, codes, pskip, pprob, qnew, info, cnt, off, sc_off, enode, ed2, egroup, eskip, escore, er, qs, qe,
+      free_cnt, pos, spts, sprob;
+  Readback* pin = nullptr;
+  hipEvent_t ev_rb = nullptr;
+  int E = 0, S = 0, batch = 0;
+  hipEvent_t ev_mark[4] = {};  // start of part 1, after selection, after the radius search, start of part 2
+};
+
 }  // namespace
 }  // namespace gprm
 
@@ -235,23 +233,10 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
   if (!ctx || !qstart || !qgoal || !prm || !res) return GPRM_E_INVALID;
   std::memset(res, 0, sizeof(*res));
   const int nd = gprm_dim(ctx);
-  if (prm->num < 2 || !(prm->D > 0.0f) || prm->max_nodes < 2 || prm->max_steps < 1 || prm->nbuf < 1 ||
-      prm->edge_budget < 0)
+  if (prm->num < 2 || prm->num > (1 << 14) || !(prm->D > 0.0f) || prm->max_nodes < 2 || prm->max_steps < 1 ||
+      prm->nbuf < 1 || prm->edge_budget < 0 || prm->workers < 0 || prm->workers > 2)
     return GPRM_E_INVALID;
   const auto t0 = Clock::now();
-  hipStream_t st = nullptr;
-  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return GPRM_E_HIP;
-  struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); } } sg{st};
-  Surrogate* model = static_cast<Surrogate*>(prm->surrogate);
-  const bool bias = model && prm->bias_sampling;
-  const bool rank_by_model = model && prm->order_edges;
-
-  Tree T[2];
-  for (int g = 0; g < 2; ++g) {
-    T[g].ndof = nd;
-    T[g].cap = prm->max_nodes;
-    if (!T[g].dq.ensure(sizeof(float) * nd * (size_t)T[g].cap)) return GPRM_E_HIP;
-  }
   // start and goal must be free (prm/src/main.cpp:135-138)
   int c0 = 0, c1 = 0;
   int rc = gprm_indicator1(ctx, qstart, &c0);
@@ -259,384 +244,393 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
   if (rc != GPRM_OK) return rc;
   res->point_checks += 2;
   if (c0 != 0 || c1 != 0) return GPRM_OK;  // found = 0
-  insert(T[0], qstart);
-  insert(T[1], qgoal);
+
+  std::unique_lock<std::mutex> lock(ctx->mu);
+  // the planner keeps its slots busy across calls; the profiling events of
+  // the check calls are not harvested inside a solve
+  struct ProfileOff {
+    gprm_ctx* c;
+    bool was;
+    ~ProfileOff() { c->profile = was; }
+  } prof_off{ctx, ctx->profile};
+  ctx->profile = false;
+  if (hipSetDevice(ctx->device) != hipSuccess) return GPRM_E_HIP;
+  Surrogate* model = static_cast<Surrogate*>(prm->surrogate);
+  const bool bias = model && prm->bias_sampling;
+  const bool rank_by_model = model && prm->order_edges;
+  const int budget = prm->edge_budget;
+  const int W = prm->workers == 0 ? 2 : prm->workers;
+  const int num = prm->num, cap = prm->max_nodes, nbuf = prm->nbuf;
+  const int groups = 2 * num;
+  const float D = prm->D, D2 = D * D;
+  // every candidate edge is shorter than D (the radius search's own float
+  // sum, which the edge prologue repeats), so testnum <= D/dq + 1
+  const int maxT_bound = (int)(D / ctx->dq * (1.0 + 1e-3)) + 2;
+  const int batch_max = 32 * num;
   std::vector<float> lo(nd), hi(nd);
   for (int i = 0; i < nd; ++i) {
-    lo[i] = gprm_min(ctx, i);
-    hi[i] = gprm_max(ctx, i);
+    lo[i] = ctx->scene.mins[i];
+    hi[i] = ctx->scene.maxs[i];
   }
-  const CellGrid grid = make_cell_grid(nd, lo.data(), hi.data(), prm->D);
-  for (int g = 0; g < 2; ++g)
-    if (!flush(T[g], st) || !index_nodes(T[g], grid, st)) return GPRM_E_HIP;
+  const CellGrid grid = make_cell_grid(nd, lo.data(), hi.data(), D);
+  const int nb = plan_scan_blocks(cap), nbc = plan_scan_blocks(grid.ncells);
+  const long long ecap_ll = std::min<long long>(kEdgesPerNode * cap, (1LL << 31) - 1);
+  const int ecap = (int)ecap_ll;
+
+  // ---- device state -----------------------------------------------------------
+  DevArr d_state;
+  DevTree dt[2];
+  PlanTree T[2];
+  if (!d_state.ensure(sizeof(PlanState))) return GPRM_E_HIP;
+  PlanState* state = d_state.as<PlanState>();
+  for (int g = 0; g < 2; ++g) {
+    DevTree& d = dt[g];
+    if (!d.q.ensure(sizeof(float) * nd * (size_t)cap) || !d.surr.ensure(sizeof(int) * (size_t)cap) ||
+        !d.cum.ensure(sizeof(double) * (size_t)cap) || !d.bsum.ensure(sizeof(double) * (size_t)(nb + 1)) ||
+        !d.efrom.ensure(sizeof(int) * (size_t)ecap) || !d.eto.ensure(sizeof(int) * (size_t)ecap) ||
+        !d.ew.ensure(sizeof(float) * (size_t)ecap) || !d.cell_id.ensure(sizeof(int) * (size_t)cap) ||
+        !d.counts.ensure(sizeof(int) * (size_t)grid.ncells) || !d.starts.ensure(sizeof(int) * (size_t)grid.ncells) ||
+        !d.cursor.ensure(sizeof(int) * (size_t)grid.ncells) || !d.cbsum.ensure(sizeof(int) * (size_t)(nbc + 1)) ||
+        !d.sorted.ensure(sizeof(float) * nd * (size_t)cap) || !d.sid.ensure(sizeof(int) * (size_t)cap))
+      return GPRM_E_HIP;
+    PlanTree& t = T[g];
+    t.q = d.q.as<float>();
+    t.surr = d.surr.as<int>();
+    t.n = &state->n[g];
+    t.cum = d.cum.as<double>();
+    t.bsum = d.bsum.as<double>();
+    t.efrom = d.efrom.as<int>();
+    t.eto = d.eto.as<int>();
+    t.ew = d.ew.as<float>();
+    t.ecount = &state->ecount[g];
+    t.ecap = ecap;
+    t.cell_id = d.cell_id.as<int>();
+    t.counts = d.counts.as<int>();
+    t.starts = d.starts.as<int>();
+    t.cursor = d.cursor.as<int>();
+    t.cbsum = d.cbsum.as<int>();
+    t.sorted = d.sorted.as<float>();
+    t.sid = d.sid.as<int>();
+  }
+
+  // ---- workers ------------------------------------------------------------------
+  std::vector<std::unique_ptr<Worker>> wk;
+  struct Slots {
+    gprm_ctx* c;
+    std::vector<std::unique_ptr<Worker>>* w;
+    ~Slots() {
+      for (auto& x : *w) {
+        if (x->st) (void)hipStreamSynchronize(x->st);
+        if (x->ev_rb) (void)hipEventDestroy(x->ev_rb);
+        for (hipEvent_t e : x->ev_mark)
+          if (e) (void)hipEventDestroy(e);
+        if (x->pin) (void)hipHostFree(x->pin);
+        if (x->slot >= 0) slot_release(c, x->slot);
+      }
+    }
+  } slots_guard{ctx, &wk};
+  for (int w = 0; w < W; ++w) {
+    wk.emplace_back(new Worker());
+    Worker& x = *wk.back();
+    if ((rc = slot_acquire(ctx, &x.slot)) != GPRM_OK) return rc;
+    x.sl = ctx->slots[x.slot].get();
+    x.st = x.sl->stream;
+    if (hipHostMalloc((void**)&x.pin, sizeof(Readback), hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&x.ev_rb, hipEventDisableTiming) != hipSuccess)
+      return GPRM_E_HIP;
+    if (!x.cand.ensure(sizeof(float) * nd * (size_t)batch_max) || !x.codes.ensure(sizeof(int) * (size_t)batch_max) ||
+        !x.qnew.ensure(sizeof(float) * nd * (size_t)num) || !x.info.ensure(sizeof(StepInfo)) ||
+        !x.cnt.ensure(sizeof(int) * (size_t)groups) || !x.off.ensure(sizeof(int) * (size_t)(groups + 1)) ||
+        !x.sc_off.ensure(sizeof(int) * (size_t)(groups + 1)) || !x.enode.ensure(sizeof(int) * (size_t)nbuf) ||
+        !x.ed2.ensure(sizeof(float) * (size_t)nbuf) || !x.egroup.ensure(sizeof(int) * (size_t)nbuf) ||
+        !x.er.ensure(sizeof(int) * (size_t)nbuf) || !x.qs.ensure(sizeof(float) * nd * (size_t)nbuf) ||
+        !x.qe.ensure(sizeof(float) * nd * (size_t)nbuf) || !x.free_cnt.ensure(sizeof(int) * (size_t)groups) ||
+        !x.pos.ensure(sizeof(int) * (size_t)groups) ||
+        (bias && (!x.pskip.ensure((size_t)batch_max) || !x.pprob.ensure(sizeof(float) * (size_t)batch_max))) ||
+        (budget > 0 && (!x.eskip.ensure((size_t)nbuf) || !x.escore.ensure(sizeof(float) * (size_t)nbuf))))
+      return GPRM_E_HIP;
+    if ((rc = reserve_slot(ctx, *x.sl, batch_max, 0, maxT_bound)) != GPRM_OK) return rc;
+  }
+
+  // ---- initial trees: the start and the goal, indexed ---------------------------------
+  {
+    hipStream_t st = wk[0]->st;
+    PlanState init{};
+    init.n[0] = init.n[1] = 1;
+    if (hipMemcpyAsync(state, &init, sizeof(init), hipMemcpyHostToDevice, st) != hipSuccess) return GPRM_E_HIP;
+    const float* roots[2] = {qstart, qgoal};
+    for (int g = 0; g < 2; ++g) {
+      for (int i = 0; i < nd; ++i)
+        if (hipMemcpyAsync(T[g].q + (size_t)cap * i, roots[g] + i, sizeof(float), hipMemcpyHostToDevice, st) !=
+            hipSuccess)
+          return GPRM_E_HIP;
+      if (launch_fill_i32(T[g].surr, 0, 1, st) != hipSuccess) return GPRM_E_HIP;
+    }
+    if (launch_plan_index(nd, T[0], T[1], cap, grid, st) != hipSuccess) return GPRM_E_HIP;
+    // the host copies above read host memory when the stream runs them
+    if (hipStreamSynchronize(st) != hipSuccess) return GPRM_E_HIP;
+  }
+  hipEvent_t ev_chain = nullptr;
+  if (hipEventCreateWithFlags(&ev_chain, hipEventDisableTiming) != hipSuccess) return GPRM_E_HIP;
+  struct EvGuard {
+    hipEvent_t e;
+    ~EvGuard() { if (e) (void)hipEventDestroy(e); }
+  } ev_chain_guard{ev_chain};
+  if (hipEventRecord(ev_chain, wk[0]->st) != hipSuccess) return GPRM_E_HIP;
+
+  // timing: pairs of events per interval, folded at the end
+  std::vector<Interval> timed;
+  struct IvGuard {
+    std::vector<Interval>* v;
+    ~IvGuard() {
+      for (auto& x : *v) {
+        (void)hipEventDestroy(x.a);
+        (void)hipEventDestroy(x.b);
+      }
+    }
+  } iv_guard{&timed};
+  // begin_iv returns the interval's index (-1 on failure), end_iv closes it
+  auto begin_iv = [&](hipStream_t st, double* total) -> int {
+    Interval iv{nullptr, nullptr, total};
+    if (hipEventCreate(&iv.a) != hipSuccess) return -1;
+    if (hipEventCreate(&iv.b) != hipSuccess) {
+      (void)hipEventDestroy(iv.a);
+      return -1;
+    }
+    timed.push_back(iv);
+    return hipEventRecord(iv.a, st) == hipSuccess ? (int)timed.size() - 1 : -1;
+  };
+  auto end_iv = [&](int i, hipStream_t st) -> bool { return i >= 0 && hipEventRecord(timed[i].b, st) == hipSuccess; };
 
   Rng rng(prm->seed * 0x9E3779B97F4A7C15ULL + 12345);
-  const int num = prm->num;
-  const float D = prm->D, D2 = prm->D * prm->D;
-  const int batch = 4 * num;  // candidates drawn per exact point-check batch
-  std::vector<float> keptT;
-  std::vector<int> cand_tree(batch), kept, ccode(batch);
-  std::vector<float> qnew((size_t)nd * num);
-  std::vector<float> prob;
-  std::vector<double> sorted_u;
-  std::vector<int> slot, base(batch);
-  std::vector<size_t> bucket;
-  DevArr d_cand, d_code, d_new, d_nq, d_nn, d_nd2, d_cnt, d_qs, d_qe, d_er, d_mq, d_mp, d_eidx, d_base, d_kidx,
-      d_kept;
-  std::vector<int> heidx;
-  if (!d_cand.ensure(sizeof(float) * nd * batch) || !d_code.ensure(sizeof(int) * batch) ||
-      !d_new.ensure(sizeof(float) * nd * num) || !d_nq.ensure(sizeof(int) * 2 * (size_t)prm->nbuf) ||
-      !d_nn.ensure(sizeof(int) * 2 * (size_t)prm->nbuf) || !d_nd2.ensure(sizeof(float) * 2 * (size_t)prm->nbuf) ||
-      !d_cnt.ensure(8) || !d_base.ensure(sizeof(int) * batch) ||
-      (bias && (!d_mp.ensure(sizeof(float) * batch) || !d_kidx.ensure(sizeof(int) * batch) ||
-                !d_kept.ensure(sizeof(float) * nd * batch))))
-    return GPRM_E_HIP;
-  std::vector<int> hq, hn, her;
-  std::vector<float> hd2, score;
+  const int want0 = num / 2, want1 = num - num / 2;
+  int batch = 4 * num;
+  double ms_wait = 0.0;
+  long long predictions = 0;
+  bool stop = false;
 
-  int conn_left = -1, conn_right = -1;
-  for (int step = 0; step < prm->max_steps && conn_left < 0; ++step) {
-    res->steps = step + 1;
-    // ---- 1. sampling ---------------------------------------------------------
-    auto ts = Clock::now();
-    int have[2] = {0, 0};
-    const int want[2] = {num / 2, num - num / 2};
-    int filled = 0;
-    int guard = 0;
-    // Base nodes: the reference draws a node uniformly and keeps it with
-    // probability 1/(1+surrnum)^3, i.e. it samples nodes with weight
-    // 1/(1+surrnum)^3; drawn here directly from cumulative weights.
-    std::vector<double> cum[2];
-    for (int g = 0; g < 2; ++g) {
-      cum[g].resize(T[g].n);
-      double acc = 0.0;
-      for (int m = 0; m < T[g].n; ++m) {
-        double x = 1.0 + T[g].surrnum[m];
-        acc += 1.0 / (x * x * x);
-        cum[g][m] = acc;
-      }
-    }
-    while (filled < num) {
-      if (++guard > 1000) break;
-      // Candidates alternate between the trees (k & 1; a tree's surplus is
-      // discarded).  A tree's base nodes are drawn as sorted variates of its
-      // weight distribution: sorted uniforms from exponential spacings, then
-      // one merge pass over the cumulative weights (the inverse-CDF draw of
-      // the reference's rule, without a search per draw).  Their candidates
-      // go to a random permutation of the tree's slots, so that the order of
-      // the batch carries no bias.
-      for (int g = 0; g < 2; ++g) {
-        const Tree& t = T[g];
-        const int kg = (batch - g + 1) / 2;  // slots k = g, g + 2, ...
-        sorted_u.resize(kg);
-        double acc = 0.0;
-        for (int j = 0; j < kg; ++j) {
-          acc -= std::log1p(-rng.uniform());
-          sorted_u[j] = acc;
-        }
-        acc -= std::log1p(-rng.uniform());
-        const double scale = cum[g].back() / acc;
-        slot.resize(kg);
-        for (int j = 0; j < kg; ++j) slot[j] = g + 2 * j;
-        for (int j = kg - 1; j > 0; --j) std::swap(slot[j], slot[rng.integer(0, j)]);
-        int m = 0;
-        for (int j = 0; j < kg; ++j) {
-          const double u = sorted_u[j] * scale;
-          while (m < t.n - 1 && cum[g][m] <= u) ++m;
-          const int k = slot[j];
-          cand_tree[k] = g;
-          base[k] = m;
-        }
-      }
-      // the candidates themselves are formed on the device from the trees'
-      // device copies (offsets from a counter-based generator keyed per batch)
-      if (hipMemcpyAsync(d_base.p, base.data(), sizeof(int) * batch, hipMemcpyHostToDevice, st) != hipSuccess ||
-          launch_sample_candidates(nd, batch, d_base.as<int>(), T[0].dq.as<float>(), T[1].dq.as<float>(), T[0].cap,
-                                   D, rng.next(), d_cand.as<float>(), st) != hipSuccess)
-        return GPRM_E_HIP;
-      // Sampling bias: every second pair of candidates must be predicted
-      // free; candidates it drops are never checked.  The others, and all
-      // candidates without a model, go to the exact check.
-      kept.clear();
-      const float* d_chk = d_cand.as<float>();
-      if (bias) {
-        auto tm = Clock::now();
-        prob.resize(batch);
-        if (surrogate_predict(model, d_cand.as<float>(), batch, batch, d_mp.as<float>(), st) != 0 ||
-            hipMemcpyAsync(prob.data(), d_mp.p, sizeof(float) * batch, hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipStreamSynchronize(st) != hipSuccess)
-          return GPRM_E_HIP;
-        res->model_predictions += batch;
-        res->ms_model += ms_since(tm);
-        for (int k = 0; k < batch; ++k)
-          if (!((k & 2) && prob[k] > 0.5f)) kept.push_back(k);
-        const int nk = (int)kept.size();
-        if (nk < batch) {
-          if (hipMemcpyAsync(d_kidx.p, kept.data(), sizeof(int) * nk, hipMemcpyHostToDevice, st) != hipSuccess ||
-              launch_gather_columns(nd, nk, d_kidx.as<int>(), d_cand.as<float>(), batch, d_kept.as<float>(), st) !=
-                  hipSuccess)
-            return GPRM_E_HIP;
-          d_chk = d_kept.as<float>();
-        }
-      } else {
-        kept.resize(batch);
-        std::iota(kept.begin(), kept.end(), 0);
-      }
-      const int nk = (int)kept.size();
-      res->candidates_skipped += batch - nk;
-      if (nk == 0) continue;
-      rc = gprm_indicator_dev(ctx, d_chk, d_code.as<int>(), nk, nk, st);
-      if (rc != GPRM_OK) return rc;
-      keptT.resize((size_t)nd * nk);
-      if (hipMemcpyAsync(ccode.data(), d_code.p, sizeof(int) * nk, hipMemcpyDeviceToHost, st) != hipSuccess ||
-          hipMemcpyAsync(keptT.data(), d_chk, sizeof(float) * nd * nk, hipMemcpyDeviceToHost, st) != hipSuccess ||
-          hipStreamSynchronize(st) != hipSuccess)
-        return GPRM_E_HIP;
-      res->point_checks += nk;
-      for (int a = 0; a < nk && filled < num; ++a) {
-        const int g = cand_tree[kept[a]];
-        if (ccode[a] != 0 || have[g] >= want[g]) continue;
-        for (int i = 0; i < nd; ++i) qnew[(size_t)nd * filled + i] = keptT[(size_t)nk * i + a];
-        ++have[g];
-        ++filled;
-      }
-    }
-    res->ms_sampling += ms_since(ts);
-    if (filled < num) break;
-
-    // ---- 2. radius neighbours in both trees ---------------------------------
-    auto tn = Clock::now();
-    std::vector<float> qnewT((size_t)nd * num);
-    for (int j = 0; j < num; ++j)
-      for (int i = 0; i < nd; ++i) qnewT[(size_t)num * i + j] = qnew[(size_t)nd * j + i];
-    if (hipMemcpyAsync(d_new.p, qnewT.data(), sizeof(float) * nd * num, hipMemcpyHostToDevice, st) != hipSuccess)
+  // part 1: sampling, selection, radius search; reads back the counts
+  auto part1 = [&](Worker& w) -> int {
+    hipStream_t st = w.st;
+    if (hipStreamWaitEvent(st, ev_chain, 0) != hipSuccess) return GPRM_E_HIP;
+    w.batch = batch;
+    const unsigned long long key = rng.next();
+    StepInfo* info = w.info.as<StepInfo>();
+    const int iv_s = begin_iv(st, &res->ms_sampling);
+    if (iv_s < 0) return GPRM_E_HIP;
+    if (launch_plan_weights(T[0], T[1], nb, st) != hipSuccess ||
+        launch_plan_sample(nd, w.batch, T[0], T[1], cap, nb, D, key, w.cand.as<float>(), st) != hipSuccess)
       return GPRM_E_HIP;
-    // candidate edges (new node j, tree g, stored node, distance) in the
-    // reference's buffer order: by new node, left tree before right tree
-    // (find_neighbours), then by node; at most nbuf per tree and per step.
-    // The two trees' searches run back to back into the two halves of the
-    // output arrays and are read back together.
-    struct Cand { int j, g, node; float d2; };
-    std::vector<Cand> cands;
-    const size_t half = (size_t)prm->nbuf;
-    int cnt[2] = {0, 0};
-    {
-      const int zero[2] = {0, 0};
-      if (hipMemcpyAsync(d_cnt.p, zero, 8, hipMemcpyHostToDevice, st) != hipSuccess) return GPRM_E_HIP;
-      for (int g = 0; g < 2; ++g)
-        if (launch_near_grid(nd, d_new.as<float>(), num, num, grid, T[g].starts.as<int>(), T[g].counts.as<int>(),
-                             T[g].sorted.as<float>(), T[g].sid.as<int>(), T[g].ndev, D2, d_nq.as<int>() + g * half,
-                             d_nn.as<int>() + g * half, d_nd2.as<float>() + g * half, d_cnt.as<int>() + g,
-                             prm->nbuf, st) != hipSuccess)
-          return GPRM_E_HIP;
-      if (hipMemcpyAsync(cnt, d_cnt.p, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
-          hipStreamSynchronize(st) != hipSuccess)
+    const unsigned char* pskip = nullptr;
+    if (bias) {
+      // every second pair of candidates must be predicted free; the others,
+      // and all candidates without a model, go to the exact check
+      const int iv_m = begin_iv(st, &res->ms_model);
+      if (iv_m < 0 || surrogate_predict(model, w.cand.as<float>(), w.batch, w.batch, w.pprob.as<float>(), st) != 0 ||
+          !end_iv(iv_m, st) ||
+          launch_plan_bias_mask(w.pprob.as<float>(), w.batch, w.pskip.as<unsigned char>(), st) != hipSuccess)
         return GPRM_E_HIP;
-      hq.resize(2 * half); hn.resize(2 * half); hd2.resize(2 * half);
-      for (int g = 0; g < 2; ++g) {
-        cnt[g] = std::min(cnt[g], prm->nbuf);
-        if (cnt[g] == 0) continue;
-        if (hipMemcpyAsync(hq.data() + g * half, d_nq.as<int>() + g * half, 4 * (size_t)cnt[g],
-                           hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipMemcpyAsync(hn.data() + g * half, d_nn.as<int>() + g * half, 4 * (size_t)cnt[g],
-                           hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipMemcpyAsync(hd2.data() + g * half, d_nd2.as<float>() + g * half, 4 * (size_t)cnt[g],
-                           hipMemcpyDeviceToHost, st) != hipSuccess)
-          return GPRM_E_HIP;
-      }
-      if (hipStreamSynchronize(st) != hipSuccess) return GPRM_E_HIP;
+      predictions += w.batch;
+      pskip = w.pskip.as<unsigned char>();
     }
-    // counting sort by (j, g), then each (small) group by node
-    bucket.assign(2 * (size_t)num + 1, 0);
-    for (int g = 0; g < 2; ++g)
-      for (int i = 0; i < cnt[g]; ++i) ++bucket[2 * (size_t)hq[g * half + i] + g + 1];
-    for (size_t b = 1; b < bucket.size(); ++b) bucket[b] += bucket[b - 1];
-    cands.resize((size_t)cnt[0] + cnt[1]);
-    for (int g = 0; g < 2; ++g)
-      for (int i = 0; i < cnt[g]; ++i) {
-        const size_t x = g * half + i;
-        cands[bucket[2 * (size_t)hq[x] + g]++] = Cand{hq[x], g, hn[x], hd2[x]};
-      }
-    for (size_t a = 0, b = 0; a < cands.size(); a = b) {
-      b = a + 1;
-      while (b < cands.size() && cands[b].j == cands[a].j && cands[b].g == cands[a].g) ++b;
-      std::sort(cands.begin() + a, cands.begin() + b, [](const Cand& x, const Cand& y) { return x.node < y.node; });
-    }
-    if ((long long)cands.size() > prm->nbuf) cands.resize(prm->nbuf);
-    res->ms_near += ms_since(tn);
+    int r = enqueue_points(ctx, *w.sl, w.cand.as<float>(), w.batch, w.batch, w.codes.as<int>(), st, pskip);
+    if (r != GPRM_OK) return r;
+    if (launch_plan_select(nd, w.batch, w.codes.as<int>(), pskip, w.cand.as<float>(), want0, want1, num,
+                           w.qnew.as<float>(), info, state, st) != hipSuccess ||
+        !end_iv(iv_s, st))
+      return GPRM_E_HIP;
+    const int iv_n = begin_iv(st, &res->ms_near);
+    if (iv_n < 0 ||
+        launch_plan_near(false, nd, num, w.qnew.as<float>(), info, T[0], T[1], cap, grid, D2, w.cnt.as<int>(),
+                         nullptr, nbuf, nullptr, nullptr, nullptr, st) != hipSuccess ||
+        launch_plan_scan_groups(groups, w.cnt.as<int>(), nbuf, budget, w.off.as<int>(), w.sc_off.as<int>(), info,
+                                st) != hipSuccess ||
+        launch_plan_near(true, nd, num, w.qnew.as<float>(), info, T[0], T[1], cap, grid, D2, w.cnt.as<int>(),
+                         w.off.as<int>(), nbuf, w.enode.as<int>(), w.ed2.as<float>(), w.egroup.as<int>(), st) !=
+            hipSuccess ||
+        !end_iv(iv_n, st) || hipEventRecord(ev_chain, st) != hipSuccess ||
+        hipMemcpyAsync(&w.pin->info, info, sizeof(StepInfo), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(&w.pin->state, state, sizeof(PlanState), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipEventRecord(w.ev_rb, st) != hipSuccess)
+      return GPRM_E_HIP;
+    const auto tw = Clock::now();
+    if (hipEventSynchronize(w.ev_rb) != hipSuccess) return GPRM_E_HIP;
+    ms_wait += ms_since(tw);
+    const StepInfo& si = w.pin->info;
+    w.E = si.E;
+    w.S = si.S;
+    // the next batch: enough candidates for 1.5 num free ones at the rate
+    // just seen, at least 2 num
+    const double rate = std::max(1e-3, (double)si.accepted / std::max(1, w.batch));
+    int want_batch = (int)std::ceil(1.5 * num / rate);
+    if (si.nnew < num) want_batch = std::max(want_batch, 2 * w.batch);
+    batch = std::min(batch_max, std::max(2 * num, (want_batch + 255) / 256 * 256));
+    if (w.pin->state.conn || w.pin->state.full || w.pin->state.over_bound) stop = true;
+    return GPRM_OK;
+  };
 
-    // ---- 2b. edge budget: rank the candidates of each (new node, tree) -------
-    if (prm->edge_budget > 0) {
-      const int B = prm->edge_budget;
-      // groups of equal (j, g) are contiguous
-      std::vector<size_t> gstart;
-      for (size_t i = 0; i < cands.size(); ++i)
-        if (i == 0 || cands[i].j != cands[i - 1].j || cands[i].g != cands[i - 1].g) gstart.push_back(i);
-      gstart.push_back(cands.size());
-      // the model scores only edges of groups with more than B candidates
-      score.assign(cands.size(), 0.0f);
-      if (rank_by_model) {
-        auto tm = Clock::now();
-        std::vector<size_t> need;
-        for (size_t a = 0; a + 1 < gstart.size(); ++a)
-          if (gstart[a + 1] - gstart[a] > (size_t)B)
-            for (size_t i = gstart[a]; i < gstart[a + 1]; ++i) need.push_back(i);
-        const int nn = (int)need.size();
-        const int np = nn * kScorePoints;
-        // the scored points are formed on the device from the new nodes and
-        // the trees' device copies; only the indices travel
-        heidx.resize(3 * (size_t)nn);
-        for (int a = 0; a < nn; ++a) {
-          const Cand& c = cands[need[a]];
-          heidx[a] = c.j;
-          heidx[nn + a] = c.g;
-          heidx[2 * (size_t)nn + a] = c.node;
-        }
-        prob.resize(np);
-        if (np > 0) {
-          if (!d_eidx.ensure(sizeof(int) * 3 * (size_t)nn) || !d_mq.ensure(sizeof(float) * nd * (size_t)np) ||
-              !d_mp.ensure(sizeof(float) * (size_t)np))
-            return GPRM_E_HIP;
-          if (hipMemcpyAsync(d_eidx.p, heidx.data(), sizeof(int) * 3 * (size_t)nn, hipMemcpyHostToDevice, st) !=
-                  hipSuccess ||
-              launch_score_points(nd, nn, kScorePoints, d_eidx.as<int>(), d_new.as<float>(), num, T[0].dq.as<float>(),
-                                  T[1].dq.as<float>(), T[0].cap, d_mq.as<float>(), st) != hipSuccess ||
-              surrogate_predict(model, d_mq.as<float>(), np, np, d_mp.as<float>(), st) != 0 ||
-              hipMemcpyAsync(prob.data(), d_mp.p, sizeof(float) * (size_t)np, hipMemcpyDeviceToHost, st) !=
-                  hipSuccess ||
-              hipStreamSynchronize(st) != hipSuccess)
+  // part 2: edge checks of the step (beside the other worker's parts)
+  auto part2 = [&](Worker& w) -> int {
+    hipStream_t st = w.st;
+    const int E = w.E;
+    if (launch_fill_i32(w.free_cnt.as<int>(), 0, groups, st) != hipSuccess) return GPRM_E_HIP;
+    if (E == 0) return GPRM_OK;
+    const int iv_e = begin_iv(st, &res->ms_edges);
+    if (iv_e < 0) return GPRM_E_HIP;
+    const unsigned char* eskip = nullptr;
+    if (budget > 0) {
+      const float* prob = nullptr;
+      if (rank_by_model && w.S > 0) {
+        const long long np = (long long)w.S * kScorePoints;
+        if (w.spts.bytes < sizeof(float) * nd * (size_t)np || w.sprob.bytes < sizeof(float) * (size_t)np) {
+          // work in flight may use the old buffers
+          if (hipDeviceSynchronize() != hipSuccess || !w.spts.ensure(sizeof(float) * nd * (size_t)np * 3 / 2) ||
+              !w.sprob.ensure(sizeof(float) * (size_t)np * 3 / 2))
             return GPRM_E_HIP;
         }
-        res->model_predictions += np;
-        for (size_t a = 0; a < need.size(); ++a) {
-          float m = 0.0f;
-          for (int s = 0; s < kScorePoints; ++s) m = std::max(m, prob[a * kScorePoints + s]);
-          score[need[a]] = m;
-        }
-        res->ms_model += ms_since(tm);
+        const int iv_m = begin_iv(st, &res->ms_model);
+        if (iv_m < 0 ||
+            launch_plan_score_points(nd, E, nbuf, budget, kScorePoints, w.cnt.as<int>(), w.off.as<int>(),
+                                     w.sc_off.as<int>(), w.egroup.as<int>(), w.enode.as<int>(), w.qnew.as<float>(), num,
+                                     T[0], T[1], cap, w.spts.as<float>(), np, st) != hipSuccess ||
+            surrogate_predict(model, w.spts.as<float>(), (int)np, np, w.sprob.as<float>(), st) != 0 ||
+            !end_iv(iv_m, st))
+          return GPRM_E_HIP;
+        predictions += np;
+        prob = w.sprob.as<float>();
       }
-      std::vector<Cand> keep;
-      std::vector<size_t> idx;
-      for (size_t a = 0; a + 1 < gstart.size(); ++a) {
-        const size_t b0 = gstart[a], b1 = gstart[a + 1];
-        idx.resize(b1 - b0);
-        std::iota(idx.begin(), idx.end(), b0);
-        if (b1 - b0 > (size_t)B) {
-          std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) {
-            return score[x] != score[y] ? score[x] < score[y] : cands[x].d2 < cands[y].d2;
-          });
-          idx.resize(B);
-          std::sort(idx.begin(), idx.end());
-        }
-        for (size_t i : idx) keep.push_back(cands[i]);
-      }
-      res->edges_skipped += (long long)(cands.size() - keep.size());
-      cands.swap(keep);
+      if (launch_plan_rank(E, nbuf, budget, kScorePoints, w.cnt.as<int>(), w.off.as<int>(), w.sc_off.as<int>(),
+                           w.egroup.as<int>(), w.ed2.as<float>(), prob, w.escore.as<float>(),
+                           w.eskip.as<unsigned char>(), state, st) != hipSuccess)
+        return GPRM_E_HIP;
+      eskip = w.eskip.as<unsigned char>();
     }
-    std::vector<int> eq(cands.size()), eg(cands.size()), en(cands.size());
-    std::vector<float> ed(cands.size());
-    for (size_t i = 0; i < cands.size(); ++i) {
-      eq[i] = cands[i].j;
-      eg[i] = cands[i].g;
-      en[i] = cands[i].node;
-      ed[i] = std::sqrt(cands[i].d2);
-    }
+    int r = reserve_slot(ctx, *w.sl, w.batch, E, maxT_bound);
+    if (r != GPRM_OK) return r;
+    if (launch_plan_gather(nd, E, w.egroup.as<int>(), w.enode.as<int>(), w.qnew.as<float>(), num, T[0], T[1], cap,
+                           w.qs.as<float>(), w.qe.as<float>(), st) != hipSuccess)
+      return GPRM_E_HIP;
+    r = enqueue_edges(ctx, *w.sl, w.qs.as<float>(), w.qe.as<float>(), E, E, w.er.as<int>(), st, maxT_bound, eskip);
+    if (r != GPRM_OK) return r;
+    if (launch_plan_edges_done(E, w.egroup.as<int>(), w.er.as<int>(), eskip, w.free_cnt.as<int>(),
+                               edge_sample_total(*w.sl), edge_max_testnum(*w.sl), maxT_bound, state, st) !=
+            hipSuccess ||
+        !end_iv(iv_e, st))
+      return GPRM_E_HIP;
+    return GPRM_OK;
+  };
 
-    // ---- 3. exact edge checks ------------------------------------------------
-    auto te = Clock::now();
-    const int E = (int)eq.size();
-    her.assign(E, 1);
-    if (E > 0) {
-      if (!d_qs.ensure(sizeof(float) * nd * (size_t)E) || !d_qe.ensure(sizeof(float) * nd * (size_t)E) ||
-          !d_er.ensure(sizeof(int) * (size_t)E) || !d_eidx.ensure(sizeof(int) * 3 * (size_t)E))
-        return GPRM_E_HIP;
-      // endpoints gathered on the device from the new nodes (d_new) and the
-      // trees' device copies; only the indices travel
-      heidx.resize(3 * (size_t)E);
-      std::copy(eq.begin(), eq.end(), heidx.begin());
-      std::copy(eg.begin(), eg.end(), heidx.begin() + E);
-      std::copy(en.begin(), en.end(), heidx.begin() + 2 * (size_t)E);
-      if (hipMemcpyAsync(d_eidx.p, heidx.data(), sizeof(int) * 3 * (size_t)E, hipMemcpyHostToDevice, st) !=
-              hipSuccess ||
-          launch_gather_edges(nd, E, d_eidx.as<int>(), d_new.as<float>(), num, T[0].dq.as<float>(),
-                              T[1].dq.as<float>(), T[0].cap, d_qs.as<float>(), d_qe.as<float>(), st) != hipSuccess)
-        return GPRM_E_HIP;
-      gprm_edge_stats es;
-      rc = gprm_indicator2_dev(ctx, d_qs.as<float>(), d_qe.as<float>(), d_er.as<int>(), E, E, st, &es);
-      if (rc != GPRM_OK) return rc;
-      if (hipMemcpyAsync(her.data(), d_er.p, sizeof(int) * E, hipMemcpyDeviceToHost, st) != hipSuccess ||
-          hipStreamSynchronize(st) != hipSuccess)
-        return GPRM_E_HIP;
-      res->edges_checked += E;
-      res->edge_samples += es.samples_total;
-    }
-    res->ms_edges += ms_since(te);
+  // part 3: insertion and the cell indices (in issue order with the parts 1)
+  auto part3 = [&](Worker& w) -> int {
+    hipStream_t st = w.st;
+    if (hipStreamWaitEvent(st, ev_chain, 0) != hipSuccess) return GPRM_E_HIP;
+    const int iv_i = begin_iv(st, &res->ms_insert);
+    if (iv_i < 0 ||
+        launch_plan_insert(nd, num, w.E, w.info.as<StepInfo>(), w.free_cnt.as<int>(), w.pos.as<int>(),
+                           w.qnew.as<float>(), w.egroup.as<int>(), w.enode.as<int>(), w.ed2.as<float>(),
+                           w.er.as<int>(), T[0], T[1], cap, state, st) != hipSuccess ||
+        launch_plan_index(nd, T[0], T[1], cap, grid, st) != hipSuccess || !end_iv(iv_i, st) ||
+        hipEventRecord(ev_chain, st) != hipSuccess)
+      return GPRM_E_HIP;
+    ++res->steps;
+    return GPRM_OK;
+  };
 
-    // ---- 4. insert nodes with free edges; connection ------------------------
-    auto th = Clock::now();
-    size_t e = 0;
-    for (int j = 0; j < num && conn_left < 0; ++j) {
-      int pos[2] = {-1, -1};
-      while (e < (size_t)E && eq[e] < j) ++e;
-      const size_t e0 = e;
-      size_t e1 = e0;
-      while (e1 < (size_t)E && eq[e1] == j) ++e1;
-      for (int g = 0; g < 2; ++g) {
-        bool any = false;
-        for (size_t k = e0; k < e1; ++k)
-          if (eg[k] == g && her[k] == 0) { any = true; break; }
-        if (!any) continue;
-        pos[g] = insert(T[g], &qnew[(size_t)nd * j]);
-        if (pos[g] < 0) break;
-        for (size_t k = e0; k < e1; ++k) {
-          if (eg[k] != g || her[k] != 0) continue;
-          const int other = en[k];
-          Tree& tg = T[g];
-          tg.ef.push_back(pos[g]);
-          tg.et.push_back(other);
-          tg.ew.push_back(ed[k]);
-          tg.ef.push_back(other);
-          tg.et.push_back(pos[g]);
-          tg.ew.push_back(ed[k]);
-          ++T[g].surrnum[other];
-          ++T[g].surrnum[pos[g]];
-          ++res->edges_free;
-        }
-      }
-      e = e1;
-      if (pos[0] >= 0 && pos[1] >= 0) {
-        conn_left = pos[0];
-        conn_right = pos[1];
-      }
-      if ((pos[0] < 0 && T[0].n >= T[0].cap) || (pos[1] < 0 && T[1].n >= T[1].cap)) break;
-    }
-    for (int g = 0; g < 2; ++g)
-      if (!flush(T[g], st) || !index_nodes(T[g], grid, st)) return GPRM_E_HIP;
-    res->ms_host += ms_since(th);
-    if (T[0].n >= T[0].cap || T[1].n >= T[1].cap) break;
+  // the process-5 schedule: part 1 and 2 of every worker, then per worker in
+  // turn part 3 (its insertion), part 1 and part 2 of its next step
+  for (int w = 0; w < W && !stop; ++w) {
+    if ((rc = part1(*wk[w])) != GPRM_OK) return rc;
+    if (!stop && (rc = part2(*wk[w])) != GPRM_OK) return rc;
   }
-  res->nodes_left = T[0].n;
-  res->nodes_right = T[1].n;
+  while (!stop && res->steps < prm->max_steps) {
+    for (int w = 0; w < W && !stop && res->steps < prm->max_steps; ++w) {
+      if ((rc = part3(*wk[w])) != GPRM_OK) return rc;
+      if (res->steps >= prm->max_steps) break;
+      if ((rc = part1(*wk[w])) != GPRM_OK) return rc;
+      if (!stop && (rc = part2(*wk[w])) != GPRM_OK) return rc;
+    }
+  }
+
+  // ---- the trees back on the host ---------------------------------------------------
+  const auto tf = Clock::now();
+  if (hipDeviceSynchronize() != hipSuccess) return GPRM_E_HIP;
+  for (auto& iv : timed) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, iv.a, iv.b) != hipSuccess) return GPRM_E_HIP;
+    *iv.total += ms;
+  }
+  PlanState fin{};
+  if (hipMemcpy(&fin, state, sizeof(fin), hipMemcpyDeviceToHost) != hipSuccess) return GPRM_E_HIP;
+  if (fin.over_bound) {
+    // some samples of an edge were never scheduled: its decision is not
+    // exact, so the trees cannot be trusted
+    ctx->err = "planner: an edge had more samples than the radius allows";
+    return GPRM_E_STATE;
+  }
+  res->point_checks += (long long)fin.point_checks;
+  res->candidates_skipped = (long long)fin.dropped;
+  res->edges_checked = (long long)fin.edges_checked;
+  res->edges_free = (long long)fin.edges_free;
+  res->edge_samples = (long long)fin.edge_samples;
+  res->edges_skipped = (long long)fin.edges_skipped;
+  res->model_predictions = predictions;
+  res->nodes_left = fin.n[0];
+  res->nodes_right = fin.n[1];
+  res->ms_wait = ms_wait;
+  Tree H[2];
+  for (int g = 0; g < 2; ++g) {
+    Tree& t = H[g];
+    t.ndof = nd;
+    t.n = fin.n[g];
+    std::vector<float> soa((size_t)nd * t.n);
+    for (int i = 0; i < nd; ++i)
+      if (hipMemcpy(soa.data() + (size_t)t.n * i, T[g].q + (size_t)cap * i, sizeof(float) * t.n,
+                    hipMemcpyDeviceToHost) != hipSuccess)
+        return GPRM_E_HIP;
+    t.q.resize((size_t)nd * t.n);
+    for (int m = 0; m < t.n; ++m)
+      for (int i = 0; i < nd; ++i) t.q[(size_t)nd * m + i] = soa[(size_t)t.n * i + m];
+    t.surrnum.resize(t.n);
+    const int ne = (int)std::min<unsigned>(fin.ecount[g], (unsigned)ecap);
+    std::vector<int> ef(ne), et(ne);
+    std::vector<float> ew(ne);
+    if (hipMemcpy(t.surrnum.data(), T[g].surr, sizeof(int) * t.n, hipMemcpyDeviceToHost) != hipSuccess ||
+        (ne > 0 && (hipMemcpy(ef.data(), T[g].efrom, sizeof(int) * ne, hipMemcpyDeviceToHost) != hipSuccess ||
+                    hipMemcpy(et.data(), T[g].eto, sizeof(int) * ne, hipMemcpyDeviceToHost) != hipSuccess ||
+                    hipMemcpy(ew.data(), T[g].ew, sizeof(float) * ne, hipMemcpyDeviceToHost) != hipSuccess)))
+      return GPRM_E_HIP;
+    t.ef.resize(2 * (size_t)ne);
+    t.et.resize(2 * (size_t)ne);
+    t.ew.resize(2 * (size_t)ne);
+    for (int k = 0; k < ne; ++k) {
+      t.ef[2 * k] = ef[k];
+      t.et[2 * k] = et[k];
+      t.ef[2 * k + 1] = et[k];
+      t.et[2 * k + 1] = ef[k];
+      t.ew[2 * k] = t.ew[2 * k + 1] = ew[k];
+    }
+  }
+  const int conn_left = fin.conn ? fin.conn_left : -1, conn_right = fin.conn ? fin.conn_right : -1;
   std::vector<int> pl, pr;
   bool connected = false;
   Csr G[2];
   if (conn_left >= 0 || prm->output_dir)
-    for (int g = 0; g < 2; ++g) G[g] = csr(T[g]);
+    for (int g = 0; g < 2; ++g) G[g] = csr(H[g]);
   if (conn_left >= 0) {
-    connected = dijkstra(T[0], G[0], 0, conn_left, &pl) && dijkstra(T[1], G[1], conn_right, 0, &pr);
+    connected = dijkstra(H[0], G[0], 0, conn_left, &pl) && dijkstra(H[1], G[1], conn_right, 0, &pr);
     if (connected) {
       // pl = [conn_left, ..., start], pr = [goal, ..., conn_right]; conn_left
       // and conn_right hold the same configuration
       std::vector<const float*> seq;
-      for (int i = (int)pl.size() - 1; i >= 0; --i) seq.push_back(&T[0].q[(size_t)nd * pl[i]]);
-      for (int i = (int)pr.size() - 2; i >= 0; --i) seq.push_back(&T[1].q[(size_t)nd * pr[i]]);
+      for (int i = (int)pl.size() - 1; i >= 0; --i) seq.push_back(&H[0].q[(size_t)nd * pl[i]]);
+      for (int i = (int)pr.size() - 2; i >= 0; --i) seq.push_back(&H[1].q[(size_t)nd * pr[i]]);
       res->found = 1;
       res->path_len = (int)seq.size();
       if (path)
         for (int i = 0; i < std::min(path_cap, res->path_len); ++i) std::memcpy(path + (size_t)nd * i, seq[i], 4 * nd);
     }
   }
-  if (hipStreamSynchronize(st) != hipSuccess) return GPRM_E_HIP;
+  res->ms_final = ms_since(tf);
   res->ms_total = ms_since(t0);
   if (prm->output_dir) {
     // store_results (prmsolver.cpp:2947-2973): graphl from the start root to
@@ -645,8 +639,8 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
     const int conn = connected ? 1 : 0;
     std::error_code ec;
     std::filesystem::create_directories(root, ec);
-    if (ec || !store_tree(root + "/graphl", T[0], G[0], connected ? &pl : nullptr, 0, std::max(conn_left, 0)) ||
-        !store_tree(root + "/graphr", T[1], G[1], connected ? &pr : nullptr, std::max(conn_right, 0), 0) ||
+    if (ec || !store_tree(root + "/graphl", H[0], G[0], connected ? &pl : nullptr, 0, std::max(conn_left, 0)) ||
+        !store_tree(root + "/graphr", H[1], G[1], connected ? &pr : nullptr, std::max(conn_right, 0), 0) ||
         !write_bin(root + "/connection.bin", &conn, 1))
       return GPRM_E_INVALID;
   }

@@ -108,8 +108,11 @@ struct FkTask {
 
 // Cell index of the radius search: nodes binned by their first `dims`
 // coordinates (cells at least D wide, at most 64 per dimension; up to
-// kGridDims dimensions while the cells number at most kGridMaxCells).
-constexpr int kGridDims = 4;
+// kGridDims dimensions while the cells number at most kGridMaxCells).  A
+// query visits the 3^dims cells around its own.  On the 7-DOF planner task
+// (D = 0.5: 12 cells per dimension) five binned coordinates leave about a
+// quarter of the candidates of four.
+constexpr int kGridDims = 5;
 constexpr int kGridMaxCells = 1 << 20;
 struct CellGrid {
   int dims;
@@ -118,10 +121,17 @@ struct CellGrid {
   int ncells;
 };
 
+// Result of an edge left unchecked by EdgeArgs::skip (nonzero: never taken
+// as free).
+constexpr int kEdgeSkipped = 0x40000000;
+
 // Edge work description.
 struct EdgeArgs {
   const float* qs; const float* qe; long long offset;  // SoA, component i of edge k at [k + offset*i]
   int nedges;
+  // optional [edge]: nonzero = leave the edge unchecked (testnum 0, result
+  // kEdgeSkipped; the planner's edge budget)
+  const unsigned char* skip;
   const int* testnum;      // [edge]
   const int* stride;       // [edge] sample order: position j -> (j*stride) mod testnum
   int* edge_res;           // [edge]
@@ -166,9 +176,12 @@ int sva_max_lds_bytes();
 
 // Kinematics of n configurations; frames link-major with item stride n.
 // zero_counter (may be null) is set to zero by the launch (the work counter
-// of the separating-vector launch that follows).
+// of the separating-vector launch that follows).  skip (may be null): items
+// with skip[i] != 0 get the out-of-bounds word, so the separating-vector
+// kernel tests none of their pairs (code 1).
 hipError_t launch_fk_points(const DevRobot& rb, const float* q, long long offset, int n, float* frames,
-                            long long fstride, int* word, int check_bounds, unsigned* zero_counter, hipStream_t st);
+                            long long fstride, int* word, int check_bounds, unsigned* zero_counter, hipStream_t st,
+                            const unsigned char* skip = nullptr);
 hipError_t launch_finalize_points(const int* word, int* codes, int n, hipStream_t st);
 struct SvaShape;
 hipError_t launch_sva(const SvaArgs& a, int edge_mode, const SvaShape& sh, hipStream_t st);

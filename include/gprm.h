@@ -147,12 +147,15 @@ int gprm_profile_read(gprm_ctx* ctx, gprm_profile_stats* out);
 
 /* Single-process bidirectional PRM on the device (the reference's
  * PRMSolver<ndof>, prm/src/lib/prm/prmsolver.{h,cpp}, process 5 restated for
- * one GPU: prmsolver.cpp:1973-2187).  Grows a tree from qstart and one from
+ * one GPU: prmsolver.cpp:1765-2187).  Grows a tree from qstart and one from
  * qgoal; every node and edge entering a tree is decided by the exact checks
- * above.  On success res->found = 1 and path (AoS, ndof floats per node,
- * up to path_cap nodes) runs from qstart to qgoal. */
+ * above.  Steps run on the device; `workers` steps are in flight at once
+ * (process 5 keeps two: one worker's edge checks run while the other
+ * worker's nodes are inserted and its next step is sampled), each read back
+ * once.  On success res->found = 1 and path (AoS, ndof floats per node, up
+ * to path_cap nodes) runs from qstart to qgoal. */
 typedef struct {
-  int num;                  /* new nodes per step (the reference's --num) */
+  int num;                  /* new nodes per step (the reference's --num), at most 16384 */
   float D;                  /* connection radius and sampling half-width (--distD) */
   int max_nodes;            /* capacity of each tree */
   int max_steps;            /* steps before giving up (--maxsteps) */
@@ -169,6 +172,8 @@ typedef struct {
   const char* output_dir;   /* NULL, or a directory that receives both trees and the
                                path in the reference's output layout (store_results,
                                prmsolver.cpp:2947-3055; prm/matlab/lib/prmoutputread.m) */
+  int workers;              /* steps in flight: 1, or 2 as the reference's process 5
+                               (prmsolver.cpp:1830-1861); 0 = 2 */
 } gprm_prm_params;
 typedef struct {
   int found, steps, path_len;
@@ -179,8 +184,14 @@ typedef struct {
   long long model_predictions;
   long long candidates_skipped;  /* sampling candidates dropped unchecked by the bias */
   long long edges_skipped;       /* candidate edges left unchecked by edge_budget */
-  double ms_total, ms_sampling, ms_near, ms_edges, ms_host;
-  double ms_model;               /* model inference inside the steps */
+  double ms_total;               /* wall time of the call */
+  /* device time of the steps' parts (HIP events, summed over steps; the
+   * workers' parts overlap, so the sum may exceed ms_total): sampling with
+   * the point checks, radius search, edge checks, insertion */
+  double ms_sampling, ms_near, ms_edges, ms_insert;
+  double ms_model;               /* model inference inside the steps (device time) */
+  double ms_wait;                /* host time blocked on the steps' read-backs */
+  double ms_final;               /* trees read back, Dijkstra, path */
 } gprm_prm_result;
 int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* qgoal, const gprm_prm_params* params,
                    gprm_prm_result* res, float* path, int path_cap);

@@ -18,11 +18,14 @@ def _space(d):
 
 
 def _verify(o, path, qs, qg, D):
+    import gpuprm_amd
     assert np.array_equal(path[0], qs) and np.array_equal(path[-1], qg)
     assert np.all(o.indicator_batch(path.T) == 0)
     steps = np.linalg.norm(np.diff(path.astype(np.float64), axis=0), axis=1)
     assert np.all(steps < D * (1 + 1e-6))
-    res, _ = o.indicator2(path[:-1].T.copy(), path[1:].T.copy(), nthreads=8)
+    # each edge in the orientation the planner checked it
+    S, E = gpuprm_amd.canonical_edges(path[:-1], path[1:])
+    res, _ = o.indicator2(S.T.copy(), E.T.copy(), nthreads=8)
     assert np.all(res == 0), np.flatnonzero(res)
 
 
@@ -79,3 +82,60 @@ def test_prm_7dof_workload(workload_dir):
     print(r.as_dict())
     assert r.found == 1
     _verify(o, path, free[0], free[1], 0.5)
+
+
+@pytest.mark.parametrize("workers", [1, 2])
+def test_prm_every_tree_node_and_edge_verified(golden_dir, tmp_path, workers):
+    """Every node and every edge of both trees (not only the path) is
+    re-decided by the oracle; the trees are connected and consistent
+    (surrnum = degree, edges shorter than D); the returned path is a shortest
+    path of the written graph (scipy Dijkstra in float64 over the edges'
+    lengths)."""
+    import gpuprm_amd
+    csgraph = pytest.importorskip("scipy.sparse.csgraph")
+    sparse = pytest.importorskip("scipy.sparse")
+    z, d = golden_dir("config1")
+    s = _space(d)
+    o = OracleScene(d)
+    qs = np.zeros(4, np.float32)
+    qg = np.full(4, 3.0, np.float32)
+    out = str(tmp_path / "trees")
+    D = 1.0
+    print("planner: config1, workers %d" % workers, flush=True)
+    r, path = s.prm_solve(qs, qg, num=512, D=D, max_nodes=1 << 16, max_steps=300, seed=11, output_dir=out,
+                          workers=workers)
+    print(r.as_dict())
+    assert r.found == 1
+    g = gpuprm_amd.read_prm_output(out, 4)
+    lengths = {}
+    for side, root in (("l", 0), ("r", 0)):
+        Q, E = g["Q" + side], g["E" + side]
+        n = len(Q)
+        # nodes: all free by the exact checker of record
+        assert np.all(o.indicator_batch(Q.T.copy(), nthreads=8) == 0)
+        # edges: each once (from < to), shorter than D, free by the oracle's
+        # host loop of indicator2 in the orientation the planner checked them
+        assert len(E) > 0
+        L = np.linalg.norm(Q[E[:, 0]].astype(np.float64) - Q[E[:, 1]], axis=1)
+        assert np.all(L < D * (1 + 1e-6))
+        A, B = gpuprm_amd.canonical_edges(Q[E[:, 0]], Q[E[:, 1]])
+        res, _ = o.indicator2(A.T.copy(), B.T.copy(), nthreads=8)
+        bad = np.flatnonzero(res != 0)
+        assert bad.size == 0, (side, bad[:8])
+        surr = np.fromfile(out + "/graph%s/surrnum.bin" % side, np.int32)
+        deg = np.bincount(E.ravel(), minlength=n)
+        assert np.array_equal(surr, deg)
+        # connected from the root, and the path is a shortest path
+        A = sparse.coo_matrix((L, (E[:, 0], E[:, 1])), shape=(n, n)).tocsr()
+        dist = csgraph.dijkstra(A, directed=False, indices=root)
+        assert np.all(np.isfinite(dist))
+        lengths[side] = dist
+    for side, conn, pth in (("l", g["conl"], g["pathl"]), ("r", g["conr"], g["pathr"])):
+        Q = g["Q" + side]
+        plen = np.linalg.norm(np.diff(Q[pth].astype(np.float64), axis=0), axis=1).sum()
+        # float32 sums in the planner's Dijkstra: equal up to their rounding
+        assert plen == pytest.approx(lengths[side][conn], rel=1e-4, abs=1e-5), (side, plen, lengths[side][conn])
+    _verify(o, path, qs, qg, D)
+    assert r.nodes_left == len(g["Ql"]) and r.nodes_right == len(g["Qr"])
+    # free edges of the last step's nodes after the connection are not inserted
+    assert r.edges_free >= len(g["El"]) + len(g["Er"])
