@@ -184,6 +184,7 @@ def lib():
     L.gprm_numthreads_all.argtypes = [vp]
     L.gprm_numthreads_all.restype = C.c_longlong
     L.gprm_indicator_dev.argtypes = [vp, vp, vp, C.c_int, C.c_longlong, vp]
+    L.gprm_indicator_dev_noalloc.argtypes = [vp, vp, vp, C.c_int, C.c_longlong, vp]
     L.gprm_indicator2_dev.argtypes = [vp, vp, vp, vp, C.c_int, C.c_longlong, vp, C.POINTER(EdgeStats)]
     L.gprm_last_error.argtypes = [vp]
     L.gprm_last_error.restype = C.c_char_p

@@ -34,7 +34,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
 ]
-SOURCES = ["kernels.hip", "gprm_api.cpp", "batch.cpp", "scene.cpp", "prm.cpp", "surrogate.hip", "planner.hip"]
+SOURCES = ["kernels.hip", "gprm_api.cpp", "batch.cpp", "scene.cpp", "prm.cpp", "surrogate.hip", "noalloc.hip", "planner.hip"]
 
 
 def _run(cmd, cwd=None):

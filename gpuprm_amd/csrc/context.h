@@ -110,6 +110,9 @@ const int* edge_max_testnum(Slot& sl);  // the same call's largest testnum
 // for point checks of n_points items and edge checks of N edges planned
 // from maxT_bound, so that the enqueue calls above allocate nothing.
 int reserve_slot(gprm_ctx* c, Slot& sl, int n_points, int N, int maxT_bound);
+// Device address of the scratch array of gprm_indicator_dev_noalloc on
+// `device` (the current device), looked up once per device (noalloc.hip).
+int noalloc_setup(int device);
 }  // namespace gprm
 
 struct gprm_ctx {

@@ -756,6 +756,8 @@ int gprm_init(gprm_ctx* c, int rank, int size) {
   // robotconfigspace.cpp:229: device = rank*devcount/size
   c->device = (int)((long long)rank * count / size);
   GPRM_HIP(hipSetDevice(c->device), "hipSetDevice");
+  if (noalloc_setup(c->device) != GPRM_OK)
+    return fail(c, GPRM_E_HIP, "hipGetSymbolAddress of the no-allocation scratch array failed");
   std::vector<char> image;
   int rc = build_image(c, &image);
   if (rc != GPRM_OK) return rc;

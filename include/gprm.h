@@ -113,6 +113,21 @@ long long gprm_numthreads_all(gprm_ctx* ctx);
  * callers that keep configurations in HBM (the planner's sample buffers). */
 int gprm_indicator_dev(gprm_ctx* ctx, const float* d_q, int* d_res, int N, long long offset,
                        void* stream);
+/* gprm_indicator_dev without any allocation (no reference counterpart; for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same results as
+ * gprm_indicator_dev.  Every buffer beyond d_q and d_res is a fixed scratch
+ * array of 74.3 MiB that the library holds on each device it runs on until
+ * the process ends.  N may be at most 65536 configurations for arms of up to
+ * 8 joints (24 MiB of frames / (48 bytes x joints), rounded down to a
+ * multiple of 64, beyond: 32768 at 16 joints); a larger N returns
+ * GPRM_E_CAPACITY.  The call
+ * only enqueues kernels on `stream` (NULL = the null stream) and returns;
+ * d_res is complete when that stream reaches the end of the call.  Calls
+ * share the device's scratch array: calls whose work overlaps on one GPU
+ * race (serialise them, e.g. on one stream). */
+int gprm_indicator_dev_noalloc(gprm_ctx* ctx, const float* d_q, int* d_res, int N, long long offset,
+                               void* stream);
 typedef struct {
   long long samples_total;     /* sum of testnum: the reference's work for the call */
   long long samples_scheduled; /* samples handed to the kernels after early exit (a
