@@ -75,22 +75,41 @@ hipError_t launch_plan_sample(int ndof, int batch, PlanTree t0, PlanTree t1, int
 hipError_t launch_plan_bias_mask(const float* prob, int batch, unsigned char* skip, hipStream_t st);
 // New nodes: in slot order, the free (code 0), unskipped candidates, at most
 // want0 of tree 0 and want1 of tree 1, into qnew (SoA, stride num);
-// info->nnew, info->accepted; state point_checks / dropped.
+// info->nnew, info->accepted (info->E_all reset); state point_checks /
+// dropped.  tile: 3 ints per 1024 candidates.
 hipError_t launch_plan_select(int ndof, int batch, const int* codes, const unsigned char* skip, const float* cand,
-                              int want0, int want1, int num, float* qnew, StepInfo* info, PlanState* state,
+                              int want0, int want1, int num, float* qnew, int* tile, StepInfo* info, PlanState* state,
                               hipStream_t st);
 // Radius search of the step's new nodes in both trees, one group b = 2j + g
-// per (new node j, tree g): COUNT writes cnt[b]; EMIT writes the group's
-// edges (node, squared distance, group) at off[b] + r for r < cnt[b], up to
-// nbuf entries in all.  Same float sum as the reference's get_near_vertices.
-hipError_t launch_plan_near(bool emit, int ndof, int num, const float* qnew, const StepInfo* info, PlanTree t0,
-                            PlanTree t1, int cap, const CellGrid& g, float D2, int* cnt, const int* off, int nbuf,
-                            int* enode, float* ed2, int* egroup, hipStream_t st);
-// off = exclusive scan of cnt over the 2 num groups (off[2 num] = all);
-// info->E_all, info->E = min(all, nbuf); with budget B > 0 the groups with
-// more than B stored edges are scored: sc_off their exclusive scan, info->S.
+// per (new node j, tree g): every (node, squared distance, group) with
+// distance^2 < D2 (the reference's get_near_vertices float sum) takes a slot
+// from info->E_all; the first nbuf slots are stored, in no fixed order;
+// cnt[b] = the group's stored edges.
+hipError_t launch_plan_near(int ndof, int num, const float* qnew, StepInfo* info, PlanTree t0, PlanTree t1, int cap,
+                            const CellGrid& g, float D2, int* cnt, int nbuf, int* enode, float* ed2, int* egroup,
+                            hipStream_t st);
+// info->E = min(E_all, nbuf).  With budget B > 0 also the group order: off =
+// exclusive scan of cnt (groups + 1 entries), sc_off the same over the groups
+// with more than B edges (scored by the model), info->S their total; fill
+// zeroed for the scatter.
 hipError_t launch_plan_scan_groups(int groups, const int* cnt, int nbuf, int budget, int* off, int* sc_off,
-                                   StepInfo* info, hipStream_t st);
+                                   int* fill, StepInfo* info, hipStream_t st);
+// The radius search's edges (gs, ns, ds) moved into group order (egroup,
+// enode, ed2), group b at off[b].
+hipError_t launch_plan_group_scatter(int nbuf, const StepInfo* info, const int* gs, const int* ns, const float* ds,
+                                     const int* off, int* fill, int* egroup, int* enode, float* ed2, hipStream_t st);
+// Shortest paths from node 0 in both trees (dist, parent: 2 x cap ints):
+// init, then relaxation rounds over the edges (max_edges >= both trees'
+// edge counts) that set changed[tree] when they lower a distance, then
+// parents, the chains from the connection nodes back to the roots (chain:
+// 2 x maxlen, len[2], -1 when broken) and their coordinates (coords AoS,
+// 2 x maxlen x ndof).
+hipError_t launch_plan_paths_init(PlanTree t0, PlanTree t1, int cap, int* dist, int* parent, hipStream_t st);
+hipError_t launch_plan_paths_relax(PlanTree t0, PlanTree t1, int cap, int max_edges, int* dist, int* changed,
+                                   hipStream_t st);
+hipError_t launch_plan_paths_finish(int ndof, PlanTree t0, PlanTree t1, int cap, int max_edges, const int* dist,
+                                    int* parent, const PlanState* state, int maxlen, int* chain, int* len, float* coords,
+                                    hipStream_t st);
 // Points scored along the edges of groups above the budget: edge e of such a
 // group b, position p = sc_off[b] + e - off[b], points p*S8 + s at
 // qa + t (qb - qa), t = (s+1)/(S8+1); out SoA with stride ns.

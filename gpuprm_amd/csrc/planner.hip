@@ -162,48 +162,73 @@ __global__ void __launch_bounds__(256) plan_bias_mask_kernel(const float* __rest
   if (k < batch) skip[k] = ((k & 2) && prob[k] > 0.5f) ? 1 : 0;
 }
 
-// One block: slot order, at most want[g] free candidates of tree g.
+// New nodes in slot order: at most want[g] free, unskipped candidates of
+// tree g (slot k is of tree k & 1).  Tiles of 1024 slots, three passes:
+// per-tile counts, their prefix (one block), then every taken candidate
+// writes itself at j = min(r0, want0) + min(r1, want1), r_g the accepted
+// candidates of tree g before it.
+__device__ __forceinline__ bool accepted_at(const int* codes, const unsigned char* skip, int k) {
+  return !(skip && skip[k]) && codes[k] == 0;
+}
+
 __global__ void __launch_bounds__(kScanBlock)
-    plan_select_kernel(int ndof, int batch, const int* __restrict__ codes, const unsigned char* __restrict__ skip,
-                       const float* __restrict__ cand, int want0, int want1, int num, float* __restrict__ qnew,
-                       StepInfo* info, PlanState* state) {
+    plan_select_count(int batch, const int* __restrict__ codes, const unsigned char* __restrict__ skip,
+                      int* __restrict__ tile) {
   __shared__ int lds[kMaxWaves + 1];
-  const int per = (batch + kScanBlock - 1) / kScanBlock;
-  const int k0 = threadIdx.x * per, k1 = min(batch, k0 + per);
-  int a0 = 0, a1 = 0, dr = 0;
-  for (int k = k0; k < k1; ++k) {
-    const bool sk = skip && skip[k];
-    dr += sk;
-    if (!sk && codes[k] == 0) ((k & 1) ? a1 : a0) += 1;
-  }
+  const int k = blockIdx.x * kScanBlock + threadIdx.x;
+  const bool in = k < batch;
+  const bool acc = in && accepted_at(codes, skip, k);
   int A0, A1, DR;
-  const int r0s = block_excl_scan(a0, &A0, lds);
-  const int r1s = block_excl_scan(a1, &A1, lds);
-  (void)block_excl_scan(dr, &DR, lds);
-  // kept candidates of this thread, then their order among all kept
-  int r0 = r0s, r1 = r1s, kept = 0;
-  for (int k = k0; k < k1; ++k) {
-    if ((skip && skip[k]) || codes[k] != 0) continue;
-    if (k & 1) kept += r1++ < want1;
-    else kept += r0++ < want0;
-  }
-  int nnew;
-  int j = block_excl_scan(kept, &nnew, lds);
-  r0 = r0s;
-  r1 = r1s;
-  for (int k = k0; k < k1; ++k) {
-    if ((skip && skip[k]) || codes[k] != 0) continue;
-    const bool take = (k & 1) ? r1++ < want1 : r0++ < want0;
-    if (!take) continue;
-    for (int i = 0; i < ndof; ++i) qnew[j + (size_t)num * i] = cand[k + (size_t)batch * i];
-    ++j;
-  }
+  (void)block_excl_scan((int)(acc && !(k & 1)), &A0, lds);
+  (void)block_excl_scan((int)(acc && (k & 1)), &A1, lds);
+  (void)block_excl_scan((int)(in && skip && skip[k]), &DR, lds);
   if (threadIdx.x == 0) {
-    info->nnew = nnew;
-    info->accepted = A0 + A1;
-    atomicAdd(&state->point_checks, (unsigned long long)(batch - DR));
-    atomicAdd(&state->dropped, (unsigned long long)DR);
+    tile[3 * blockIdx.x] = A0;
+    tile[3 * blockIdx.x + 1] = A1;
+    tile[3 * blockIdx.x + 2] = DR;
   }
+}
+
+// Exclusive prefix of the tiles' counts in place; totals into info/state.
+__global__ void __launch_bounds__(kScanBlock)
+    plan_select_prefix(int batch, int ntiles, int want0, int want1, int* __restrict__ tile, StepInfo* info,
+                       PlanState* state) {
+  __shared__ int lds[kMaxWaves + 1];
+  const int per = (ntiles + kScanBlock - 1) / kScanBlock;
+  const int t0 = threadIdx.x * per, t1 = min(ntiles, t0 + per);
+  int s[3] = {0, 0, 0}, tot[3];
+  for (int t = t0; t < t1; ++t)
+    for (int c = 0; c < 3; ++c) s[c] += tile[3 * t + c];
+  int run[3];
+  for (int c = 0; c < 3; ++c) run[c] = block_excl_scan(s[c], &tot[c], lds);
+  for (int t = t0; t < t1; ++t)
+    for (int c = 0; c < 3; ++c) {
+      const int x = tile[3 * t + c];
+      tile[3 * t + c] = run[c];
+      run[c] += x;
+    }
+  if (threadIdx.x == 0) {
+    info->nnew = min(tot[0], want0) + min(tot[1], want1);
+    info->accepted = tot[0] + tot[1];
+    info->E_all = 0;  // the radius search's edge counter
+    atomicAdd(&state->point_checks, (unsigned long long)(batch - tot[2]));
+    atomicAdd(&state->dropped, (unsigned long long)tot[2]);
+  }
+}
+
+__global__ void __launch_bounds__(kScanBlock)
+    plan_select_write(int ndof, int batch, const int* __restrict__ codes, const unsigned char* __restrict__ skip,
+                      const float* __restrict__ cand, int want0, int want1, int num, const int* __restrict__ tile,
+                      float* __restrict__ qnew) {
+  __shared__ int lds[kMaxWaves + 1];
+  const int k = blockIdx.x * kScanBlock + threadIdx.x;
+  const bool acc = k < batch && accepted_at(codes, skip, k);
+  int T0, T1;
+  const int r0 = tile[3 * blockIdx.x] + block_excl_scan((int)(acc && !(k & 1)), &T0, lds);
+  const int r1 = tile[3 * blockIdx.x + 1] + block_excl_scan((int)(acc && (k & 1)), &T1, lds);
+  if (!acc || ((k & 1) ? r1 >= want1 : r0 >= want0)) return;
+  const int j = min(r0, want0) + min(r1, want1);
+  for (int i = 0; i < ndof; ++i) qnew[j + (size_t)num * i] = cand[k + (size_t)batch * i];
 }
 
 // ---- radius search -----------------------------------------------------------
@@ -216,23 +241,28 @@ constexpr int pow3(int d) { return d == 0 ? 1 : 3 * pow3(d - 1); }
 constexpr int kNeighbourCells = pow3(kGridDims);
 static_assert(kNeighbourCells <= 256, "one thread per neighbour cell in a 256-thread block");
 
-template <bool EMIT>
+// One block per group b = 2 j + g: the nonempty cells among the 3^dims
+// around new node j are compacted (block scans), the block strides through
+// their nodes, and matches take slots from the step's edge counter
+// (info->E_all, one atomic per wave); slots beyond nbuf are dropped.  cnt[b]
+// gets the group's stored edges.
 __global__ void __launch_bounds__(256)
-    plan_near_kernel(int ndof, int num, const float* __restrict__ qnew, const StepInfo* __restrict__ info,
-                     PlanTree t0, PlanTree t1, int cap, CellGrid g, float D2, int* __restrict__ cnt,
-                     const int* __restrict__ off, int nbuf, int* __restrict__ enode, float* __restrict__ ed2,
-                     int* __restrict__ egroup) {
+    plan_near_kernel(int ndof, int num, const float* __restrict__ qnew, StepInfo* info, PlanTree t0, PlanTree t1,
+                     int cap, CellGrid g, float D2, int* __restrict__ cnt, int nbuf, int* __restrict__ enode,
+                     float* __restrict__ ed2, int* __restrict__ egroup) {
   constexpr int NK = kNeighbourCells;
   __shared__ int s_start[NK], s_pre[NK + 1];
-  __shared__ int s_wave[4];
+  __shared__ int lds[kMaxWaves + 1];
+  __shared__ int s_stored;
   const int b = blockIdx.x, j = b >> 1;
   if (j >= info->nnew) {
-    if (!EMIT && threadIdx.x == 0) cnt[b] = 0;
+    if (threadIdx.x == 0) cnt[b] = 0;
     return;
   }
   const PlanTree& T = tree_of(t0, t1, b & 1);
   float qr[kMaxDofDev];
   for (int i = 0; i < ndof; ++i) qr[i] = qnew[j + (size_t)num * i];
+  int cstart = 0, ccount = 0;
   if (threadIdx.x < NK) {
     // neighbour k: offset digit(d) - 1 in grid dimension d; a dimension
     // without grid (d >= dims) takes only offset 0
@@ -252,29 +282,34 @@ __global__ void __launch_bounds__(256)
       if (x < 0 || x >= g.cells[d]) ok = false;
       c = c * g.cells[d] + x;
     }
-    s_start[threadIdx.x] = ok ? T.starts[c] : 0;
-    s_pre[threadIdx.x + 1] = ok ? T.counts[c] : 0;
+    if (ok) {
+      cstart = T.starts[c];
+      ccount = T.counts[c];
+    }
   }
-  __syncthreads();
+  int K, total;
+  const int slot = block_excl_scan((int)(ccount > 0), &K, lds);
+  const int pre = block_excl_scan(ccount, &total, lds);
+  if (ccount > 0) {
+    s_start[slot] = cstart;
+    s_pre[slot] = pre;
+  }
   if (threadIdx.x == 0) {
-    s_pre[0] = 0;
-    for (int k = 0; k < NK; ++k) s_pre[k + 1] += s_pre[k];
+    s_pre[K] = total;
+    s_stored = 0;
   }
   __syncthreads();
-  const int total = s_pre[NK];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int mine = 0;                            // COUNT: this thread's hits
-  const int gbase = EMIT ? off[b] : 0;     // EMIT: the group's first slot
-  int run = 0;                             // EMIT: hits of the earlier iterations
+  const int lane = threadIdx.x & 63;
+  int stored = 0;  // lane 0: this wave's stored matches
   for (int base = 0; base < total; base += blockDim.x) {
     const int idx = base + threadIdx.x;
     float normsq = 0.0f;
     bool hit = false;
     int node = 0;
     if (idx < total) {
-      int k = 0;  // the neighbour cell holding idx: last k with s_pre[k] <= idx
+      int k = 0;  // the nonempty cell holding idx: last k with s_pre[k] <= idx
       for (int step = 128; step > 0; step >>= 1)
-        if (k + step < NK && s_pre[k + step] <= idx) k += step;
+        if (k + step < K && s_pre[k + step] <= idx) k += step;
       const int pos = s_start[k] + (idx - s_pre[k]);
       for (int i = 0; i < ndof; ++i) {
         const float diff = T.sorted[pos + (size_t)cap * i] - qr[i];
@@ -283,82 +318,82 @@ __global__ void __launch_bounds__(256)
       hit = normsq < D2;
       node = T.sid[pos];
     }
-    if (!EMIT) {
-      mine += hit;
-      continue;
-    }
     const unsigned long long m = __ballot(hit);
-    if (lane == 0) s_wave[w] = __popcll(m);
-    __syncthreads();
-    int before = 0, all = 0;
-    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
-      before += i < w ? s_wave[i] : 0;
-      all += s_wave[i];
+    if (m == 0ull) continue;
+    int wbase = 0;
+    if (lane == 0) {
+      wbase = atomicAdd(&info->E_all, __popcll(m));
+      stored += max(0, min(__popcll(m), nbuf - wbase));
     }
-    if (hit) {
-      const int o = gbase + run + before + __popcll(m & ((1ull << lane) - 1ull));
-      if (o < nbuf) {
-        enode[o] = node;
-        ed2[o] = normsq;
-        egroup[o] = b;
-      }
+    wbase = __shfl(wbase, 0, 64);
+    const int o = wbase + __popcll(m & ((1ull << lane) - 1ull));
+    if (hit && o < nbuf) {
+      enode[o] = node;
+      ed2[o] = normsq;
+      egroup[o] = b;
     }
-    run += all;
-    __syncthreads();  // s_wave is rewritten by the next iteration
   }
-  if (!EMIT) {
-    __shared__ int s_cnt;
-    if (threadIdx.x == 0) s_cnt = 0;
-    __syncthreads();
-    for (int d = 32; d >= 1; d >>= 1) mine += __shfl_xor(mine, d, 64);
-    if (lane == 0) atomicAdd(&s_cnt, mine);
-    __syncthreads();
-    if (threadIdx.x == 0) cnt[b] = s_cnt;
-  }
+  if (lane == 0 && stored) atomicAdd(&s_stored, stored);
+  __syncthreads();
+  if (threadIdx.x == 0) cnt[b] = s_stored;
 }
 
-// Edges of group b actually stored (the first nbuf edges of the step are).
+// Edges of group b stored (cnt counts only slots below nbuf).
 __device__ __forceinline__ int stored_edges(const int* off, const int* cnt, int b, int nbuf) {
-  const int a = min(off[b], nbuf), e = min(off[b] + cnt[b], nbuf);
-  return e - a;
+  (void)off;
+  (void)nbuf;
+  return cnt[b];
 }
 
 __global__ void __launch_bounds__(kScanBlock)
     plan_scan_groups_kernel(int groups, const int* __restrict__ cnt, int nbuf, int budget, int* __restrict__ off,
-                            int* __restrict__ sc_off, StepInfo* info) {
+                            int* __restrict__ sc_off, int* __restrict__ fill, StepInfo* info) {
   __shared__ int lds[kMaxWaves + 1];
-  const int per = (groups + kScanBlock - 1) / kScanBlock;
-  const int b0 = threadIdx.x * per, b1 = min(groups, b0 + per);
-  int s = 0;
-  for (int b = b0; b < b1; ++b) s += cnt[b];
-  int all;
-  int run = block_excl_scan(s, &all, lds);
-  for (int b = b0; b < b1; ++b) {
-    off[b] = run;
-    run += cnt[b];
-  }
-  if (threadIdx.x == 0) off[groups] = all;
-  __syncthreads();
+  const int all = info->E_all;
   int S = 0;
   if (budget > 0) {
-    s = 0;
+    // group offsets of the stored edges, and of the scored groups' edges
+    const int per = (groups + kScanBlock - 1) / kScanBlock;
+    const int b0 = threadIdx.x * per, b1 = min(groups, b0 + per);
+    int s = 0, sc = 0;
     for (int b = b0; b < b1; ++b) {
-      const int w = stored_edges(off, cnt, b, nbuf);
-      s += w > budget ? w : 0;
+      s += cnt[b];
+      sc += cnt[b] > budget ? cnt[b] : 0;
     }
-    run = block_excl_scan(s, &S, lds);
+    int stored;
+    int run = block_excl_scan(s, &stored, lds);
+    int srun = block_excl_scan(sc, &S, lds);
     for (int b = b0; b < b1; ++b) {
-      sc_off[b] = run;
-      const int w = stored_edges(off, cnt, b, nbuf);
-      run += w > budget ? w : 0;
+      off[b] = run;
+      run += cnt[b];
+      sc_off[b] = srun;
+      srun += cnt[b] > budget ? cnt[b] : 0;
+      fill[b] = 0;
     }
-    if (threadIdx.x == 0) sc_off[groups] = S;
+    if (threadIdx.x == 0) {
+      off[groups] = stored;
+      sc_off[groups] = S;
+    }
   }
   if (threadIdx.x == 0) {
-    info->E_all = all;
     info->E = all < nbuf ? all : nbuf;
     info->S = S;
   }
+}
+
+// Edges of the radius search (any order) into group order: edge e of group
+// b to off[b] + its rank of arrival.
+__global__ void __launch_bounds__(256)
+    plan_group_scatter(const StepInfo* __restrict__ info, const int* __restrict__ gs, const int* __restrict__ ns,
+                       const float* __restrict__ ds, const int* __restrict__ off, int* __restrict__ fill,
+                       int* __restrict__ egroup, int* __restrict__ enode, float* __restrict__ ed2) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= info->E) return;
+  const int b = gs[e];
+  const int p = off[b] + atomicAdd(&fill[b], 1);
+  egroup[p] = b;
+  enode[p] = ns[e];
+  ed2[p] = ds[e];
 }
 
 // ---- edge budget ----------------------------------------------------------------
@@ -551,14 +586,29 @@ __global__ void __launch_bounds__(256)
                              const float* __restrict__ ed2, const int* __restrict__ res, const int* __restrict__ pos,
                              PlanTree t0, PlanTree t1, PlanState* state) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= E || res[e] != 0) return;
-  const int b = egroup[e];
-  const int p = pos[b];
+  int b = 0, p = -1;
+  if (e < E && res[e] == 0) {
+    b = egroup[e];
+    p = pos[b];
+  }
+  const int g = b & 1;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  // one edge-counter atomic per wave and tree
+  unsigned x = 0;
+  for (int t = 0; t < 2; ++t) {
+    const unsigned long long m = __ballot(p >= 0 && g == t);
+    if (m == 0ull) continue;
+    const int leader = __builtin_ctzll(m);
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(tree_of(t0, t1, t).ecount, (unsigned)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (p >= 0 && g == t) x = base + (unsigned)__popcll(m & lt);
+  }
   if (p < 0) return;
-  const PlanTree& T = tree_of(t0, t1, b & 1);
+  const PlanTree& T = tree_of(t0, t1, g);
   const int node = enode[e];
   atomicAdd(&T.surr[node], 1);
-  const unsigned x = atomicAdd(T.ecount, 1u);
   if (x < (unsigned)T.ecap) {
     T.efrom[x] = p;
     T.eto[x] = node;
@@ -628,6 +678,98 @@ __global__ void __launch_bounds__(256) plan_grid_scatter(int ndof, PlanTree t0, 
   T.sid[p] = l;
 }
 
+// ---- shortest paths from the roots (replaces the host's do_dijkstra) ----------
+// Bellman-Ford over each tree's edges: distances as float bits (non-negative
+// floats order as their bits), lowered by atomicMin; relaxation rounds until
+// one changes nothing.  Then every node's parent is the lowest-numbered
+// neighbour u with dist[u] + w == dist[v] and dist[u] < dist[v], and the path
+// is followed from the connection node back to the root.
+constexpr int kInfBits = 0x7f800000;
+
+__global__ void __launch_bounds__(256) plan_sssp_init(PlanTree t0, PlanTree t1, int cap, int* __restrict__ dist,
+                                                      int* __restrict__ parent) {
+  const PlanTree& T = tree_of(t0, t1, blockIdx.y);
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= *T.n) return;
+  dist[(size_t)blockIdx.y * cap + l] = l == 0 ? 0 : kInfBits;
+  parent[(size_t)blockIdx.y * cap + l] = 0x7fffffff;
+}
+
+__device__ __forceinline__ int tree_edges(const PlanTree& T) {
+  const unsigned c = *T.ecount;
+  return (int)(c < (unsigned)T.ecap ? c : (unsigned)T.ecap);
+}
+
+__global__ void __launch_bounds__(256) plan_sssp_relax(PlanTree t0, PlanTree t1, int cap, int* __restrict__ dist,
+                                                       int* __restrict__ changed) {
+  const PlanTree& T = tree_of(t0, t1, blockIdx.y);
+  int* D = dist + (size_t)blockIdx.y * cap;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  bool ch = false;
+  if (x < tree_edges(T)) {
+    const int u = T.efrom[x], v = T.eto[x];
+    const float w = T.ew[x];
+    const float du = __int_as_float(D[u]), dv = __int_as_float(D[v]);
+    const float a = du + w, c = dv + w;
+    if (a < dv) {
+      atomicMin(&D[v], __float_as_int(a));
+      ch = true;
+    }
+    if (c < du) {
+      atomicMin(&D[u], __float_as_int(c));
+      ch = true;
+    }
+  }
+  if (__ballot(ch) && (threadIdx.x & 63) == 0) changed[blockIdx.y] = 1;
+}
+
+__global__ void __launch_bounds__(256) plan_sssp_parent(PlanTree t0, PlanTree t1, int cap, const int* __restrict__ dist,
+                                                        int* __restrict__ parent) {
+  const PlanTree& T = tree_of(t0, t1, blockIdx.y);
+  const int* D = dist + (size_t)blockIdx.y * cap;
+  int* P = parent + (size_t)blockIdx.y * cap;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= tree_edges(T)) return;
+  const int u = T.efrom[x], v = T.eto[x];
+  const float w = T.ew[x];
+  const float du = __int_as_float(D[u]), dv = __int_as_float(D[v]);
+  if (du < dv && du + w == dv) atomicMin(&P[v], u);
+  if (dv < du && dv + w == du) atomicMin(&P[u], v);
+}
+
+// One thread per tree: the chain from `from` (conn_left, conn_right) back to
+// node 0, at most maxlen nodes; len[tree] = -1 when a parent is missing.
+__global__ void plan_path_chase(int cap, const int* __restrict__ parent, const PlanState* __restrict__ state,
+                                int maxlen, int* __restrict__ chain, int* __restrict__ len) {
+  const int g = threadIdx.x;
+  if (g >= 2) return;
+  const int* P = parent + (size_t)g * cap;
+  int v = g == 0 ? state->conn_left : state->conn_right;
+  int n = 0;
+  for (;;) {
+    if (n >= maxlen || v < 0 || v >= cap) {
+      n = -1;
+      break;
+    }
+    chain[(size_t)g * maxlen + n++] = v;
+    if (v == 0) break;
+    v = P[v];
+  }
+  len[g] = n;
+}
+
+// Coordinates of the chain nodes of both trees, AoS: out[(g*maxlen + i)*ndof + c].
+__global__ void __launch_bounds__(256) plan_path_gather(int ndof, PlanTree t0, PlanTree t1, int cap, int maxlen,
+                                                        const int* __restrict__ chain, const int* __restrict__ len,
+                                                        float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.y;
+  if (i >= len[g]) return;
+  const PlanTree& T = tree_of(t0, t1, g);
+  const int v = chain[(size_t)g * maxlen + i];
+  for (int c = 0; c < ndof; ++c) out[((size_t)g * maxlen + i) * ndof + c] = T.q[v + (size_t)cap * c];
+}
+
 inline unsigned blocks_for(long long n, int block) { return (unsigned)((n + block - 1) / block); }
 
 }  // namespace
@@ -654,30 +796,62 @@ hipError_t launch_plan_bias_mask(const float* prob, int batch, unsigned char* sk
 }
 
 hipError_t launch_plan_select(int ndof, int batch, const int* codes, const unsigned char* skip, const float* cand,
-                              int want0, int want1, int num, float* qnew, StepInfo* info, PlanState* state,
+                              int want0, int want1, int num, float* qnew, int* tile, StepInfo* info, PlanState* state,
                               hipStream_t st) {
-  hipLaunchKernelGGL(plan_select_kernel, dim3(1), dim3(kScanBlock), 0, st, ndof, batch, codes, skip, cand, want0, want1,
-                     num, qnew, info, state);
+  const int ntiles = (int)blocks_for(batch, kScanBlock);
+  hipLaunchKernelGGL(plan_select_count, dim3(ntiles), dim3(kScanBlock), 0, st, batch, codes, skip, tile);
+  hipLaunchKernelGGL(plan_select_prefix, dim3(1), dim3(kScanBlock), 0, st, batch, ntiles, want0, want1, tile, info,
+                     state);
+  hipLaunchKernelGGL(plan_select_write, dim3(ntiles), dim3(kScanBlock), 0, st, ndof, batch, codes, skip, cand, want0,
+                     want1, num, tile, qnew);
   return hipGetLastError();
 }
 
-hipError_t launch_plan_near(bool emit, int ndof, int num, const float* qnew, const StepInfo* info, PlanTree t0,
-                            PlanTree t1, int cap, const CellGrid& g, float D2, int* cnt, const int* off, int nbuf,
-                            int* enode, float* ed2, int* egroup, hipStream_t st) {
+hipError_t launch_plan_near(int ndof, int num, const float* qnew, StepInfo* info, PlanTree t0, PlanTree t1, int cap,
+                            const CellGrid& g, float D2, int* cnt, int nbuf, int* enode, float* ed2, int* egroup,
+                            hipStream_t st) {
   if (num <= 0) return hipSuccess;
-  if (emit)
-    hipLaunchKernelGGL(plan_near_kernel<true>, dim3(2 * num), dim3(256), 0, st, ndof, num, qnew, info, t0, t1, cap, g,
-                       D2, cnt, off, nbuf, enode, ed2, egroup);
-  else
-    hipLaunchKernelGGL(plan_near_kernel<false>, dim3(2 * num), dim3(256), 0, st, ndof, num, qnew, info, t0, t1, cap, g,
-                       D2, cnt, off, nbuf, enode, ed2, egroup);
+  hipLaunchKernelGGL(plan_near_kernel, dim3(2 * num), dim3(256), 0, st, ndof, num, qnew, info, t0, t1, cap, g, D2, cnt,
+                     nbuf, enode, ed2, egroup);
   return hipGetLastError();
 }
 
 hipError_t launch_plan_scan_groups(int groups, const int* cnt, int nbuf, int budget, int* off, int* sc_off,
-                                   StepInfo* info, hipStream_t st) {
+                                   int* fill, StepInfo* info, hipStream_t st) {
   hipLaunchKernelGGL(plan_scan_groups_kernel, dim3(1), dim3(kScanBlock), 0, st, groups, cnt, nbuf, budget, off, sc_off,
-                     info);
+                     fill, info);
+  return hipGetLastError();
+}
+
+hipError_t launch_plan_group_scatter(int nbuf, const StepInfo* info, const int* gs, const int* ns, const float* ds,
+                                     const int* off, int* fill, int* egroup, int* enode, float* ed2, hipStream_t st) {
+  hipLaunchKernelGGL(plan_group_scatter, dim3(blocks_for(nbuf, 256)), dim3(256), 0, st, info, gs, ns, ds, off, fill,
+                     egroup, enode, ed2);
+  return hipGetLastError();
+}
+
+hipError_t launch_plan_paths_init(PlanTree t0, PlanTree t1, int cap, int* dist, int* parent, hipStream_t st) {
+  hipLaunchKernelGGL(plan_sssp_init, dim3(blocks_for(cap, 256), 2), dim3(256), 0, st, t0, t1, cap, dist, parent);
+  return hipGetLastError();
+}
+
+hipError_t launch_plan_paths_relax(PlanTree t0, PlanTree t1, int cap, int max_edges, int* dist, int* changed,
+                                   hipStream_t st) {
+  if (max_edges <= 0) return hipSuccess;
+  hipLaunchKernelGGL(plan_sssp_relax, dim3(blocks_for(max_edges, 256), 2), dim3(256), 0, st, t0, t1, cap, dist,
+                     changed);
+  return hipGetLastError();
+}
+
+hipError_t launch_plan_paths_finish(int ndof, PlanTree t0, PlanTree t1, int cap, int max_edges, const int* dist,
+                                    int* parent, const PlanState* state, int maxlen, int* chain, int* len, float* coords,
+                                    hipStream_t st) {
+  if (max_edges > 0)
+    hipLaunchKernelGGL(plan_sssp_parent, dim3(blocks_for(max_edges, 256), 2), dim3(256), 0, st, t0, t1, cap, dist,
+                       parent);
+  hipLaunchKernelGGL(plan_path_chase, dim3(1), dim3(64), 0, st, cap, parent, state, maxlen, chain, len);
+  hipLaunchKernelGGL(plan_path_gather, dim3(blocks_for(maxlen, 256), 2), dim3(256), 0, st, ndof, t0, t1, cap, maxlen,
+                     chain, len, coords);
   return hipGetLastError();
 }
 

@@ -215,8 +215,8 @@ struct Worker {
   int slot = -1;
   Slot* sl = nullptr;
   hipStream_t st = nullptr;
-  DevArr cand, codes, pskip, pprob, qnew, info, cnt, off, sc_off, enode, ed2, egroup, eskip, escore, er, qs, qe,
-      free_cnt, pos, spts, sprob;
+  DevArr cand, codes, pskip, pprob, qnew, info, tile, cnt, off, sc_off, fill, enode, ed2, egroup, snode, sd2, sgroup,
+      eskip, escore, er, qs, qe, free_cnt, pos, spts, sprob;
   Readback* pin = nullptr;
   hipEvent_t ev_rb = nullptr;
   int E = 0, S = 0, batch = 0;
@@ -259,7 +259,7 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
   const bool bias = model && prm->bias_sampling;
   const bool rank_by_model = model && prm->order_edges;
   const int budget = prm->edge_budget;
-  const int W = prm->workers == 0 ? 2 : prm->workers;
+  const int W = prm->workers == 0 ? 1 : prm->workers;
   const int num = prm->num, cap = prm->max_nodes, nbuf = prm->nbuf;
   const int groups = 2 * num;
   const float D = prm->D, D2 = D * D;
@@ -346,8 +346,11 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
         !x.er.ensure(sizeof(int) * (size_t)nbuf) || !x.qs.ensure(sizeof(float) * nd * (size_t)nbuf) ||
         !x.qe.ensure(sizeof(float) * nd * (size_t)nbuf) || !x.free_cnt.ensure(sizeof(int) * (size_t)groups) ||
         !x.pos.ensure(sizeof(int) * (size_t)groups) ||
+        !x.tile.ensure(sizeof(int) * 3 * (size_t)(batch_max / 1024 + 1)) ||
         (bias && (!x.pskip.ensure((size_t)batch_max) || !x.pprob.ensure(sizeof(float) * (size_t)batch_max))) ||
-        (budget > 0 && (!x.eskip.ensure((size_t)nbuf) || !x.escore.ensure(sizeof(float) * (size_t)nbuf))))
+        (budget > 0 && (!x.eskip.ensure((size_t)nbuf) || !x.escore.ensure(sizeof(float) * (size_t)nbuf) ||
+                        !x.fill.ensure(sizeof(int) * (size_t)groups) || !x.snode.ensure(sizeof(int) * (size_t)nbuf) ||
+                        !x.sd2.ensure(sizeof(float) * (size_t)nbuf) || !x.sgroup.ensure(sizeof(int) * (size_t)nbuf))))
       return GPRM_E_HIP;
     if ((rc = reserve_slot(ctx, *x.sl, batch_max, 0, maxT_bound)) != GPRM_OK) return rc;
   }
@@ -436,18 +439,22 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
     int r = enqueue_points(ctx, *w.sl, w.cand.as<float>(), w.batch, w.batch, w.codes.as<int>(), st, pskip);
     if (r != GPRM_OK) return r;
     if (launch_plan_select(nd, w.batch, w.codes.as<int>(), pskip, w.cand.as<float>(), want0, want1, num,
-                           w.qnew.as<float>(), info, state, st) != hipSuccess ||
+                           w.qnew.as<float>(), w.tile.as<int>(), info, state, st) != hipSuccess ||
         !end_iv(iv_s, st))
       return GPRM_E_HIP;
+    // the radius search stores its edges in arrival order; the edge budget
+    // ranks within groups, so it moves them into group order first
+    const bool grouped = budget > 0;
     const int iv_n = begin_iv(st, &res->ms_near);
     if (iv_n < 0 ||
-        launch_plan_near(false, nd, num, w.qnew.as<float>(), info, T[0], T[1], cap, grid, D2, w.cnt.as<int>(),
-                         nullptr, nbuf, nullptr, nullptr, nullptr, st) != hipSuccess ||
-        launch_plan_scan_groups(groups, w.cnt.as<int>(), nbuf, budget, w.off.as<int>(), w.sc_off.as<int>(), info,
-                                st) != hipSuccess ||
-        launch_plan_near(true, nd, num, w.qnew.as<float>(), info, T[0], T[1], cap, grid, D2, w.cnt.as<int>(),
-                         w.off.as<int>(), nbuf, w.enode.as<int>(), w.ed2.as<float>(), w.egroup.as<int>(), st) !=
-            hipSuccess ||
+        launch_plan_near(nd, num, w.qnew.as<float>(), info, T[0], T[1], cap, grid, D2, w.cnt.as<int>(), nbuf,
+                         (grouped ? w.snode : w.enode).as<int>(), (grouped ? w.sd2 : w.ed2).as<float>(),
+                         (grouped ? w.sgroup : w.egroup).as<int>(), st) != hipSuccess ||
+        launch_plan_scan_groups(groups, w.cnt.as<int>(), nbuf, budget, w.off.as<int>(), w.sc_off.as<int>(),
+                                w.fill.as<int>(), info, st) != hipSuccess ||
+        (grouped && launch_plan_group_scatter(nbuf, info, w.sgroup.as<int>(), w.snode.as<int>(), w.sd2.as<float>(),
+                                              w.off.as<int>(), w.fill.as<int>(), w.egroup.as<int>(),
+                                              w.enode.as<int>(), w.ed2.as<float>(), st) != hipSuccess) ||
         !end_iv(iv_n, st) || hipEventRecord(ev_chain, st) != hipSuccess ||
         hipMemcpyAsync(&w.pin->info, info, sizeof(StepInfo), hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipMemcpyAsync(&w.pin->state, state, sizeof(PlanState), hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -577,58 +584,118 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
   res->nodes_left = fin.n[0];
   res->nodes_right = fin.n[1];
   res->ms_wait = ms_wait;
-  Tree H[2];
-  for (int g = 0; g < 2; ++g) {
-    Tree& t = H[g];
-    t.ndof = nd;
-    t.n = fin.n[g];
-    std::vector<float> soa((size_t)nd * t.n);
-    for (int i = 0; i < nd; ++i)
-      if (hipMemcpy(soa.data() + (size_t)t.n * i, T[g].q + (size_t)cap * i, sizeof(float) * t.n,
+  const int conn_left = fin.conn ? fin.conn_left : -1, conn_right = fin.conn ? fin.conn_right : -1;
+  // chains conn_left -> start root (tree 0) and conn_right -> goal root
+  // (tree 1); pl = [conn_left, ..., 0], pr = [0, ..., conn_right] as the
+  // reference's dijkstra_result paths
+  std::vector<int> pl, pr;
+  std::vector<float> coords;  // path configurations, AoS
+  bool connected = false;
+  if (conn_left >= 0) {
+    // shortest paths on the device: relaxation rounds in batches of 8, each
+    // round with its own changed flags, until a round changes nothing
+    const int max_edges = (int)std::min<unsigned>(std::max(fin.ecount[0], fin.ecount[1]), (unsigned)ecap);
+    const int maxlen = std::min(cap, 1 << 16);
+    constexpr int kRounds = 8;
+    DevArr dist, parent, flags, chain, lens, pcoords;
+    hipStream_t st = wk[0]->st;
+    if (!dist.ensure(sizeof(int) * 2 * (size_t)cap) || !parent.ensure(sizeof(int) * 2 * (size_t)cap) ||
+        !flags.ensure(sizeof(int) * 2 * kRounds) || !chain.ensure(sizeof(int) * 2 * (size_t)maxlen) ||
+        !lens.ensure(sizeof(int) * 2) || !pcoords.ensure(sizeof(float) * 2 * (size_t)maxlen * nd))
+      return GPRM_E_HIP;
+    if (launch_plan_paths_init(T[0], T[1], cap, dist.as<int>(), parent.as<int>(), st) != hipSuccess)
+      return GPRM_E_HIP;
+    const long long max_rounds = (long long)std::max(fin.n[0], fin.n[1]) + 1;
+    int hf[2 * kRounds];
+    for (long long r = 0; r < max_rounds; r += kRounds) {
+      if (launch_fill_i32(flags.as<int>(), 0, 2 * kRounds, st) != hipSuccess) return GPRM_E_HIP;
+      for (int k = 0; k < kRounds; ++k)
+        if (launch_plan_paths_relax(T[0], T[1], cap, max_edges, dist.as<int>(), flags.as<int>() + 2 * k, st) !=
+            hipSuccess)
+          return GPRM_E_HIP;
+      if (hipMemcpyAsync(hf, flags.p, sizeof(hf), hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        return GPRM_E_HIP;
+      if (hf[2 * kRounds - 2] == 0 && hf[2 * kRounds - 1] == 0) break;
+    }
+    int hl[2] = {-1, -1};
+    if (launch_plan_paths_finish(nd, T[0], T[1], cap, max_edges, dist.as<int>(), parent.as<int>(), state, maxlen,
+                                 chain.as<int>(), lens.as<int>(), pcoords.as<float>(), st) != hipSuccess ||
+        hipMemcpyAsync(hl, lens.p, sizeof(hl), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return GPRM_E_HIP;
+    if (hl[0] > 0 && hl[1] > 0) {
+      std::vector<int> c0(hl[0]), c1(hl[1]);
+      std::vector<float> x0((size_t)hl[0] * nd), x1((size_t)hl[1] * nd);
+      if (hipMemcpy(c0.data(), chain.p, sizeof(int) * hl[0], hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(c1.data(), chain.as<int>() + maxlen, sizeof(int) * hl[1], hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(x0.data(), pcoords.p, sizeof(float) * x0.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(x1.data(), pcoords.as<float>() + (size_t)maxlen * nd, sizeof(float) * x1.size(),
                     hipMemcpyDeviceToHost) != hipSuccess)
         return GPRM_E_HIP;
-    t.q.resize((size_t)nd * t.n);
-    for (int m = 0; m < t.n; ++m)
-      for (int i = 0; i < nd; ++i) t.q[(size_t)nd * m + i] = soa[(size_t)t.n * i + m];
-    t.surrnum.resize(t.n);
-    const int ne = (int)std::min<unsigned>(fin.ecount[g], (unsigned)ecap);
-    std::vector<int> ef(ne), et(ne);
-    std::vector<float> ew(ne);
-    if (hipMemcpy(t.surrnum.data(), T[g].surr, sizeof(int) * t.n, hipMemcpyDeviceToHost) != hipSuccess ||
-        (ne > 0 && (hipMemcpy(ef.data(), T[g].efrom, sizeof(int) * ne, hipMemcpyDeviceToHost) != hipSuccess ||
-                    hipMemcpy(et.data(), T[g].eto, sizeof(int) * ne, hipMemcpyDeviceToHost) != hipSuccess ||
-                    hipMemcpy(ew.data(), T[g].ew, sizeof(float) * ne, hipMemcpyDeviceToHost) != hipSuccess)))
-      return GPRM_E_HIP;
-    t.ef.resize(2 * (size_t)ne);
-    t.et.resize(2 * (size_t)ne);
-    t.ew.resize(2 * (size_t)ne);
-    for (int k = 0; k < ne; ++k) {
-      t.ef[2 * k] = ef[k];
-      t.et[2 * k] = et[k];
-      t.ef[2 * k + 1] = et[k];
-      t.et[2 * k + 1] = ef[k];
-      t.ew[2 * k] = t.ew[2 * k + 1] = ew[k];
+      pl = c0;
+      pr.assign(c1.rbegin(), c1.rend());
+      // start ... conn (tree 0), then conn's successors ... goal (tree 1);
+      // conn_left and conn_right hold the same configuration
+      for (int i = hl[0] - 1; i >= 0; --i) coords.insert(coords.end(), &x0[(size_t)nd * i], &x0[(size_t)nd * (i + 1)]);
+      for (int i = 1; i < hl[1]; ++i) coords.insert(coords.end(), &x1[(size_t)nd * i], &x1[(size_t)nd * (i + 1)]);
+      connected = true;
     }
   }
-  const int conn_left = fin.conn ? fin.conn_left : -1, conn_right = fin.conn ? fin.conn_right : -1;
-  std::vector<int> pl, pr;
-  bool connected = false;
+  // the trees on the host: for the output files, or when the device chains
+  // broke (a parent lost to float absorption), for Dijkstra on the host
+  Tree H[2];
   Csr G[2];
-  if (conn_left >= 0 || prm->output_dir)
-    for (int g = 0; g < 2; ++g) G[g] = csr(H[g]);
-  if (conn_left >= 0) {
-    connected = dijkstra(H[0], G[0], 0, conn_left, &pl) && dijkstra(H[1], G[1], conn_right, 0, &pr);
-    if (connected) {
-      // pl = [conn_left, ..., start], pr = [goal, ..., conn_right]; conn_left
-      // and conn_right hold the same configuration
-      std::vector<const float*> seq;
-      for (int i = (int)pl.size() - 1; i >= 0; --i) seq.push_back(&H[0].q[(size_t)nd * pl[i]]);
-      for (int i = (int)pr.size() - 2; i >= 0; --i) seq.push_back(&H[1].q[(size_t)nd * pr[i]]);
-      res->found = 1;
-      res->path_len = (int)seq.size();
-      if (path)
-        for (int i = 0; i < std::min(path_cap, res->path_len); ++i) std::memcpy(path + (size_t)nd * i, seq[i], 4 * nd);
+  const bool need_host = prm->output_dir != nullptr || (conn_left >= 0 && !connected);
+  if (need_host) {
+    for (int g = 0; g < 2; ++g) {
+      Tree& t = H[g];
+      t.ndof = nd;
+      t.n = fin.n[g];
+      std::vector<float> soa((size_t)nd * t.n);
+      for (int i = 0; i < nd; ++i)
+        if (hipMemcpy(soa.data() + (size_t)t.n * i, T[g].q + (size_t)cap * i, sizeof(float) * t.n,
+                      hipMemcpyDeviceToHost) != hipSuccess)
+          return GPRM_E_HIP;
+      t.q.resize((size_t)nd * t.n);
+      for (int m = 0; m < t.n; ++m)
+        for (int i = 0; i < nd; ++i) t.q[(size_t)nd * m + i] = soa[(size_t)t.n * i + m];
+      t.surrnum.resize(t.n);
+      const int ne = (int)std::min<unsigned>(fin.ecount[g], (unsigned)ecap);
+      std::vector<int> ef(ne), et(ne);
+      std::vector<float> ew(ne);
+      if (hipMemcpy(t.surrnum.data(), T[g].surr, sizeof(int) * t.n, hipMemcpyDeviceToHost) != hipSuccess ||
+          (ne > 0 && (hipMemcpy(ef.data(), T[g].efrom, sizeof(int) * ne, hipMemcpyDeviceToHost) != hipSuccess ||
+                      hipMemcpy(et.data(), T[g].eto, sizeof(int) * ne, hipMemcpyDeviceToHost) != hipSuccess ||
+                      hipMemcpy(ew.data(), T[g].ew, sizeof(float) * ne, hipMemcpyDeviceToHost) != hipSuccess)))
+        return GPRM_E_HIP;
+      t.ef.resize(2 * (size_t)ne);
+      t.et.resize(2 * (size_t)ne);
+      t.ew.resize(2 * (size_t)ne);
+      for (int k = 0; k < ne; ++k) {
+        t.ef[2 * k] = ef[k];
+        t.et[2 * k] = et[k];
+        t.ef[2 * k + 1] = et[k];
+        t.et[2 * k + 1] = ef[k];
+        t.ew[2 * k] = t.ew[2 * k + 1] = ew[k];
+      }
+      G[g] = csr(t);
     }
+    if (conn_left >= 0 && !connected) {
+      connected = dijkstra(H[0], G[0], 0, conn_left, &pl) && dijkstra(H[1], G[1], conn_right, 0, &pr);
+      if (connected) {
+        // pl = [conn_left, ..., start], pr = [goal, ..., conn_right]
+        for (int i = (int)pl.size() - 1; i >= 0; --i)
+          coords.insert(coords.end(), &H[0].q[(size_t)nd * pl[i]], &H[0].q[(size_t)nd * (pl[i] + 1)]);
+        for (int i = (int)pr.size() - 2; i >= 0; --i)
+          coords.insert(coords.end(), &H[1].q[(size_t)nd * pr[i]], &H[1].q[(size_t)nd * (pr[i] + 1)]);
+      }
+    }
+  }
+  if (connected) {
+    res->found = 1;
+    res->path_len = (int)(coords.size() / nd);
+    if (path) std::memcpy(path, coords.data(), sizeof(float) * nd * (size_t)std::min(path_cap, res->path_len));
   }
   res->ms_final = ms_since(tf);
   res->ms_total = ms_since(t0);
