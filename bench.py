@@ -10,7 +10,16 @@ robotconfigspace.cpp:660-673).
 
 One step = deciding every edge (gprm_indicator2_dev, inputs resident in HBM).
 `value` counts reference-equivalent checks: the sum of testnum, i.e. the
-configuration checks the reference's indicator2 performs for the same job.
+configuration checks the reference's indicator2 performs for the same job;
+`evaluated_checks_per_s` counts the configurations actually decided (after
+early exit).
+
+Further legs (--extras): point checks of the 4-DOF / 8-box configuration
+with their roofline and the latency of small host-array calls; 64 scenes in
+one graph-replayed batch (the committed tuning table, a fresh tuning run
+beside it) with roofline and CPU baseline; the planner's time-to-path on the
+7-DOF scene, the 7-DOF configuration as written (200,000 uniform samples),
+and plain vs surrogate over 16 tasks of the 6-DOF scene.
 
 Prints one JSON line.
 """
@@ -30,13 +39,59 @@ sys.path.insert(0, ROOT)
 
 METRIC = "exact collision checks/s on 1 MI355X; PRM time-to-path, surrogate vs plain"
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
+# the FP32 operations of a wave64 VALU instruction without FMA or packed math
+# (the reference's rounding admits neither): 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+VALU_NOFMA_TOPS = 39.3
 SEED = 2024
 NDOF = 6
 M_PAIRS = 0
-# HBM bytes per sva_kernel<1> launch of this workload (FETCH_SIZE + WRITE_SIZE,
-# each scaled by the same counter's reading for a known-size copy kernel),
-# from profiles/sva_kernel_edges_wl2_traffic.md
-TRAFFIC_BYTES_PER_LAUNCH = 3.463e9
+# Counter figures of the dominant kernel on this workload (HBM bytes per
+# launch from FETCH_SIZE / WRITE_SIZE, scaled by a known-size copy; VALU
+# instruction counts), measured by rocprofv3 on a build of the sources whose
+# hash the file records; see profiles/sva_kernel_edges_wl2_traffic.md.
+PROFILE_SIDECAR = os.path.join(ROOT, "profiles", "sva_kernel_edges_wl2.json")
+PROFILED_SOURCES = ("gpuprm_amd/csrc/kernels.hip", "gpuprm_amd/csrc/gprm_api.cpp", "gpuprm_amd/csrc/runtime.h")
+
+
+def sources_sha16():
+    """Hash of the sources that define the separating-vector kernel and its
+    launches (the counter sidecar holds the hash of the build it measured)."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for f in PROFILED_SOURCES:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def profile_sidecar():
+    """The counter figures if they were measured on these sources, else None."""
+    try:
+        with open(PROFILE_SIDECAR) as f:
+            p = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return p if p.get("sources_sha16") == sources_sha16() else None
+
+
+def host_cpu():
+    """Model, sockets, cores and threads of the host (lscpu), for the CPU baseline."""
+    import subprocess
+
+    info = {"logical_cpus": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                v = v.strip()
+                info[keys[k.strip()]] = int(v) if v.isdigit() else v
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
 
 
 def log(msg):
@@ -158,9 +213,51 @@ def _cpu_pool(scene_dir, nthreads):
     return _Port(), "port"
 
 
+def flops_per_check(scene_dir, Q, nsamp, rng):
+    """Mean algorithmic FP32 operations of one configuration check (the
+    reference's pair loop to the first collision, separating-vector tests
+    only), from the oracle's counting build on nsamp of the configurations."""
+    from oracle.oracle import OracleScene
+
+    o = OracleScene(scene_dir, 0.01, counting=True)
+    lo = np.fromfile(os.path.join(scene_dir, "mins.bin"), np.float32)
+    hi = np.fromfile(os.path.join(scene_dir, "maxs.bin"), np.float32)
+    idx = rng.choice(Q.shape[1], size=min(nsamp, Q.shape[1]), replace=False)
+    tot = 0.0
+    for k in idx:
+        q = np.ascontiguousarray(Q[:, k])
+        if np.any(q < lo) or np.any(q > hi):
+            continue  # out of bounds: no pair test
+        tot += float(o.pair_profile(q)[1].sum())
+    return tot / len(idx)
+
+
+def latency(sp, Q, QS, QE):
+    """Microseconds per host-array call of the calls a reference-style planner
+    makes: indicator(q) of one configuration (prmsolver.cpp:2611), indicator
+    of 8, and indicator2 of 8 edges (through ctypes, about 1 us of it)."""
+    q1 = np.ascontiguousarray(Q[:, 0])
+    Q8 = np.ascontiguousarray(Q[:, :8])
+    S8, E8 = np.ascontiguousarray(QS[:, :8]), np.ascontiguousarray(QE[:, :8])
+    r8 = np.zeros(8, np.int32)
+    out = {}
+    for name, fn, reps in (("us_indicator1", lambda: sp.indicator(q1), 400),
+                           ("us_indicator_8", lambda: sp.indicator(Q8.ravel(), r8, 8, 8), 200),
+                           ("us_indicator2_8_edges", lambda: sp.indicator2(S8.ravel(), E8.ravel(), r8, 8, 8), 100)):
+        for _ in range(10):
+            fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        out[name] = 1e6 * (time.perf_counter() - t0) / reps
+    return out
+
+
 def leg_points(steps):
     """Point checks: 4-DOF arm, 8 boxes, 65,536 configurations (one
-    gprm_indicator_dev call per step), with the CPU path on a bounded sample."""
+    gprm_indicator_dev call per step), with the CPU path on a bounded sample;
+    the roofline of the call (reference FP32 operations per check x
+    checks/s) and the latency of small host-array calls."""
     import torch
 
     import gpuprm_amd
@@ -182,6 +279,13 @@ def leg_points(steps):
         sp.indicator_dev(q, r)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
+    # the same calls, HIP events around the separating-vector launches
+    sp.profile_enable(True)
+    for _ in range(steps):
+        sp.indicator_dev(q, r)
+    torch.cuda.synchronize()
+    prof = sp.profile_read()
+    sp.profile_enable(False)
     got = r.cpu().numpy()
     sub = rng.choice(65536, 8192, replace=False)
     mism = int((got[sub] != OracleScene(d).indicator_batch(Q[:, sub].copy(), nthreads=min(16, os.cpu_count() or 1))).sum())
@@ -196,8 +300,18 @@ def leg_points(steps):
         pool.indicator_batch(Q)
         reps += 1
     cdt = (time.perf_counter() - t0) / reps
+    fpc = flops_per_check(d, Q, 4000, rng)
+    achieved = 65536 * fpc / dt / 1e12
+    QE = np.clip(Q + rng.normal(0, 0.2, size=Q.shape), -np.pi, np.pi).astype(np.float32)
     return {"workload": "4-DOF arm, 8 obstacle boxes, 65536 configurations", "checks_per_s": 65536 / dt,
             "ms_per_call": 1000 * dt, "code_mismatches": mism,
+            "sva_ms_per_call": prof.sva_ms / max(1, prof.sva_launches),
+            "roofline": {"bound": "valu", "pipe": "fp32 valu", "unit": "TFLOP/s", "peak": PEAK_FP32_TFLOPS,
+                         "achieved": round(achieved, 3), "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                         "flops_per_check": round(fpc, 1), "traffic": None,
+                         "note": "whole call (kinematics, separating-vector kernel, finalize) over the reference's "
+                                 "FP32 operations per check from the oracle's counting build"},
+            "latency": latency(sp, Q, Q, QE),
             "cpu": {"value": 65536 / cdt, "unit": "checks/s", "cores": nthreads, "kind": kind,
                     "sample": "the 65536 configurations %d times, %.2f s each" % (reps, cdt)},
             "speedup_vs_cpu": cdt / dt}
@@ -224,8 +338,10 @@ def leg_multi_scene(reps):
     sizes = [int(x) for x in rng.integers(1000, 65537, 64)]
     qs = [torch.from_numpy(rng.uniform(-np.pi, np.pi, size=(7, n)).astype(np.float32)).cuda() for n in sizes]
     rs = [torch.zeros(n, dtype=torch.int32, device="cuda") for n in sizes]
+    # the committed MI355X table (SceneBatch's default); a fresh tuning run
+    # is reported beside it, not applied
     b = gpuprm_amd.SceneBatch(spaces)
-    table = b.autotune(qs, rs, reps=1)
+    table = b.tuning()
     b.indicator_dev(qs, rs, use_graph=True)  # capture
     torch.cuda.synchronize()
     times, lat = [], []
@@ -243,12 +359,45 @@ def leg_multi_scene(reps):
     dt = float(np.median(times))
     L = np.median(np.stack(lat), axis=0)
     info = b.bucket_info()
+    # algorithmic work: per scene the reference's FP32 operations per check
+    # (oracle counting build, 48 configurations each) x its configurations
+    frng = np.random.default_rng(SEED + 6)
+    flops = sum(sizes[s] * flops_per_check(dirs[s], qs[s][:, :2048].cpu().numpy(), 48, frng) for s in range(64))
+    achieved = flops / dt / 1e12
+    # the reference's CPU path on a bounded sample: configurations of 8 of
+    # the scenes, all host cores, about 4 s in all (sized from a first
+    # 2000-configuration call)
+    nthreads = max(1, min(16, os.cpu_count() or 1))
+    csum, ct = 0, 0.0
+    kind = None
+    per = 2000
+    for s in range(0, 64, 8):
+        pool, kind = _cpu_pool(dirs[s], nthreads)
+        Qs = qs[s][:, :per].cpu().numpy().copy()
+        t0 = time.perf_counter()
+        pool.indicator_batch(Qs)
+        ct += time.perf_counter() - t0
+        csum += Qs.shape[1]
+        if s == 0:
+            per = int(min(sizes[s], max(2000, 0.5 * per / max(ct, 1e-6))))
+    retune = b.autotune(qs, rs, reps=3)
     out = {"workload": "64 scenes: 7-DOF arm, 8..128 hulls of 8..24 points, 1k-64k configurations each",
            "configs": int(sum(sizes)), "checks_per_s": sum(sizes) / dt, "ms_per_batch": 1000 * dt,
            "scene_done_ms_p50": float(np.percentile(L, 50)), "scene_done_ms_p99": float(np.percentile(L, 99)),
            "buckets": [{"vertices": [x["min_vertices"], x["max_vertices"]], "scenes": x["scenes"],
                         "shape": x["shape"]} for x in info if x["scenes"]],
-           "tuning": table, "mismatches_vs_single_scene": mism}
+           "tuning": table, "tuning_source": "gpuprm_amd/tuning_mi355x.json (committed)",
+           "retune": {"table": retune, "reps": 3, "statistic": "median", "same_as_committed": retune == table},
+           "roofline": {"bound": "valu", "pipe": "fp32 valu", "unit": "TFLOP/s", "peak": PEAK_FP32_TFLOPS,
+                        "achieved": round(achieved, 3), "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                        "flops_per_check": round(flops / sum(sizes), 1), "traffic": None,
+                        "note": "graph replay (kinematics, one separating-vector launch per bucket, finalize) "
+                                "over the reference's FP32 operations per check, oracle counting build"},
+           "cpu_baseline": {"value": csum / ct, "unit": "checks/s", "cores": nthreads, "kind": kind,
+                            "sample": "%d configurations of 8 of the 64 scenes, %.1f s" % (csum, ct)},
+           "speedup_vs_cpu": (sum(sizes) / dt) / (csum / ct),
+           "mismatches_vs_single_scene": mism}
+    b.apply_tuning(table)
     # edge sets: per scene the k=10 nearest-neighbour edges among its first
     # 2048 configurations, all scenes in one graph-replayed edge batch
     QS, QE = [], []
@@ -319,11 +468,76 @@ def _path_ok(o, path):
     return bool(np.all(res == 0) and np.all(o.indicator_batch(path.T) == 0))
 
 
+def _prm_run(sp, o, a, b, verify=True, **kw):
+    r, path = sp.prm_solve(a, b, **kw)
+    return {"found": r.found, "valid": _path_ok(o, path) if verify else None, "ms": r.ms_total, "steps": r.steps,
+            "nodes": r.nodes_left + r.nodes_right, "edges_checked": r.edges_checked,
+            "edge_samples": r.edge_samples, "point_checks": r.point_checks,
+            # reference-equivalent checks (edge samples + point checks) per second of the whole run
+            "checks_per_s": (r.edge_samples + r.point_checks) / max(1e-9, r.ms_total / 1000),
+            "ms_device": {"sampling": round(r.ms_sampling, 2), "near": round(r.ms_near, 2),
+                          "edges": round(r.ms_edges, 2), "insert": round(r.ms_insert, 2),
+                          "model": round(r.ms_model, 2)},
+            "ms_wait": round(r.ms_wait, 2), "ms_final": round(r.ms_final, 2)}
+
+
+def leg_uniform_prm(sp3, o3, start, goal):
+    """The 7-DOF configuration as written: 200,000 uniform samples in
+    [-pi, pi]^7, every pair closer than 0.5 (L2) an edge candidate, start and
+    goal added; all decided by the exact checks, then the components."""
+    import torch
+
+    import gpuprm_amd
+
+    from scipy import sparse
+    from scipy.sparse import csgraph
+
+    rng = np.random.default_rng(SEED + 17)
+    N, D = 200000, 0.5
+    t0 = time.perf_counter()
+    Q = torch.from_numpy(rng.uniform(-np.pi, np.pi, size=(7, N)).astype(np.float32)).cuda()
+    codes = torch.zeros(N, dtype=torch.int32, device="cuda")
+    sp3.indicator_dev(Q, codes)
+    free = torch.nonzero(codes == 0).flatten()
+    V = torch.cat([Q[:, free], torch.from_numpy(np.stack([start, goal], axis=1)).cuda()], dim=1).contiguous()
+    n = V.shape[1]
+    torch.cuda.synchronize()
+    t_points = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    lo = np.full(7, -np.pi, np.float32)
+    qi, ni, _ = gpuprm_amd.radius_neighbours(V, V, lo, -lo, D, cap=1 << 24)
+    keep = qi < ni
+    I, J = qi[keep], ni[keep]
+    t_near = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Vh = V.cpu().numpy()
+    S, E = gpuprm_amd.canonical_edges(Vh[:, I].T, Vh[:, J].T)
+    res = torch.zeros(max(1, len(I)), dtype=torch.int32, device="cuda")
+    samples = 0
+    if len(I):
+        st = sp3.indicator2_dev(torch.from_numpy(S.T.copy()).cuda(), torch.from_numpy(E.T.copy()).cuda(), res)
+        samples = int(st.samples_total)
+    okE = res[:len(I)].cpu().numpy() == 0
+    t_edges = time.perf_counter() - t0
+    A = sparse.coo_matrix((np.ones(int(okE.sum())), (I[okE], J[okE])), shape=(n, n))
+    ncomp, lab = csgraph.connected_components(A, directed=False)
+    sizes = np.bincount(lab)
+    return {"workload": "7-DOF arm, 32 hulls of 16 points: 200000 uniform samples, radius 0.5 (L2)",
+            "samples": N, "free": int(free.numel()), "candidate_edges": int(len(I)), "free_edges": int(okE.sum()),
+            "edge_samples": samples, "components": int(ncomp), "largest_component": int(sizes.max()),
+            "start_goal_connected": bool(lab[n - 2] == lab[n - 1]),
+            "expected_neighbours_per_sample": round(float(len(I) * 2 / max(1, n)), 4),
+            "ms": {"points": 1000 * t_points, "near": 1000 * t_near, "edges": 1000 * t_edges}}
+
+
 def leg_prm():
-    """Time-to-path of the single-GPU planner: the 7-DOF scene (32 hulls of
-    16 points, radius 0.5), and plain vs surrogate on the 6-DOF edge-checking
-    scene with a model trained on 4M exact labels.  Every path is re-verified
-    by the oracle."""
+    """Time-to-path of the single-GPU planner on the 7-DOF scene (32 hulls of
+    16 points, radius 0.5, 8192 new nodes per step): the three tasks of the
+    earlier measurements and six more, one worker (steps in issue order) and
+    the reference's two-worker scheme; the configuration as written (200,000
+    uniform samples); and plain vs surrogate on the 6-DOF edge-checking scene
+    with a model trained on 4M exact labels, 16 tasks.  Every path is
+    re-verified by the oracle."""
     import torch
 
     import gpuprm_amd
@@ -336,18 +550,18 @@ def leg_prm():
     sp3 = gpuprm_amd.RobotConfigspace(d3, 0.01, 1 << 20)
     sp3.init(0, 1)
     o3 = OracleScene(d3)
-    runs = []
-    for i, (a, b) in enumerate(_tasks(o3, 7, 3, SEED + 7)):
-        r, path = sp3.prm_solve(a, b, num=8192, D=0.5, max_nodes=1 << 20, max_steps=400, seed=i)
-        runs.append({"found": r.found, "valid": _path_ok(o3, path), "ms": r.ms_total, "steps": r.steps,
-                     "nodes": r.nodes_left + r.nodes_right, "edge_samples": r.edge_samples,
-                     "point_checks": r.point_checks,
-                     # reference-equivalent checks (edge samples + point checks) per second of the whole run
-                     "checks_per_s": (r.edge_samples + r.point_checks) / max(1e-9, r.ms_total / 1000),
-                     "ms_phases": {"sampling": r.ms_sampling, "near": r.ms_near, "edges": r.ms_edges,
-                                   "host": r.ms_host}})
-    out["7dof_time_to_path"] = {"workload": "7-DOF arm, 32 hulls of 16 points, D=0.5, 8192 new nodes/step",
-                                "ms_median": float(np.median([x["ms"] for x in runs])), "runs": runs}
+    tasks = _tasks(o3, 7, 9, SEED + 7)
+    kw3 = dict(num=8192, D=0.5, max_nodes=1 << 20, max_steps=400)
+    runs = [_prm_run(sp3, o3, a, b, seed=i, workers=1, **kw3) for i, (a, b) in enumerate(tasks)]
+    runs2 = [_prm_run(sp3, o3, a, b, seed=i, workers=2, **kw3) for i, (a, b) in enumerate(tasks[:3])]
+    out["7dof_time_to_path"] = {
+        "workload": "7-DOF arm, 32 hulls of 16 points, D=0.5, 8192 new nodes/step",
+        "ms_median": float(np.median([x["ms"] for x in runs[:3]])),
+        "ms_median_9": float(np.median([x["ms"] for x in runs])),
+        "ms_median_workers2": float(np.median([x["ms"] for x in runs2])),
+        "all_paths_valid": all(x["valid"] for x in runs + runs2),
+        "runs": runs, "runs_workers2": runs2}
+    out["7dof_uniform_as_written"] = leg_uniform_prm(sp3, o3, *tasks[0])
 
     d2 = tempfile.mkdtemp(prefix="gprm_prm2_")
     gpuprm_amd.generate_scene(2, SEED, d2)
@@ -384,28 +598,42 @@ def leg_prm():
     torch.cuda.synchronize()
     t_pred = time.perf_counter() - t0
     flop = 2 * (16 * 256 + 2 * 256 * 256 + 256)
-    modes = {"plain": {}, "surrogate": {"order_edges": True, "bias_sampling": True, "edge_budget": 4}}
-    res = {k: [] for k in modes}
-    for i, (a, b) in enumerate(_tasks(o2, 6, 8, SEED + 13)):
-        for k, kw in modes.items():
-            r, path = sp2.prm_solve(a, b, num=1024, D=1.0, max_nodes=1 << 18, max_steps=300, seed=i,
-                                    surrogate=model if kw else None, **kw)
-            res[k].append({"found": r.found, "valid": _path_ok(o2, path), "ms": r.ms_total, "steps": r.steps,
-                           "edge_samples": r.edge_samples, "point_checks": r.point_checks,
-                           "ms_model": r.ms_model})
-    med = {k: float(np.median([x["ms"] for x in v])) for k, v in res.items()}
+    modes = {"plain": {},
+             "surrogate": {"surrogate": model, "order_edges": True, "bias_sampling": True, "edge_budget": 4},
+             "budget_shortest_no_model": {"edge_budget": 4}}
+    tasks2 = _tasks(o2, 6, 16, SEED + 13)
+    sweep = {}
+    for num in (1024, 4096):
+        res = {k: [] for k in modes}
+        for i, (a, b) in enumerate(tasks2):
+            for k, kw in modes.items():
+                res[k].append(_prm_run(sp2, o2, a, b, verify=(num == 1024), num=num, D=1.0, max_nodes=1 << 18,
+                                       max_steps=300, seed=i, **kw))
+        med = {k: float(np.median([x["ms"] for x in v])) for k, v in res.items()}
+        sweep[num] = {"ms_median": med, "speedup": med["plain"] / med["surrogate"],
+                      "speedup_budget_no_model": med["plain"] / med["budget_shortest_no_model"],
+                      "edge_samples_saved": 1 - sum(x["edge_samples"] for x in res["surrogate"]) /
+                      max(1, sum(x["edge_samples"] for x in res["plain"])),
+                      "point_checks_saved": 1 - sum(x["point_checks"] for x in res["surrogate"]) /
+                      max(1, sum(x["point_checks"] for x in res["plain"])),
+                      "all_found": all(x["found"] for v in res.values() for x in v),
+                      "all_paths_valid": all(x["valid"] for v in res.values() for x in v)
+                      if num == 1024 else None,
+                      "runs": res}
+    best = {k: min(sweep[n]["ms_median"][k] for n in sweep) for k in modes}
     out["surrogate_vs_plain"] = {
-        "workload": "6-DOF arm, 16 hulls of 12 points (edge-checking scene), D=1.0, 1024 new nodes/step, 8 tasks",
+        "workload": "6-DOF arm, 16 hulls of 12 points (edge-checking scene), D=1.0, 16 tasks, "
+                    "1024 and 4096 new nodes/step",
         "labels": N, "label_ms": 1000 * t_label, "train_epochs": epochs, "train_batch": B, "train_s": t_train,
         "train_tflops": epochs * (ntr // B) * B * 3 * flop / t_train / 1e12, "auc": auc,
         "predict_configs_per_s": N / t_pred, "predict_tflops": N * flop / t_pred / 1e12,
         "predict_mfma_frac": N * flop / t_pred / 2.5e15,
-        "ms_median": med, "speedup": med["plain"] / med["surrogate"],
-        "edge_samples_saved": 1 - sum(x["edge_samples"] for x in res["surrogate"]) /
-        max(1, sum(x["edge_samples"] for x in res["plain"])),
-        "point_checks_saved": 1 - sum(x["point_checks"] for x in res["surrogate"]) /
-        max(1, sum(x["point_checks"] for x in res["plain"])),
-        "all_paths_valid": all(x["valid"] for v in res.values() for x in v), "runs": res}
+        "ms_median": sweep[1024]["ms_median"], "speedup": sweep[1024]["speedup"],
+        "speedup_4096": sweep[4096]["speedup"],
+        "best_of_num_ms": best, "speedup_best_of_num": best["plain"] / best["surrogate"],
+        "edge_samples_saved": sweep[1024]["edge_samples_saved"],
+        "point_checks_saved": sweep[1024]["point_checks_saved"],
+        "all_paths_valid": sweep[1024]["all_paths_valid"], "by_num": sweep}
     return out
 
 
@@ -419,6 +647,8 @@ def main():
     ap.add_argument("--verify-edges", type=int, default=3000)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the point-check, multi-scene and planner legs")
+    ap.add_argument("--extras", default="points,multi,prm",
+                    help="comma-separated legs after the headline: points, multi, prm")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the timed steps, run a plain elementwise kernel that reads and writes "
                          "MIB MiB once each (the known-size kernel that scales HBM byte counters)")
@@ -495,16 +725,33 @@ def main():
     flops = prof.pairs_separated * fpp["separated"] + prof.pairs_colliding * fpp["colliding"]
     sva_s = prof.sva_ms / 1000.0
     achieved = flops / sva_s / 1e12 if sva_s > 0 else 0.0
+    side = profile_sidecar()
+    avg_launch_s = sva_s / max(prof.sva_launches, 1)
+    hw = None
+    if side:
+        # what the hardware executed, from the counters of the same sources
+        hw = {"valu_busy": side["valu_busy"], "lane_utilisation": side["lane_utilisation"],
+              "executed_fp32_tflops": round(side["fp32_ops_per_launch"] / avg_launch_s / 1e12, 3),
+              "executed_frac_of_nofma_valu": round(side["fp32_ops_per_launch"] / avg_launch_s / 1e12
+                                                   / VALU_NOFMA_TOPS, 4),
+              "hbm_tb_s": round(side["bytes_per_launch"] / avg_launch_s / 1e12, 3)}
     roofline = {
-        # compute-bound on the FP32 pipe; the test is scalar geometry, so the
-        # FP32 vector ALU (same 157.3 TFLOP/s published peak) does the work
-        "bound": "mfma",
+        # compute-bound on the FP32 vector pipe: scalar geometry without FMA
+        # (the reference's rounding), so VALU, not the matrix cores
+        "bound": "valu",
         "pipe": "fp32 valu",
         "achieved": round(achieved, 3),
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-        "traffic": TRAFFIC_BYTES_PER_LAUNCH,
+        # HBM bytes per launch measured by rocprofv3 on these very sources
+        # (profiles/sva_kernel_edges_wl2_traffic.md), else unknown
+        "traffic": side["bytes_per_launch"] if side else None,
+        "traffic_sources_sha16": sources_sha16(),
+        "hw": hw,
+        "frac_note": "achieved counts the reference algorithm's FP32 operations for the pair tests run "
+                     "(oracle counting build); the bound test decides most pairs in far fewer executed "
+                     "operations, so it can exceed what the VALU executes (hw.executed_fp32_tflops)",
         "kernel": "sva_kernel<1> (separating-vector, edge mode)",
         "launches_per_step": prof.sva_launches / args.steps,
         "avg_launch_ms": prof.sva_ms / max(prof.sva_launches, 1),
@@ -520,6 +767,7 @@ def main():
     if not args.no_cpu:
         log("cpu baseline (about %.0f s)" % args.cpu_seconds)
         cpu, csub, cres = cpu_baseline(scene_dir, QS, QE, testnum, args.cpu_seconds, rng)
+        cpu["host"] = host_cpu()
         mismatches += int(((gres[csub] != 0) != (cres != 0)).sum())
         log("cpu baseline %.3g checks/s (%s, %d threads)" % (cpu["value"], cpu["kind"], cpu["cores"]))
 
@@ -554,11 +802,14 @@ def main():
     }
     if cpu:
         line["speedup_vs_cpu"] = value / cpu["value"]
-    if not args.no_extras:
+    extras = [] if args.no_extras else [x.strip() for x in args.extras.split(",") if x.strip()]
+    if "points" in extras:
         log("point checks")
         line["points"] = leg_points(20)
+    if "multi" in extras:
         log("multi-scene batch")
         line["multi_scene"] = leg_multi_scene(10)
+    if "prm" in extras:
         log("planner")
         line["prm"] = leg_prm()
     print(json.dumps(line), flush=True)

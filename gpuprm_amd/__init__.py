@@ -653,15 +653,15 @@ class RobotConfigspace:
     # -- single-process PRM ----------------------------------------------
     def prm_solve(self, qstart, qgoal, num=2048, D=0.5, max_nodes=1 << 18, max_steps=200, seed=1,
                   nbuf=1 << 20, surrogate=None, bias_sampling=False, order_edges=False, edge_budget=0,
-                  path_cap=1 << 16, output_dir=None, workers=2):
+                  path_cap=1 << 16, output_dir=None, workers=1):
         """Bidirectional PRM from qstart to qgoal (the reference's PRMSolver
-        process, prm/src/lib/prm/prmsolver.cpp:1973-2187, on one GPU).
+        process, prm/src/lib/prm/prmsolver.cpp:1765-2187, on one GPU).
         edge_budget=0 checks every candidate edge, as the reference does; k > 0
-        checks at most k per (new node, tree), ranked by the surrogate when
-        order_edges is set, else shortest first.  output_dir: write both trees
-        in the reference's output layout (see read_prm_output).  workers: steps
-        in flight (2 as the reference's process 5, or 1).  Returns
-        (PrmResult, path (n, ndof) or None)."""
+        checks at most k per (new node, tree), shortest first (with
+        order_edges, the edges the surrogate predicts blocked last).
+        output_dir: write both trees in the reference's output layout (see
+        read_prm_output).  workers: steps in flight (1, or 2 as the
+        reference's process 5).  Returns (PrmResult, path (n, ndof) or None)."""
         p = PrmParams(int(num), float(D), int(max_nodes), int(max_steps), int(seed), int(nbuf),
                       C.c_void_p(surrogate.handle if surrogate is not None else None),
                       int(bool(bias_sampling)), int(bool(order_edges)), int(edge_budget),

@@ -60,6 +60,7 @@ struct Slot {
   DevBuf frames, status, item_edge, item_pos, counters, qs, qe, res, testnum, stride, misc, tally, hist;
   DevBuf frames_odd, item_edge_odd, item_pos_odd;  // buffer set of the odd edge launches
   int maxT_hint = 0;  // largest testnum of the slot's earlier edge calls (sizes buffers ahead)
+  int* small_out = nullptr;  // pinned codes of small point-check calls (kSmallMax ints)
   int* host_res = nullptr;
   int N = 0;
   bool busy = false;
@@ -79,6 +80,10 @@ struct Slot {
     if (prep) (void)hipStreamDestroy(prep);
     if (ev_prologue) (void)hipEventDestroy(ev_prologue);
     if (pinned) (void)hipHostFree(pinned);
+    if (small_out) {
+      if (stream) (void)hipStreamSynchronize(stream);
+      (void)hipHostFree(small_out);
+    }
     if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -122,6 +127,7 @@ struct gprm_ctx {
   int nbuf = 0;
   int device = -1;
   bool inited = false;
+  bool small_calls = false;  // the scene fits the small point-check kernel's LDS
   gprm::DevBuf img_buf;
   gprm::SceneImg img{};
   gprm::SvaShape shape;

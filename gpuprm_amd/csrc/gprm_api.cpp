@@ -766,6 +766,7 @@ int gprm_init(gprm_ctx* c, int rank, int size) {
   c->img.blob = c->img_buf.as<int4>();
   c->shape = sva_default_shape(c->img);
   GPRM_HIP(sva_shape_finish(c->img, &c->shape, c->device), "separating-vector launch shape");
+  c->small_calls = point_small_prepare(c->img) == hipSuccess;
   c->inited = true;
   return GPRM_OK;
 }
@@ -789,6 +790,21 @@ int gprm_indicator(gprm_ctx* c, const float* q, int* res, int N, int offset) {
   Slot& sl = *c->slots[si];
   struct Release { Slot& s; ~Release() { s.busy = false; } } rel{sl};
   hipStream_t st = sl.stream;
+  if (N <= kSmallMax && c->small_calls && !c->profile) {
+    // one launch with the configurations as arguments, codes written to
+    // pinned memory: the reference's per-candidate indicator(q) calls
+    // (prmsolver.cpp:2611) pay one round trip
+    if (!sl.small_out)
+      GPRM_HIP(hipHostMalloc((void**)&sl.small_out, sizeof(int) * kSmallMax, hipHostMallocDefault), "hipHostMalloc");
+    SmallPoints sp{};
+    sp.n = N;
+    for (int k = 0; k < N; ++k)
+      for (int i = 0; i < c->scene.ndof; ++i) sp.q[k][i] = q[k + (long long)offset * i];
+    GPRM_HIP(launch_point_small(c->img, c->rb, sp, sl.small_out, st), "point-check launch");
+    GPRM_HIP(hipStreamSynchronize(st), "sync");
+    std::memcpy(res, sl.small_out, sizeof(int) * N);
+    return GPRM_OK;
+  }
   if ((rc = prof_prepare(c, sl, st))) return rc;
   if ((rc = upload_soa(c, sl.qs, q, N, offset, st))) return rc;
   GPRM_HIP(sl.res.ensure(sizeof(int) * (size_t)N), "hipMalloc res");

@@ -1274,6 +1274,108 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVE
   if (tally != nullptr) add_tally<BLOCK>(c, smem, tally);
 }
 
+// ---- small point-check calls ------------------------------------------------
+// One block per configuration of a call of at most kSmallMax: the
+// configurations come as kernel arguments and the codes go straight to
+// (pinned host) memory, so a call is one launch and one synchronisation.
+// Thread 0 computes the frames into LDS; the threads then run the full
+// separating-vector test on the pairs t, t + BLOCK, ... (no bound test: it
+// only shortcuts pair tests that end separated), skipping pairs later in
+// CRS order than a collision already found, and the block keeps the first
+// colliding pair's code, as the batched kernels do.
+
+// Frames 1..ndof of one configuration (kinematics.h:27-37), the same
+// operations as fk_store.
+__device__ void fk_frames(const DevRobot& rb, const float* q, T12* out) {
+  T12 T;
+  {
+    const float cq = rb.c0, sq = rb.s0, ca = rb.c0, sa = rb.s0, a = 0.0f, d = 0.0f;
+    T.m[0] = cq;      T.m[3] = -sq;     T.m[6] = 0.0f;  T.m[9] = a;
+    T.m[1] = sq * ca; T.m[4] = cq * ca; T.m[7] = -sa;   T.m[10] = -sa * d;
+    T.m[2] = sq * sa; T.m[5] = cq * sa; T.m[8] = ca;    T.m[11] = ca * d;
+  }
+  for (int i = 0; i < rb.ndof; ++i) {
+    float qset = rb.q[i], dset = rb.d[i];
+    if (rb.types[i] == 0) qset = q[i];
+    else dset = q[i];
+    const T12 A = t_set_cs(rb.a[i], rb.ca[i], rb.sa[i], qset, dset);
+    T = t_lapply(A, T);
+    out[i] = T;
+  }
+}
+
+// The full separating-vector test of one pair (collision4.h:227-332), as
+// sva_drain runs it per lane; H: this lane's iteration history, stride hs.
+__device__ int sva_pair_full(const Lds& L, const T12& tp, const T12& tq, bool sysP0, bool sysQ0, int vbP, int vbQ,
+                             float4* H, size_t hs) {
+  const V3 vq0 = v3(L.verts[vbQ]), vp0 = v3(L.verts[vbP]);
+  const V3 Pq = sysQ0 ? vq0 : t_apply(tq, vq0);
+  const V3 Pp = sysP0 ? vp0 : t_apply(tp, vp0);
+  V3 S{Pq.x - Pp.x, Pq.y - Pp.y, Pq.z - Pp.z};
+  const float norm = normalize3(S);
+  if (norm <= kEpsBelow) S.x = 1.0f;
+  int p = vbP, q = vbQ, k = 0;
+  V3 w{0.0f, 0.0f, 0.0f};
+  for (int l = 0;;) {
+    const V3 Sp = sysP0 ? S : t_apply_rot_inv(tp, S);
+    const V3 Sn{-S.x, -S.y, -S.z};
+    const V3 Sq = sysQ0 ? Sn : t_apply_rot_inv(tq, Sn);
+    p = climb(L, p, Sp);
+    q = climb(L, q, Sq);
+    const V3 vp = v3(L.verts[p]), vq = v3(L.verts[q]);
+    const V3 p0 = sysP0 ? vp : t_apply(tp, vp);
+    const V3 q0 = sysQ0 ? vq : t_apply(tq, vq);
+    V3 r{q0.x - p0.x, q0.y - p0.y, q0.z - p0.z};
+    normalize3(r);
+    const float dp = dot3(S, r);
+    if (dp >= -kEpsBelow) return 0;
+    const int code = sva_slow(H, hs, k, l, (p - vbP) | ((q - vbQ) << 16), r, dp, S, w);
+    if (code != 0) return code;
+    if (++l >= kMaxIter) return -1;
+  }
+}
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) point_small_kernel(SceneImg img, DevRobot rb, SmallPoints qs, int* out) {
+  extern __shared__ int4 smem[];
+  // dynamic LDS: the scene image, the frames, the lanes' iteration history
+  const Lds L = stage_scene<BLOCK>(img, smem);
+  T12* frames = reinterpret_cast<T12*>(reinterpret_cast<char*>(smem) + img.bytes);
+  float4* H = reinterpret_cast<float4*>(reinterpret_cast<char*>(smem) + img.bytes + sizeof(T12) * kMaxDofDev) +
+              threadIdx.x;
+  __shared__ int s_word;
+  const float* q = qs.q[blockIdx.x];
+  if (threadIdx.x == 0) {
+    // check_boundaries(q) (robotconfigspace.cpp:795-802)
+    int ob = 0;
+    for (int j = 0; j < rb.ndof; ++j)
+      if (q[j] < rb.mins[j] || q[j] > rb.maxs[j]) { ob = 1; break; }
+    s_word = ob ? kWordOutOfBounds : kWordFree;
+    if (!ob) fk_frames(rb, q, frames);
+  }
+  __syncthreads();
+  if (s_word != kWordOutOfBounds) {
+    for (int pair = threadIdx.x; pair < img.M; pair += BLOCK) {
+      const int4 pd = L.pairs[pair];
+      const int crs = pd.z;
+      if ((__hip_atomic_load(&s_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >> 4) <= crs) continue;
+      const int sP = pd.y & 255, sQ = pd.y >> 8;
+      T12 tp, tq;
+#pragma unroll
+      for (int i = 0; i < 12; ++i) tp.m[i] = tq.m[i] = 0.0f;
+      if (sP != 0) tp = frames[sP - 1];
+      if (sQ != 0) tq = frames[sQ - 1];
+      const int code = sva_pair_full(L, tp, tq, sP == 0, sQ == 0, pd.x & 0xffff, pd.x >> 16, H, BLOCK);
+      if (code != 0) atomicMin(&s_word, ((crs + 1) << 4) | (code + 1));
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int w = s_word;
+    out[blockIdx.x] = w == kWordFree ? 0 : (w & 15) - 1;
+  }
+}
+
 // Kinematics of the configurations of many scenes in one launch; block b
 // belongs to the task with the largest block0 <= b.
 template <int NDOF>
@@ -1455,6 +1557,37 @@ hipError_t launch_sva(const SvaArgs& a, int edge_mode, const SvaShape& sh, hipSt
   if (!fn) return hipErrorInvalidValue;
   void* args[] = {const_cast<SvaArgs*>(&a)};
   return hipLaunchKernel(fn, dim3(sh.grid), dim3(sh.block), args, lds, st);
+}
+
+// LDS of a small point-check block: the image, the frames and BLOCK lanes'
+// iteration history.
+static size_t point_small_lds(const SceneImg& img, int block) {
+  return (size_t)img.bytes + sizeof(T12) * kMaxDofDev + sizeof(float4) * kMaxIter * (size_t)block;
+}
+
+hipError_t point_small_prepare(const SceneImg& img) {
+  for (int block : {256, 64}) {
+    const size_t lds = point_small_lds(img, block);
+    if (lds > (size_t)sva_max_lds_bytes()) continue;
+    const void* fn = block == 256 ? reinterpret_cast<const void*>(&point_small_kernel<256>)
+                                  : reinterpret_cast<const void*>(&point_small_kernel<64>);
+    if (lds > 64 * 1024) return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    return hipSuccess;
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_point_small(const SceneImg& img, const DevRobot& rb, const SmallPoints& qs, int* out,
+                              hipStream_t st) {
+  if (qs.n <= 0) return hipSuccess;
+  if (qs.n > kSmallMax) return hipErrorInvalidValue;
+  if (point_small_lds(img, 256) <= (size_t)sva_max_lds_bytes())
+    hipLaunchKernelGGL(point_small_kernel<256>, dim3(qs.n), dim3(256), point_small_lds(img, 256), st, img, rb, qs, out);
+  else if (point_small_lds(img, 64) <= (size_t)sva_max_lds_bytes())
+    hipLaunchKernelGGL(point_small_kernel<64>, dim3(qs.n), dim3(64), point_small_lds(img, 64), st, img, rb, qs, out);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
 }
 
 hipError_t launch_sva_multi(const SceneTask* d_tasks, int ntasks, int edge_mode, const SvaShape& sh, size_t img_bytes,

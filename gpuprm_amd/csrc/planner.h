@@ -117,8 +117,10 @@ hipError_t launch_plan_score_points(int ndof, int E, int nbuf, int budget, int S
                                     const int* sc_off, const int* egroup, const int* enode, const float* qnew, int num,
                                     PlanTree t0, PlanTree t1, int cap, float* out, long long ns, hipStream_t st);
 // Edge budget: in every group with more than B edges keep the B lowest by
-// (score, squared length, index), score = max of the edge's S8 predictions
-// (prob may be null: 0); skip the others.  state edges_skipped.
+// (score > 1/2, squared length, index), score = max of the edge's S8
+// predictions (prob may be null: score 0): the shortest edges the model
+// predicts free first, then the shortest predicted blocked; skip the
+// others.  state edges_skipped.
 hipError_t launch_plan_rank(int E, int nbuf, int budget, int S8, const int* cnt, const int* off, const int* sc_off,
                             const int* egroup, const float* ed2, const float* prob, float* score,
                             unsigned char* skip, PlanState* state, hipStream_t st);

@@ -444,12 +444,16 @@ __global__ void __launch_bounds__(256)
     const int b = egroup[e];
     const int w = stored_edges(off, cnt, b, nbuf);
     if (w > budget) {
+      // key: (predicted blocked, squared length, index); predicted blocked
+      // means a score above one half
       const int o = off[b];
-      const float se = score[e], de = ed2[e];
+      const bool be = score[e] > 0.5f;
+      const float de = ed2[e];
       int rank = 0;
       for (int x = o; x < o + w; ++x) {
-        const float sx = score[x], dx = ed2[x];
-        rank += (sx < se) || (sx == se && (dx < de || (dx == de && x < e)));
+        const bool bx = score[x] > 0.5f;
+        const float dx = ed2[x];
+        rank += (bx < be) || (bx == be && (dx < de || (dx == de && x < e)));
       }
       sk = rank >= budget;
     }

@@ -57,7 +57,9 @@ double ms_since(Clock::time_point t) {
 }
 
 // Points scored along an edge by the model: interior points at t = s/(S+1).
-constexpr int kScorePoints = 8;
+// Three points (t = 1/4, 1/2, 3/4) find the edges the model predicts blocked
+// at 3/8 of the inference of eight.
+constexpr int kScorePoints = 3;
 // Undirected edges a tree may store, per node of its capacity.
 constexpr long long kEdgesPerNode = 16;
 

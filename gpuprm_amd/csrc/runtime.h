@@ -200,6 +200,17 @@ hipError_t launch_edge_compact(EdgeArgs ea, int e0, int e1, int lo, int hi, int*
 // with item stride cap.
 hipError_t launch_edge_fk(const DevRobot& rb, EdgeArgs ea, const int* item_edge, const int* item_pos,
                           const int* nitems, int cap, float* frames, hipStream_t st);  // cap: plane stride
+// Small point-check calls: up to kSmallMax configurations passed by value,
+// one block each, codes written to out (device-visible, e.g. pinned host
+// memory).  point_small_prepare sets the kernels' LDS limit once per scene.
+constexpr int kSmallMax = 8;
+struct SmallPoints {
+  int n;
+  float q[kSmallMax][kMaxDofDev];
+};
+hipError_t point_small_prepare(const SceneImg& img);
+hipError_t launch_point_small(const SceneImg& img, const DevRobot& rb, const SmallPoints& qs, int* out,
+                              hipStream_t st);
 hipError_t launch_fill_i32(int* p, int v, long long n, hipStream_t st);
 CellGrid make_cell_grid(int ndof, const float* mins, const float* maxs, float D);
 // Counting sort of n nodes (SoA, stride noff) into the cell index: counts and

@@ -167,7 +167,8 @@ int gprm_profile_read(gprm_ctx* ctx, gprm_profile_stats* out);
  * above.  Steps run on the device; `workers` steps are in flight at once
  * (process 5 keeps two: one worker's edge checks run while the other
  * worker's nodes are inserted and its next step is sampled), each read back
- * once.  On success res->found = 1 and path (AoS, ndof floats per node, up
+ * once.  Its edges are checked with the lexicographically smaller endpoint
+ * first.  On success res->found = 1 and path (AoS, ndof floats per node, up
  * to path_cap nodes) runs from qstart to qgoal. */
 typedef struct {
   int num;                  /* new nodes per step (the reference's --num), at most 16384 */
@@ -179,16 +180,17 @@ typedef struct {
   void* surrogate;          /* optional model (gprm_surrogate_*), or NULL */
   int bias_sampling;        /* with a model: every second pair of candidates must be
                                predicted free (p < 0.5); the rest are dropped unchecked */
-  int order_edges;          /* with a model and edge_budget: rank a node's candidate edges
-                               by the model (max probability at 8 interior points) */
+  int order_edges;          /* with a model and edge_budget: edges the model predicts
+                               blocked (max probability at 3 interior points above 1/2)
+                               rank after the others */
   int edge_budget;          /* 0: check every candidate edge (the reference's rule);
-                               k > 0: at most k per (new node, tree), lowest rank first
-                               (by the model with order_edges, else shortest first) */
+                               k > 0: at most k per (new node, tree), shortest first
+                               (among predicted-free edges first with order_edges) */
   const char* output_dir;   /* NULL, or a directory that receives both trees and the
                                path in the reference's output layout (store_results,
                                prmsolver.cpp:2947-3055; prm/matlab/lib/prmoutputread.m) */
   int workers;              /* steps in flight: 1, or 2 as the reference's process 5
-                               (prmsolver.cpp:1830-1861); 0 = 2 */
+                               (prmsolver.cpp:1830-1861); 0 = 1 */
 } gprm_prm_params;
 typedef struct {
   int found, steps, path_len;
