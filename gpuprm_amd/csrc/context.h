@@ -93,6 +93,7 @@ struct Slot {
 
 struct gprm_ctx;
 namespace gprm {
+struct PlanCache;  // the planner's device state between solves (prm.cpp)
 // Pairs per work unit of a separating-vector launch over about n items, and
 // the context's launch shape cut to what a launch of `units` work units can
 // use (gprm_api.cpp).
@@ -139,4 +140,5 @@ struct gprm_ctx {
   gprm_profile_stats prof{};
   std::string err;
   std::mutex mu;
+  std::shared_ptr<gprm::PlanCache> plan_cache;
 };

@@ -222,10 +222,31 @@ struct Worker {
   Readback* pin = nullptr;
   hipEvent_t ev_rb = nullptr;
   int E = 0, S = 0, batch = 0;
-  hipEvent_t ev_mark[4] = {};  // start of part 1, after selection, after the radius search, start of part 2
+  ~Worker() {
+    if (ev_rb) (void)hipEventDestroy(ev_rb);
+    if (pin) (void)hipHostFree(pin);
+  }
 };
 
 }  // namespace
+
+// Device state of the planner kept by a context between solves, so that a
+// solve allocates only when it needs more than an earlier one did (the
+// 6-DOF surrogate comparison's solves take about 10 ms).
+struct PlanCache {
+  DevArr state;
+  DevTree dt[2];
+  std::vector<std::unique_ptr<Worker>> workers;
+  DevArr dist, parent, flags, chain, lens, pcoords;  // path extraction
+  std::vector<hipEvent_t> timing;                    // interval events, reused
+  hipEvent_t ev_chain = nullptr;
+  ~PlanCache() {
+    (void)hipDeviceSynchronize();
+    for (hipEvent_t e : timing) (void)hipEventDestroy(e);
+    if (ev_chain) (void)hipEventDestroy(ev_chain);
+  }
+};
+
 }  // namespace gprm
 
 using namespace gprm;
@@ -279,9 +300,11 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
   const long long ecap_ll = std::min<long long>(kEdgesPerNode * cap, (1LL << 31) - 1);
   const int ecap = (int)ecap_ll;
 
-  // ---- device state -----------------------------------------------------------
-  DevArr d_state;
-  DevTree dt[2];
+  // ---- device state (kept by the context between solves) ------------------------
+  if (!ctx->plan_cache) ctx->plan_cache = std::make_shared<PlanCache>();
+  PlanCache& pc = *ctx->plan_cache;
+  DevArr& d_state = pc.state;
+  DevTree* dt = pc.dt;
   PlanTree T[2];
   if (!d_state.ensure(sizeof(PlanState))) return GPRM_E_HIP;
   PlanState* state = d_state.as<PlanState>();
@@ -316,29 +339,33 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
   }
 
   // ---- workers ------------------------------------------------------------------
-  std::vector<std::unique_ptr<Worker>> wk;
+  std::vector<std::unique_ptr<Worker>>& wk = pc.workers;
+  while ((int)wk.size() < W) wk.emplace_back(new Worker());
+  // the slots are held for the solve and released, with their streams
+  // drained, however it ends
   struct Slots {
     gprm_ctx* c;
     std::vector<std::unique_ptr<Worker>>* w;
+    int W;
     ~Slots() {
-      for (auto& x : *w) {
-        if (x->st) (void)hipStreamSynchronize(x->st);
-        if (x->ev_rb) (void)hipEventDestroy(x->ev_rb);
-        for (hipEvent_t e : x->ev_mark)
-          if (e) (void)hipEventDestroy(e);
-        if (x->pin) (void)hipHostFree(x->pin);
-        if (x->slot >= 0) slot_release(c, x->slot);
+      for (int i = 0; i < W; ++i) {
+        Worker& x = *(*w)[i];
+        if (x.st) (void)hipStreamSynchronize(x.st);
+        if (x.slot >= 0) slot_release(c, x.slot);
+        x.slot = -1;
+        x.sl = nullptr;
+        x.st = nullptr;
       }
     }
-  } slots_guard{ctx, &wk};
+  } slots_guard{ctx, &wk, W};
   for (int w = 0; w < W; ++w) {
-    wk.emplace_back(new Worker());
-    Worker& x = *wk.back();
+    Worker& x = *wk[w];
+    x.E = x.S = x.batch = 0;
     if ((rc = slot_acquire(ctx, &x.slot)) != GPRM_OK) return rc;
     x.sl = ctx->slots[x.slot].get();
     x.st = x.sl->stream;
-    if (hipHostMalloc((void**)&x.pin, sizeof(Readback), hipHostMallocDefault) != hipSuccess ||
-        hipEventCreateWithFlags(&x.ev_rb, hipEventDisableTiming) != hipSuccess)
+    if ((!x.pin && hipHostMalloc((void**)&x.pin, sizeof(Readback), hipHostMallocDefault) != hipSuccess) ||
+        (!x.ev_rb && hipEventCreateWithFlags(&x.ev_rb, hipEventDisableTiming) != hipSuccess))
       return GPRM_E_HIP;
     if (!x.cand.ensure(sizeof(float) * nd * (size_t)batch_max) || !x.codes.ensure(sizeof(int) * (size_t)batch_max) ||
         !x.qnew.ensure(sizeof(float) * nd * (size_t)num) || !x.info.ensure(sizeof(StepInfo)) ||
@@ -375,35 +402,23 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
     // the host copies above read host memory when the stream runs them
     if (hipStreamSynchronize(st) != hipSuccess) return GPRM_E_HIP;
   }
-  hipEvent_t ev_chain = nullptr;
-  if (hipEventCreateWithFlags(&ev_chain, hipEventDisableTiming) != hipSuccess) return GPRM_E_HIP;
-  struct EvGuard {
-    hipEvent_t e;
-    ~EvGuard() { if (e) (void)hipEventDestroy(e); }
-  } ev_chain_guard{ev_chain};
+  if (!pc.ev_chain && hipEventCreateWithFlags(&pc.ev_chain, hipEventDisableTiming) != hipSuccess) return GPRM_E_HIP;
+  hipEvent_t ev_chain = pc.ev_chain;
   if (hipEventRecord(ev_chain, wk[0]->st) != hipSuccess) return GPRM_E_HIP;
 
-  // timing: pairs of events per interval, folded at the end
+  // timing: pairs of events per interval (from the context's pool), folded
+  // at the end
   std::vector<Interval> timed;
-  struct IvGuard {
-    std::vector<Interval>* v;
-    ~IvGuard() {
-      for (auto& x : *v) {
-        (void)hipEventDestroy(x.a);
-        (void)hipEventDestroy(x.b);
-      }
-    }
-  } iv_guard{&timed};
   // begin_iv returns the interval's index (-1 on failure), end_iv closes it
   auto begin_iv = [&](hipStream_t st, double* total) -> int {
-    Interval iv{nullptr, nullptr, total};
-    if (hipEventCreate(&iv.a) != hipSuccess) return -1;
-    if (hipEventCreate(&iv.b) != hipSuccess) {
-      (void)hipEventDestroy(iv.a);
-      return -1;
+    const size_t k = 2 * timed.size();
+    while (pc.timing.size() < k + 2) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreate(&e) != hipSuccess) return -1;
+      pc.timing.push_back(e);
     }
-    timed.push_back(iv);
-    return hipEventRecord(iv.a, st) == hipSuccess ? (int)timed.size() - 1 : -1;
+    timed.push_back(Interval{pc.timing[k], pc.timing[k + 1], total});
+    return hipEventRecord(timed.back().a, st) == hipSuccess ? (int)timed.size() - 1 : -1;
   };
   auto end_iv = [&](int i, hipStream_t st) -> bool { return i >= 0 && hipEventRecord(timed[i].b, st) == hipSuccess; };
 
@@ -599,7 +614,8 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
     const int max_edges = (int)std::min<unsigned>(std::max(fin.ecount[0], fin.ecount[1]), (unsigned)ecap);
     const int maxlen = std::min(cap, 1 << 16);
     constexpr int kRounds = 8;
-    DevArr dist, parent, flags, chain, lens, pcoords;
+    DevArr &dist = pc.dist, &parent = pc.parent, &flags = pc.flags, &chain = pc.chain, &lens = pc.lens,
+           &pcoords = pc.pcoords;
     hipStream_t st = wk[0]->st;
     if (!dist.ensure(sizeof(int) * 2 * (size_t)cap) || !parent.ensure(sizeof(int) * 2 * (size_t)cap) ||
         !flags.ensure(sizeof(int) * 2 * kRounds) || !chain.ensure(sizeof(int) * 2 * (size_t)maxlen) ||
