@@ -464,26 +464,35 @@ class Surrogate:
         assert pa.size == lib().gprm_surrogate_param_count()
         _check(lib().gprm_surrogate_set_params(self._h, pa.ctypes.data_as(_FP)))
 
-    def gradients(self, q, codes):
-        """q: float32 device tensor (ndof, n); codes: int32 device tensor (n,). Returns (grads, loss)."""
-        n = int(q.shape[1])
+    def gradients(self, q, codes, offset: Optional[int] = None):
+        """q: float32 device tensor (ndof, >= n), component i of row k at
+        q[k + offset*i] (offset defaults to q.shape[1]); codes: int32 device
+        tensor (n,).  Returns (grads, loss)."""
+        n = int(codes.shape[0])
         g = np.zeros(lib().gprm_surrogate_param_count(), np.float32)
         loss = C.c_float(0)
-        _check(lib().gprm_surrogate_gradients(self._h, C.c_void_p(q.data_ptr()), n, C.c_void_p(codes.data_ptr()), n,
-                                              g.ctypes.data_as(_FP), C.byref(loss)))
+        _check(lib().gprm_surrogate_gradients(self._h, C.c_void_p(q.data_ptr()), int(offset or q.shape[1]),
+                                              C.c_void_p(codes.data_ptr()), n, g.ctypes.data_as(_FP),
+                                              C.byref(loss)))
         return g, loss.value
 
-    def train_step(self, q, codes, lr: float = 1e-3, offset: Optional[int] = None) -> float:
-        n = int(q.shape[1]) if offset is None else int(q.shape[-1])
+    def train_step(self, q, codes, lr: float = 1e-3, offset: Optional[int] = None, want_loss: bool = True):
+        """One Adam step on the n = len(codes) rows of q; returns the batch
+        loss, or None with want_loss=False (no read-back: the step stays
+        queued on the device)."""
+        n = int(codes.shape[0])
         loss = C.c_float(0)
-        _check(lib().gprm_surrogate_train_step(self._h, C.c_void_p(q.data_ptr()), int(offset or n),
-                                               C.c_void_p(codes.data_ptr()), n, float(lr), C.byref(loss)))
-        return loss.value
+        _check(lib().gprm_surrogate_train_step(self._h, C.c_void_p(q.data_ptr()), int(offset or q.shape[1]),
+                                               C.c_void_p(codes.data_ptr()), n, float(lr),
+                                               C.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
 
-    def predict(self, q, out, stream=None) -> None:
-        n = int(q.shape[1])
-        _check(lib().gprm_surrogate_predict(self._h, C.c_void_p(q.data_ptr()), n, n, C.c_void_p(out.data_ptr()),
-                                            C.c_void_p(stream)))
+    def predict(self, q, out, stream=None, offset: Optional[int] = None) -> None:
+        """Probabilities of the n = len(out) rows of q (SoA, offset defaults
+        to q.shape[1]) into the float32 device tensor out."""
+        n = int(out.shape[0])
+        _check(lib().gprm_surrogate_predict(self._h, C.c_void_p(q.data_ptr()), n, int(offset or q.shape[1]),
+                                            C.c_void_p(out.data_ptr()), C.c_void_p(stream)))
 
 
 class RobotConfigspace:

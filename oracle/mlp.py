@@ -7,8 +7,8 @@ Parameters use the kernels' flat order: W1 [256 x 16] (input columns beyond
 ndof zero), b1 [256], W2 [256 x 256], b2, W3, b3, w4 [256], b4 [1].
 
 mode "bf16": values are rounded to bfloat16 where the kernels round them
-(inputs, weight copies, hidden activations, activation gradients), products
-accumulate in float32.  mode "f64": plain float64 mathematics, used to pin the
+(inputs, weight copies, hidden activations, the SiLU derivatives kept for
+the backward pass, activation gradients), products accumulate in float32.  mode "f64": plain float64 mathematics, used to pin the
 backward formulas by central finite differences.
 """
 from __future__ import annotations
@@ -92,13 +92,14 @@ def gradients(p, X, y, mode="bf16"):
     dz = ((pr - y) / n).astype(dt)
     gw4 = c["h3"].T @ dz
     gb4 = np.array([dz.sum()], dt)
-    dA3 = r(dz[:, None] * w4.astype(dt)[None, :] * _dsilu(c["a3"])).astype(dt)
+    s3, s2, s1 = (r(_dsilu(c["a%d" % l])).astype(dt) for l in (3, 2, 1))
+    dA3 = r(dz[:, None] * w4.astype(dt)[None, :] * s3).astype(dt)
     gW3 = dA3.T @ c["h2"]
     gb3 = dA3.sum(0)
-    dA2 = r((dA3 @ c["W3"]) * _dsilu(c["a2"])).astype(dt)
+    dA2 = r((dA3 @ c["W3"]) * s2).astype(dt)
     gW2 = dA2.T @ c["h1"]
     gb2 = dA2.sum(0)
-    dA1 = r((dA2 @ c["W2"]) * _dsilu(c["a1"])).astype(dt)
+    dA1 = r((dA2 @ c["W2"]) * s1).astype(dt)
     gW1 = dA1.T @ c["X"]
     gb1 = dA1.sum(0)
     z = c["z"]
