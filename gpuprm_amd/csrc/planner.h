@@ -75,29 +75,23 @@ hipError_t launch_plan_sample(int ndof, int batch, PlanTree t0, PlanTree t1, int
 hipError_t launch_plan_bias_mask(const float* prob, int batch, unsigned char* skip, hipStream_t st);
 // New nodes: in slot order, the free (code 0), unskipped candidates, at most
 // want0 of tree 0 and want1 of tree 1, into qnew (SoA, stride num);
-// info->nnew, info->accepted (info->E_all reset); state point_checks /
+// info->nnew, info->accepted; state point_checks /
 // dropped.  tile: 3 ints per 1024 candidates.
 hipError_t launch_plan_select(int ndof, int batch, const int* codes, const unsigned char* skip, const float* cand,
                               int want0, int want1, int num, float* qnew, int* tile, StepInfo* info, PlanState* state,
                               hipStream_t st);
 // Radius search of the step's new nodes in both trees, one group b = 2j + g
 // per (new node j, tree g): every (node, squared distance, group) with
-// distance^2 < D2 (the reference's get_near_vertices float sum) takes a slot
-// from info->E_all; the first nbuf slots are stored, in no fixed order;
-// cnt[b] = the group's stored edges.
+// distance^2 < D2 (the reference's get_near_vertices float sum), in group
+// order, group b's at off[b]; the first nbuf of them in that order are
+// stored (enode, ed2, egroup) and cnt[b] = the group's stored edges.
+// info->E_all = all matches, info->E = min(E_all, nbuf).  With budget B > 0
+// also sc_off = exclusive scan of the stored edges of the groups with more
+// than B (the groups the model scores) and info->S their total.  off and
+// sc_off have groups + 1 entries.
 hipError_t launch_plan_near(int ndof, int num, const float* qnew, StepInfo* info, PlanTree t0, PlanTree t1, int cap,
-                            const CellGrid& g, float D2, int* cnt, int nbuf, int* enode, float* ed2, int* egroup,
-                            hipStream_t st);
-// info->E = min(E_all, nbuf).  With budget B > 0 also the group order: off =
-// exclusive scan of cnt (groups + 1 entries), sc_off the same over the groups
-// with more than B edges (scored by the model), info->S their total; fill
-// zeroed for the scatter.
-hipError_t launch_plan_scan_groups(int groups, const int* cnt, int nbuf, int budget, int* off, int* sc_off,
-                                   int* fill, StepInfo* info, hipStream_t st);
-// The radius search's edges (gs, ns, ds) moved into group order (egroup,
-// enode, ed2), group b at off[b].
-hipError_t launch_plan_group_scatter(int nbuf, const StepInfo* info, const int* gs, const int* ns, const float* ds,
-                                     const int* off, int* fill, int* egroup, int* enode, float* ed2, hipStream_t st);
+                            const CellGrid& g, float D2, int* cnt, int* off, int* sc_off, int nbuf, int budget,
+                            int* enode, float* ed2, int* egroup, hipStream_t st);
 // Shortest paths from node 0 in both trees (dist, parent: 2 x cap ints):
 // init, then relaxation rounds over the edges (max_edges >= both trees'
 // edge counts) that set changed[tree] when they lower a distance, then

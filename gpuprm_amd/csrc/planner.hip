@@ -210,7 +210,6 @@ __global__ void __launch_bounds__(kScanBlock)
   if (threadIdx.x == 0) {
     info->nnew = min(tot[0], want0) + min(tot[1], want1);
     info->accepted = tot[0] + tot[1];
-    info->E_all = 0;  // the radius search's edge counter
     atomicAdd(&state->point_checks, (unsigned long long)(batch - tot[2]));
     atomicAdd(&state->dropped, (unsigned long long)tot[2]);
   }
@@ -242,23 +241,30 @@ constexpr int kNeighbourCells = pow3(kGridDims);
 static_assert(kNeighbourCells <= 256, "one thread per neighbour cell in a 256-thread block");
 
 // One block per group b = 2 j + g: the nonempty cells among the 3^dims
-// around new node j are compacted (block scans), the block strides through
-// their nodes, and matches take slots from the step's edge counter
-// (info->E_all, one atomic per wave); slots beyond nbuf are dropped.  cnt[b]
-// gets the group's stored edges.
+// around new node j are compacted (block scans) and the block strides
+// through their nodes in cell order.  Two passes, no atomics on a shared
+// counter: COUNT stores the group's matches in cnt[b]; after the scan
+// (plan_scan_groups_kernel: off[b], cnt[b] clipped to the nbuf slots),
+// WRITE stores the group's first cnt[b] matches, in traversal order, at
+// off[b].  Edges are thus in group order and the kept ones (when the step
+// has more than nbuf) do not depend on scheduling.
+template <bool WRITE>
 __global__ void __launch_bounds__(256)
-    plan_near_kernel(int ndof, int num, const float* __restrict__ qnew, StepInfo* info, PlanTree t0, PlanTree t1,
-                     int cap, CellGrid g, float D2, int* __restrict__ cnt, int nbuf, int* __restrict__ enode,
-                     float* __restrict__ ed2, int* __restrict__ egroup) {
+    plan_near_kernel(int ndof, int num, const float* __restrict__ qnew, const StepInfo* __restrict__ info,
+                     PlanTree t0, PlanTree t1, int cap, CellGrid g, float D2, int* __restrict__ cnt,
+                     const int* __restrict__ off, int* __restrict__ enode, float* __restrict__ ed2,
+                     int* __restrict__ egroup) {
   constexpr int NK = kNeighbourCells;
   __shared__ int s_start[NK], s_pre[NK + 1];
   __shared__ int lds[kMaxWaves + 1];
-  __shared__ int s_stored;
+  __shared__ int s_count;
   const int b = blockIdx.x, j = b >> 1;
   if (j >= info->nnew) {
-    if (threadIdx.x == 0) cnt[b] = 0;
+    if (!WRITE && threadIdx.x == 0) cnt[b] = 0;
     return;
   }
+  const int keep = WRITE ? cnt[b] : 0;
+  if (WRITE && keep == 0) return;
   const PlanTree& T = tree_of(t0, t1, b & 1);
   float qr[kMaxDofDev];
   for (int i = 0; i < ndof; ++i) qr[i] = qnew[j + (size_t)num * i];
@@ -296,11 +302,11 @@ __global__ void __launch_bounds__(256)
   }
   if (threadIdx.x == 0) {
     s_pre[K] = total;
-    s_stored = 0;
+    s_count = 0;
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  int stored = 0;  // lane 0: this wave's stored matches
+  int run = 0;  // WRITE: matches of the earlier chunks
   for (int base = 0; base < total; base += blockDim.x) {
     const int idx = base + threadIdx.x;
     float normsq = 0.0f;
@@ -318,82 +324,74 @@ __global__ void __launch_bounds__(256)
       hit = normsq < D2;
       node = T.sid[pos];
     }
-    const unsigned long long m = __ballot(hit);
-    if (m == 0ull) continue;
-    int wbase = 0;
-    if (lane == 0) {
-      wbase = atomicAdd(&info->E_all, __popcll(m));
-      stored += max(0, min(__popcll(m), nbuf - wbase));
-    }
-    wbase = __shfl(wbase, 0, 64);
-    const int o = wbase + __popcll(m & ((1ull << lane) - 1ull));
-    if (hit && o < nbuf) {
-      enode[o] = node;
-      ed2[o] = normsq;
-      egroup[o] = b;
+    if (WRITE) {
+      int chunk;
+      const int r = run + block_excl_scan((int)hit, &chunk, lds);
+      if (hit && r < keep) {
+        const int o = off[b] + r;
+        enode[o] = node;
+        ed2[o] = normsq;
+        egroup[o] = b;
+      }
+      run += chunk;
+      if (run >= keep) break;  // uniform: run and keep are the same in every thread
+    } else {
+      const unsigned long long m = __ballot(hit);
+      if (lane == 0 && m) atomicAdd(&s_count, __popcll(m));
     }
   }
-  if (lane == 0 && stored) atomicAdd(&s_stored, stored);
-  __syncthreads();
-  if (threadIdx.x == 0) cnt[b] = s_stored;
+  if (!WRITE) {
+    __syncthreads();
+    if (threadIdx.x == 0) cnt[b] = s_count;
+  }
 }
 
-// Edges of group b stored (cnt counts only slots below nbuf).
+// Stored edges of group b.
 __device__ __forceinline__ int stored_edges(const int* off, const int* cnt, int b, int nbuf) {
   (void)off;
   (void)nbuf;
   return cnt[b];
 }
 
+// After the counting pass: off[b] = exclusive scan of the groups' matches
+// (clipped to nbuf), info->E_all = their total, info->E = min(E_all, nbuf),
+// cnt[b] = the group's matches that fit below nbuf; with budget B > 0 also
+// sc_off over the groups with more than B stored edges and info->S.
 __global__ void __launch_bounds__(kScanBlock)
-    plan_scan_groups_kernel(int groups, const int* __restrict__ cnt, int nbuf, int budget, int* __restrict__ off,
-                            int* __restrict__ sc_off, int* __restrict__ fill, StepInfo* info) {
+    plan_scan_groups_kernel(int groups, int* __restrict__ cnt, int nbuf, int budget, int* __restrict__ off,
+                            int* __restrict__ sc_off, StepInfo* info) {
   __shared__ int lds[kMaxWaves + 1];
-  const int all = info->E_all;
+  const int per = (groups + kScanBlock - 1) / kScanBlock;
+  const int b0 = threadIdx.x * per, b1 = min(groups, b0 + per);
+  int s = 0;
+  for (int b = b0; b < b1; ++b) s += cnt[b];
+  int all;
+  int run = block_excl_scan(s, &all, lds);
+  int sc = 0;
+  for (int b = b0; b < b1; ++b) {
+    const int c = cnt[b];
+    const int kept = max(0, min(c, nbuf - run));
+    off[b] = min(run, nbuf);
+    cnt[b] = kept;
+    sc += kept > budget ? kept : 0;
+    run += c;
+  }
   int S = 0;
   if (budget > 0) {
-    // group offsets of the stored edges, and of the scored groups' edges
-    const int per = (groups + kScanBlock - 1) / kScanBlock;
-    const int b0 = threadIdx.x * per, b1 = min(groups, b0 + per);
-    int s = 0, sc = 0;
-    for (int b = b0; b < b1; ++b) {
-      s += cnt[b];
-      sc += cnt[b] > budget ? cnt[b] : 0;
-    }
-    int stored;
-    int run = block_excl_scan(s, &stored, lds);
     int srun = block_excl_scan(sc, &S, lds);
     for (int b = b0; b < b1; ++b) {
-      off[b] = run;
-      run += cnt[b];
       sc_off[b] = srun;
       srun += cnt[b] > budget ? cnt[b] : 0;
-      fill[b] = 0;
-    }
-    if (threadIdx.x == 0) {
-      off[groups] = stored;
-      sc_off[groups] = S;
     }
   }
   if (threadIdx.x == 0) {
-    info->E = all < nbuf ? all : nbuf;
+    const int E = all < nbuf ? all : nbuf;
+    off[groups] = E;
+    if (budget > 0) sc_off[groups] = S;
+    info->E_all = all;
+    info->E = E;
     info->S = S;
   }
-}
-
-// Edges of the radius search (any order) into group order: edge e of group
-// b to off[b] + its rank of arrival.
-__global__ void __launch_bounds__(256)
-    plan_group_scatter(const StepInfo* __restrict__ info, const int* __restrict__ gs, const int* __restrict__ ns,
-                       const float* __restrict__ ds, const int* __restrict__ off, int* __restrict__ fill,
-                       int* __restrict__ egroup, int* __restrict__ enode, float* __restrict__ ed2) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= info->E) return;
-  const int b = gs[e];
-  const int p = off[b] + atomicAdd(&fill[b], 1);
-  egroup[p] = b;
-  enode[p] = ns[e];
-  ed2[p] = ds[e];
 }
 
 // ---- edge budget ----------------------------------------------------------------
@@ -449,8 +447,10 @@ __global__ void __launch_bounds__(256)
       const int o = off[b];
       const bool be = score[e] > 0.5f;
       const float de = ed2[e];
+      // skipped once budget edges rank ahead: only the kept edges scan
+      // the whole group
       int rank = 0;
-      for (int x = o; x < o + w; ++x) {
+      for (int x = o; x < o + w && rank < budget; ++x) {
         const bool bx = score[x] > 0.5f;
         const float dx = ed2[x];
         rank += (bx < be) || (bx == be && (dx < de || (dx == de && x < e)));
@@ -812,25 +812,15 @@ hipError_t launch_plan_select(int ndof, int batch, const int* codes, const unsig
 }
 
 hipError_t launch_plan_near(int ndof, int num, const float* qnew, StepInfo* info, PlanTree t0, PlanTree t1, int cap,
-                            const CellGrid& g, float D2, int* cnt, int nbuf, int* enode, float* ed2, int* egroup,
-                            hipStream_t st) {
-  if (num <= 0) return hipSuccess;
-  hipLaunchKernelGGL(plan_near_kernel, dim3(2 * num), dim3(256), 0, st, ndof, num, qnew, info, t0, t1, cap, g, D2, cnt,
-                     nbuf, enode, ed2, egroup);
-  return hipGetLastError();
-}
-
-hipError_t launch_plan_scan_groups(int groups, const int* cnt, int nbuf, int budget, int* off, int* sc_off,
-                                   int* fill, StepInfo* info, hipStream_t st) {
-  hipLaunchKernelGGL(plan_scan_groups_kernel, dim3(1), dim3(kScanBlock), 0, st, groups, cnt, nbuf, budget, off, sc_off,
-                     fill, info);
-  return hipGetLastError();
-}
-
-hipError_t launch_plan_group_scatter(int nbuf, const StepInfo* info, const int* gs, const int* ns, const float* ds,
-                                     const int* off, int* fill, int* egroup, int* enode, float* ed2, hipStream_t st) {
-  hipLaunchKernelGGL(plan_group_scatter, dim3(blocks_for(nbuf, 256)), dim3(256), 0, st, info, gs, ns, ds, off, fill,
-                     egroup, enode, ed2);
+                            const CellGrid& g, float D2, int* cnt, int* off, int* sc_off, int nbuf, int budget,
+                            int* enode, float* ed2, int* egroup, hipStream_t st) {
+  const int groups = 2 * num;
+  hipLaunchKernelGGL(plan_near_kernel<false>, dim3(groups), dim3(256), 0, st, ndof, num, qnew, info, t0, t1, cap, g,
+                     D2, cnt, off, enode, ed2, egroup);
+  hipLaunchKernelGGL(plan_scan_groups_kernel, dim3(1), dim3(kScanBlock), 0, st, groups, cnt, nbuf, budget, off,
+                     sc_off, info);
+  hipLaunchKernelGGL(plan_near_kernel<true>, dim3(groups), dim3(256), 0, st, ndof, num, qnew, info, t0, t1, cap, g,
+                     D2, cnt, off, enode, ed2, egroup);
   return hipGetLastError();
 }
 

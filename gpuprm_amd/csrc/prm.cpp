@@ -59,7 +59,7 @@ double ms_since(Clock::time_point t) {
 // Points scored along an edge by the model: interior points at t = s/(S+1).
 // Three points (t = 1/4, 1/2, 3/4) find the edges the model predicts blocked
 // at 3/8 of the inference of eight.
-constexpr int kScorePoints = 3;
+constexpr int kScorePoints = 1;
 // Undirected edges a tree may store, per node of its capacity.
 constexpr long long kEdgesPerNode = 16;
 
@@ -217,8 +217,8 @@ struct Worker {
   int slot = -1;
   Slot* sl = nullptr;
   hipStream_t st = nullptr;
-  DevArr cand, codes, pskip, pprob, qnew, info, tile, cnt, off, sc_off, fill, enode, ed2, egroup, snode, sd2, sgroup,
-      eskip, escore, er, qs, qe, free_cnt, pos, spts, sprob;
+  DevArr cand, codes, pskip, pprob, qnew, info, tile, cnt, off, sc_off, enode, ed2, egroup, eskip, escore, er, qs,
+      qe, free_cnt, pos, spts, sprob;
   Readback* pin = nullptr;
   hipEvent_t ev_rb = nullptr;
   int E = 0, S = 0, batch = 0;
@@ -377,9 +377,7 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
         !x.pos.ensure(sizeof(int) * (size_t)groups) ||
         !x.tile.ensure(sizeof(int) * 3 * (size_t)(batch_max / 1024 + 1)) ||
         (bias && (!x.pskip.ensure((size_t)batch_max) || !x.pprob.ensure(sizeof(float) * (size_t)batch_max))) ||
-        (budget > 0 && (!x.eskip.ensure((size_t)nbuf) || !x.escore.ensure(sizeof(float) * (size_t)nbuf) ||
-                        !x.fill.ensure(sizeof(int) * (size_t)groups) || !x.snode.ensure(sizeof(int) * (size_t)nbuf) ||
-                        !x.sd2.ensure(sizeof(float) * (size_t)nbuf) || !x.sgroup.ensure(sizeof(int) * (size_t)nbuf))))
+        (budget > 0 && (!x.eskip.ensure((size_t)nbuf) || !x.escore.ensure(sizeof(float) * (size_t)nbuf))))
       return GPRM_E_HIP;
     if ((rc = reserve_slot(ctx, *x.sl, batch_max, 0, maxT_bound)) != GPRM_OK) return rc;
   }
@@ -459,19 +457,13 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
                            w.qnew.as<float>(), w.tile.as<int>(), info, state, st) != hipSuccess ||
         !end_iv(iv_s, st))
       return GPRM_E_HIP;
-    // the radius search stores its edges in arrival order; the edge budget
-    // ranks within groups, so it moves them into group order first
-    const bool grouped = budget > 0;
+    // the radius search stores its edges in group order (the edge budget
+    // ranks within groups)
     const int iv_n = begin_iv(st, &res->ms_near);
     if (iv_n < 0 ||
-        launch_plan_near(nd, num, w.qnew.as<float>(), info, T[0], T[1], cap, grid, D2, w.cnt.as<int>(), nbuf,
-                         (grouped ? w.snode : w.enode).as<int>(), (grouped ? w.sd2 : w.ed2).as<float>(),
-                         (grouped ? w.sgroup : w.egroup).as<int>(), st) != hipSuccess ||
-        launch_plan_scan_groups(groups, w.cnt.as<int>(), nbuf, budget, w.off.as<int>(), w.sc_off.as<int>(),
-                                w.fill.as<int>(), info, st) != hipSuccess ||
-        (grouped && launch_plan_group_scatter(nbuf, info, w.sgroup.as<int>(), w.snode.as<int>(), w.sd2.as<float>(),
-                                              w.off.as<int>(), w.fill.as<int>(), w.egroup.as<int>(),
-                                              w.enode.as<int>(), w.ed2.as<float>(), st) != hipSuccess) ||
+        launch_plan_near(nd, num, w.qnew.as<float>(), info, T[0], T[1], cap, grid, D2, w.cnt.as<int>(),
+                         w.off.as<int>(), w.sc_off.as<int>(), nbuf, budget, w.enode.as<int>(), w.ed2.as<float>(),
+                         w.egroup.as<int>(), st) != hipSuccess ||
         !end_iv(iv_n, st) || hipEventRecord(ev_chain, st) != hipSuccess ||
         hipMemcpyAsync(&w.pin->info, info, sizeof(StepInfo), hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipMemcpyAsync(&w.pin->state, state, sizeof(PlanState), hipMemcpyDeviceToHost, st) != hipSuccess ||
