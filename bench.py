@@ -569,7 +569,7 @@ def leg_prm():
     sp2.init(0, 1)
     o2 = OracleScene(d2)
     rng = np.random.default_rng(SEED + 11)
-    N, ntest, B = 1 << 22, 1 << 16, 8192
+    N, ntest, B = 1 << 22, 1 << 16, 32768
     Q = torch.from_numpy(rng.uniform(-np.pi, np.pi, size=(6, N)).astype(np.float32)).cuda()
     codes = torch.zeros(N, dtype=torch.int32, device="cuda")
     torch.cuda.synchronize()
@@ -579,12 +579,12 @@ def leg_prm():
     t_label = time.perf_counter() - t0
     model = gpuprm_amd.Surrogate(ndof=6, batch=B, seed=3)
     ntr = N - ntest
-    epochs = 3
+    epochs = 8
     t0 = time.perf_counter()
     for ep in range(epochs):
         for it in range(ntr // B):
-            model.train_step(Q[:, it * B:(it + 1) * B], codes[it * B:(it + 1) * B], lr=2e-3 if ep < 2 else 5e-4,
-                             offset=N, want_loss=False)
+            model.train_step(Q[:, it * B:(it + 1) * B], codes[it * B:(it + 1) * B],
+                             lr=4e-3 if ep < epochs - 1 else 1e-3, offset=N, want_loss=False)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     pt = torch.zeros(ntest, dtype=torch.float32, device="cuda")
@@ -603,14 +603,16 @@ def leg_prm():
              "budget_shortest_no_model": {"edge_budget": 4}}
     tasks2 = _tasks(o2, 6, 16, SEED + 13)
     sweep = {}
-    for num in (1024, 4096):
+    for num in (1024, 2048, 4096):
         res = {k: [] for k in modes}
         for i, (a, b) in enumerate(tasks2):
             for k, kw in modes.items():
                 res[k].append(_prm_run(sp2, o2, a, b, verify=(num == 1024), num=num, D=1.0, max_nodes=1 << 18,
                                        max_steps=300, seed=i, **kw))
         med = {k: float(np.median([x["ms"] for x in v])) for k, v in res.items()}
-        sweep[num] = {"ms_median": med, "speedup": med["plain"] / med["surrogate"],
+        phases = {k: {ph: float(np.median([x["ms_device"][ph] for x in v])) for ph in v[0]["ms_device"]}
+                  for k, v in res.items()}
+        sweep[num] = {"ms_median": med, "ms_device_median": phases, "speedup": med["plain"] / med["surrogate"],
                       "speedup_budget_no_model": med["plain"] / med["budget_shortest_no_model"],
                       "edge_samples_saved": 1 - sum(x["edge_samples"] for x in res["surrogate"]) /
                       max(1, sum(x["edge_samples"] for x in res["plain"])),
@@ -623,13 +625,13 @@ def leg_prm():
     best = {k: min(sweep[n]["ms_median"][k] for n in sweep) for k in modes}
     out["surrogate_vs_plain"] = {
         "workload": "6-DOF arm, 16 hulls of 12 points (edge-checking scene), D=1.0, 16 tasks, "
-                    "1024 and 4096 new nodes/step",
+                    "1024, 2048 and 4096 new nodes/step",
         "labels": N, "label_ms": 1000 * t_label, "train_epochs": epochs, "train_batch": B, "train_s": t_train,
         "train_tflops": epochs * (ntr // B) * B * 3 * flop / t_train / 1e12, "auc": auc,
         "predict_configs_per_s": N / t_pred, "predict_tflops": N * flop / t_pred / 1e12,
         "predict_mfma_frac": N * flop / t_pred / 2.5e15,
         "ms_median": sweep[1024]["ms_median"], "speedup": sweep[1024]["speedup"],
-        "speedup_4096": sweep[4096]["speedup"],
+        "speedup_2048": sweep[2048]["speedup"], "speedup_4096": sweep[4096]["speedup"],
         "best_of_num_ms": best, "speedup_best_of_num": best["plain"] / best["surrogate"],
         "edge_samples_saved": sweep[1024]["edge_samples_saved"],
         "point_checks_saved": sweep[1024]["point_checks_saved"],

@@ -204,6 +204,60 @@ int build_image(gprm_ctx* c, std::vector<char>* out) {
     int kx = s.sys[s.to[x]] * 256 + s.sys[s.from[x]], ky = s.sys[s.to[y]] * 256 + s.sys[s.from[y]];
     return kx < ky;
   });
+  // The kernel keeps the last P frame and the last Q frame of a segment.
+  // Within a block of equal Q frame, the pairs whose P is also a moving link
+  // (link-link pairs) each load their P frame unless it is the one kept, so
+  // each block's P frames are ordered to start with the frame the previous
+  // block ended with and, where possible, to end with one the next block
+  // needs (on the 6-DOF arm: 7 P loads per item instead of 10).  The order
+  // of the pairs does not change results: edge results only ask whether any
+  // pair collides, point codes take the smallest CRS index (atomicMin).
+  {
+    const auto sysP = [&](int i) { return s.sys[s.from[i]]; };
+    const auto sysQ = [&](int i) { return s.sys[s.to[i]]; };
+    std::vector<std::pair<int, int>> blocks;  // [begin, end) of equal sysQ
+    for (int j = 0; j < M;) {
+      int e = j;
+      while (e < M && sysQ(order[e]) == sysQ(order[j])) ++e;
+      blocks.push_back({j, e});
+      j = e;
+    }
+    int prevP = -1;
+    for (size_t b = 0; b < blocks.size(); ++b) {
+      const int j0 = blocks[b].first, j1 = blocks[b].second;
+      // the block's P frames in sorted order (runs of equal sysP stay together)
+      std::vector<int> ps;
+      for (int j = j0; j < j1; ++j)
+        if (sysP(order[j]) != 0 && (ps.empty() || ps.back() != sysP(order[j]))) ps.push_back(sysP(order[j]));
+      if (ps.empty()) continue;
+      std::vector<int> next;
+      if (b + 1 < blocks.size())
+        for (int j = blocks[b + 1].first; j < blocks[b + 1].second; ++j)
+          if (sysP(order[j]) != 0) next.push_back(sysP(order[j]));
+      auto in_next = [&](int v) { return std::find(next.begin(), next.end(), v) != next.end(); };
+      // the kept frame first, then the others; the first of those the next
+      // block needs goes last
+      std::vector<int> seq, rest;
+      if (std::find(ps.begin(), ps.end(), prevP) != ps.end()) seq.push_back(prevP);
+      for (int v : ps)
+        if (v != prevP) rest.push_back(v);
+      for (size_t k = 0; k < rest.size(); ++k)
+        if (in_next(rest[k])) {
+          std::rotate(rest.begin() + k, rest.begin() + k + 1, rest.end());
+          break;
+        }
+      seq.insert(seq.end(), rest.begin(), rest.end());
+      // stable regroup: obstacle pairs (sysP 0) first, then the P runs in seq
+      std::vector<int> reord;
+      for (int j = j0; j < j1; ++j)
+        if (sysP(order[j]) == 0) reord.push_back(order[j]);
+      for (int v : seq)
+        for (int j = j0; j < j1; ++j)
+          if (sysP(order[j]) == v) reord.push_back(order[j]);
+      std::copy(reord.begin(), reord.end(), order.begin() + j0);
+      prevP = seq.back();
+    }
+  }
   for (int j = 0; j < M; ++j) {
     const int i = order[j];
     int pr[4] = {vbase[s.from[i]] | (vbase[s.to[i]] << 16), s.sys[s.from[i]] | (s.sys[s.to[i]] << 8), i,
