@@ -58,6 +58,7 @@ struct Slot {
   hipEvent_t ev_first = nullptr;
   std::vector<hipEvent_t> ev_fk, ev_sva;
   DevBuf frames, status, item_edge, item_pos, counters, qs, qe, res, testnum, stride, misc, tally, hist;
+  DevBuf groups;  // point calls: completed units per group of 64 items
   DevBuf frames_odd, item_edge_odd, item_pos_odd;  // buffer set of the odd edge launches
   int maxT_hint = 0;  // largest testnum of the slot's earlier edge calls (sizes buffers ahead)
   int* small_out = nullptr;  // pinned codes of small point-check calls (kSmallMax ints)

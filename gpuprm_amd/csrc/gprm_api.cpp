@@ -404,13 +404,15 @@ int run_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n,
   const int nchunks = (n + chunk - 1) / chunk;
   GPRM_HIP(sl.hist.ensure(sizeof(float) * 4 * kMaxIter * (size_t)c->shape.grid * c->shape.block), "hipMalloc history");
   GPRM_HIP(sl.counters.ensure(sizeof(unsigned) * (size_t)nchunks), "hipMalloc counters");
+  GPRM_HIP(sl.groups.ensure(sizeof(int) * (size_t)((chunk + 63) / 64)), "hipMalloc group counters");
   for (int ci = 0; ci < nchunks; ++ci) {
     int c0 = ci * chunk, cn = std::min(chunk, n - c0);
     size_t ev = 0;
     GPRM_HIP(prof_begin(c, sl, 1, st, &ev), "event");
     GPRM_HIP(launch_fk_points(c->rb, d_q + c0, offset, cn, sl.frames.as<float>(), frame_stride(cn),
                               sl.status.as<int>(), 1,
-                              sl.counters.as<unsigned>() + ci, st, skip ? skip + c0 : nullptr),
+                              sl.counters.as<unsigned>() + ci, st, skip ? skip + c0 : nullptr,
+                              sl.groups.as<int>()),
              "kinematics launch");
     GPRM_HIP(prof_end(c, sl, ev, st), "event");
     SvaArgs a{};
@@ -424,13 +426,15 @@ int run_points(gprm_ctx* c, Slot& sl, const float* d_q, long long offset, int n,
     a.pairs_per_unit = unit_pairs(c, cn);
     a.units_per_item = (c->img.M + a.pairs_per_unit - 1) / a.pairs_per_unit;
     a.word = sl.status.as<int>();
+    // the separating-vector launch writes the codes (no finalize launch)
+    a.group_done = sl.groups.as<int>();
+    a.codes = d_res + c0;
     a.tally = tally_ptr(c, sl);
     a.static_first = 1;
     GPRM_HIP(prof_begin(c, sl, 0, st, &ev), "event");
     GPRM_HIP(launch_sva(a, 0, launch_shape(c, (long long)((cn + 63) / 64) * a.units_per_item), st),
              "separating-vector launch");
     GPRM_HIP(prof_end(c, sl, ev, st), "event");
-    GPRM_HIP(launch_finalize_points(sl.status.as<int>(), d_res + c0, cn, st), "finalize launch");
   }
   return GPRM_OK;
 }
@@ -711,6 +715,7 @@ int reserve_slot(gprm_ctx* c, Slot& sl, int n_points, int N, int maxT_bound) {
     need.push_back({&sl.frames, sizeof(float) * 12 * nd * (size_t)frame_stride(chunk)});
     need.push_back({&sl.status, sizeof(int) * (size_t)chunk});
     need.push_back({&sl.counters, sizeof(unsigned) * (size_t)((n_points + chunk - 1) / chunk)});
+    need.push_back({&sl.groups, sizeof(int) * (size_t)((chunk + 63) / 64)});
   }
   if (N > 0) {
     // the launches of run_edges's plan: windows split into edge ranges,

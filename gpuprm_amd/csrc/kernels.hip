@@ -177,10 +177,13 @@ template <int NDOF>
 __global__ void __launch_bounds__(kFkBlock)
     fk_points_kernel(DevRobot rb, const float* __restrict__ q, long long offset, int n,
                      float* __restrict__ frames, long long fstride, int* __restrict__ word, int check_bounds,
-                     unsigned* __restrict__ zero_counter, const unsigned char* __restrict__ skip) {
+                     unsigned* __restrict__ zero_counter, const unsigned char* __restrict__ skip,
+                     int* __restrict__ group_done) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   // the separating-vector launch that follows claims units from this counter
+  // and counts completed units per group of 64 items in group_done
   if (i == 0 && zero_counter != nullptr) *zero_counter = 0u;
+  if (group_done != nullptr && (i & 63) == 0 && i < n) group_done[i >> 6] = 0;
   if (i >= n) return;
   const int nd = NDOF > 0 ? NDOF : rb.ndof;
   float qq[kMaxDofDev];
@@ -992,6 +995,46 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, const 
   unsigned d_head = 0, d_tail = 0;  // the ring's cursors (the same in every lane)
   unsigned n_sep = 0, n_col = 0, n_dropped = 0, n_units = 0, n_iter = 0, n_bound = 0, n_claimed = 0;
 
+  // Point mode with the codes written here: a unit is complete once the
+  // ring holds none of its deferred pairs, which is certain whenever the
+  // ring is empty.  Units whose pairs are all walked wait in a per-wave list
+  // (the ring's edge array, unused in point mode) until the ring next
+  // empties; then each adds one to its group's counter, and the wave that
+  // brings a group to units_per_item writes that group's 64 codes.  No wave
+  // waits for another.
+  const bool fuse = !EDGE && A.group_done != nullptr;
+  unsigned n_pend = 0;  // the same in every lane
+  auto complete_pending = [&]() {
+    for (unsigned k = 0; k < n_pend; ++k) {
+      const unsigned cu = (unsigned)R.edge[k];
+      const unsigned g = cu % ngroups;
+      int old = 0;
+      // Words and counters are only touched by agent-scope atomics, which
+      // are coherent across the XCDs' L2s; what matters is that this wave's
+      // word updates (any lane) have completed before its count, which
+      // waiting for all its outstanding memory operations gives.  (An
+      // agent-scope release or acquire fence would also write back or
+      // invalidate the XCD's whole L2, per wave: it made the launch three
+      // times slower.)
+      __builtin_amdgcn_s_waitcnt(0);
+      if (lane == 0)
+        old = __hip_atomic_fetch_add(&A.group_done[g], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      old = __shfl(old, 0, 64);
+      if (old == A.units_per_item - 1) {
+        // every unit of the group has finished: the words are final (the
+        // loads below are issued after the count came back)
+        const int it = (int)(g << 6) + lane;
+        if (it < nitems) {
+          const int w = __hip_atomic_load(&A.word[it], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          A.codes[it] = w == 0x7fffffff ? 0 : (w & 15) - 1;
+        }
+      }
+    }
+    n_pend = 0;
+    // the list's entries were read by every lane before lane 0 refills it
+    __builtin_amdgcn_wave_barrier();
+  };
+
   // Work distribution.  Claims on one shared counter serialise at the
   // memory side, so most units are assigned statically: with static_first,
   // wave w takes units w, w + nwaves, ... for three quarters of the full
@@ -1178,12 +1221,26 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, const 
           break;
         }
       }
-      if (d_tail - d_head >= CAP - 64)
+      if (d_tail - d_head >= CAP - 64) {
         sva_drain<EDGE, CAP>(A, L, R, H, hs, d_head, d_tail, n_sep, n_col, n_dropped, n_iter);
+        if (fuse) complete_pending();  // the units before this one
+      }
     }
     ++n_units;
+    if (fuse) {
+      if (lane == 0) R.edge[n_pend] = (int)u;
+      __builtin_amdgcn_wave_barrier();
+      ++n_pend;
+      if (d_tail == d_head) {
+        complete_pending();
+      } else if (n_pend == (unsigned)CAP) {
+        sva_drain<EDGE, CAP>(A, L, R, H, hs, d_head, d_tail, n_sep, n_col, n_dropped, n_iter);
+        complete_pending();
+      }
+    }
   }
   if (d_tail != d_head) sva_drain<EDGE, CAP>(A, L, R, H, hs, d_head, d_tail, n_sep, n_col, n_dropped, n_iter);
+  if (fuse) complete_pending();
 
   cnt.sep += n_sep;
   cnt.col += n_col;
@@ -1442,13 +1499,19 @@ int sva_max_lds_bytes() { return 160 * 1024; }
     default: hipLaunchKernelGGL(KERNEL<0>, GRID, BLOCK, 0, ST, __VA_ARGS__); break;        \
   }
 
+// Threads per kinematics block: kFkBlock when that still gives at least two
+// blocks per CU, else 256 (65,536 configurations in 1024-thread blocks
+// would occupy 64 of the 256 CUs).
+static int fk_block(long long n) { return n >= 2LL * 256 * kFkBlock ? kFkBlock : 256; }
+
 hipError_t launch_fk_points(const DevRobot& rb, const float* q, long long offset, int n, float* frames,
                             long long fstride, int* word, int check_bounds, unsigned* zero_counter,
-                            hipStream_t st, const unsigned char* skip) {
+                            hipStream_t st, const unsigned char* skip, int* group_done) {
   if (n <= 0) return hipSuccess;
-  dim3 grid((n + kFkBlock - 1) / kFkBlock), block(kFkBlock);
+  const int fb = fk_block(n);
+  dim3 grid((n + fb - 1) / fb), block(fb);
   GPRM_FK_DISPATCH(fk_points_kernel, grid, block, st, rb, q, offset, n, frames, fstride, word,
-                   check_bounds, zero_counter, skip);
+                   check_bounds, zero_counter, skip, group_done);
   return hipGetLastError();
 }
 
@@ -1461,9 +1524,10 @@ hipError_t launch_finalize_points(const int* word, int* codes, int n, hipStream_
 hipError_t launch_edge_fk(const DevRobot& rb, EdgeArgs ea, const int* item_edge, const int* item_pos,
                           const int* nitems, int cap, float* frames, hipStream_t st) {
   if (cap <= 0) return hipSuccess;
-  int blocks = (cap + kFkBlock - 1) / kFkBlock;
+  const int fb = fk_block(cap);
+  int blocks = (cap + fb - 1) / fb;
   if (blocks > 4096) blocks = 4096;
-  dim3 grid(blocks), block(kFkBlock);
+  dim3 grid(blocks), block(fb);
   GPRM_FK_DISPATCH(fk_edges_kernel, grid, block, st, rb, ea, item_edge, item_pos, nitems, cap, frames,
                    (long long)cap);
   return hipGetLastError();

@@ -63,6 +63,12 @@ struct SvaArgs {
   int units_per_item;      // ceil(M / pairs_per_unit)
   // point mode
   int* word;               // [item] first colliding pair and code, see fk_points_kernel
+  // point mode, optional (both null: a separate finalize launch writes the
+  // codes): the wave that completes a group's last unit writes the group's
+  // codes from the words.  group_done: [groups of 64 items] completed units
+  // (zero before launch).
+  int* group_done;
+  int* codes;
   // edge mode
   const int* item_edge;    // [item] owning edge
   int* edge_res;           // [edge] nonzero once any sample collides
@@ -181,7 +187,7 @@ int sva_max_lds_bytes();
 // kernel tests none of their pairs (code 1).
 hipError_t launch_fk_points(const DevRobot& rb, const float* q, long long offset, int n, float* frames,
                             long long fstride, int* word, int check_bounds, unsigned* zero_counter, hipStream_t st,
-                            const unsigned char* skip = nullptr);
+                            const unsigned char* skip = nullptr, int* group_done = nullptr);
 hipError_t launch_finalize_points(const int* word, int* codes, int n, hipStream_t st);
 struct SvaShape;
 hipError_t launch_sva(const SvaArgs& a, int edge_mode, const SvaShape& sh, hipStream_t st);
