@@ -325,19 +325,11 @@ def leg_multi_scene(reps):
     import torch
 
     import gpuprm_amd
+    from gpuprm_amd.tune import multi_scene_workload
 
-    rng = np.random.default_rng(SEED + 5)
-    spaces, dirs = [], []
-    for s in range(64):
-        d = tempfile.mkdtemp(prefix="gprm_ms_")
-        gpuprm_amd.generate_scene(5, 1000 + s, d)
-        sp = gpuprm_amd.RobotConfigspace(d, 0.01, 1024)
-        sp.init(0, 1)
-        spaces.append(sp)
-        dirs.append(d)
-    sizes = [int(x) for x in rng.integers(1000, 65537, 64)]
-    qs = [torch.from_numpy(rng.uniform(-np.pi, np.pi, size=(7, n)).astype(np.float32)).cuda() for n in sizes]
-    rs = [torch.zeros(n, dtype=torch.int32, device="cuda") for n in sizes]
+    # the workload the committed tuning table was made on (python -m
+    # gpuprm_amd.tune)
+    spaces, dirs, sizes, qs, rs = multi_scene_workload(SEED)
     # the committed MI355X table (SceneBatch's default); a fresh tuning run
     # is reported beside it, not applied
     b = gpuprm_amd.SceneBatch(spaces)
