@@ -23,7 +23,9 @@ __all__ = ["RobotConfigspace", "Surrogate", "GprmError", "EdgeStats", "ProfileSt
            "BucketInfo", "SceneBatch", "radius_neighbours",
            "read_prm_output", "canonical_edges"]
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgprm.so")
+# GPRM_LIBRARY names another build of the same sources (the sanitized host
+# build of build.build_sanitized(), for CPU-side tests)
+LIB_PATH = os.environ.get("GPRM_LIBRARY") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgprm.so")
 
 GPRM_OK = 0
 _ERRORS = {

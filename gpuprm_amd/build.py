@@ -54,6 +54,23 @@ def build_hip() -> str:
     return LIB
 
 
+LIB_SAN = os.path.join(PKG, "libgprm_san.so")
+# AddressSanitizer and UndefinedBehaviorSanitizer on the host code only (the
+# device code is built as usual); load with clang's ASan runtime preloaded.
+SAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+             "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer", "-g"]
+
+
+def build_sanitized() -> str:
+    """libgprm_san.so (host code sanitized) and the sanitized oracle, for the
+    CPU-side tests (tests/run_sanitized.sh)."""
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    flags = [f for f in HIP_FLAGS if f != "-O3"] + ["-O1"]
+    _run([hipcc] + flags + SAN_FLAGS + ["-o", LIB_SAN] + [os.path.join(CSRC, s) for s in SOURCES])
+    _run(["make", "-B", "-C", os.path.join(ROOT, "oracle"), "san"])
+    return LIB_SAN
+
+
 def build_oracle() -> None:
     _run(["make", "-B", "-C", os.path.join(ROOT, "oracle"), "all"])
 

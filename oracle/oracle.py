@@ -14,8 +14,11 @@ from typing import Optional
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_SO = os.path.join(HERE, "liboracle.so")
-ORACLE_COUNT_SO = os.path.join(HERE, "liboracle_count.so")
+# GPRM_ORACLE_SUFFIX selects another build of the same sources, e.g. "_san"
+# for the AddressSanitizer/UndefinedBehaviorSanitizer build (make san)
+_SFX = os.environ.get("GPRM_ORACLE_SUFFIX", "")
+ORACLE_SO = os.path.join(HERE, "liboracle%s.so" % _SFX)
+ORACLE_COUNT_SO = os.path.join(HERE, "liboracle_count%s.so" % _SFX)
 REF_SO = os.path.join(HERE, "_ref", "libref.so")
 
 _FP = C.POINTER(C.c_float)
