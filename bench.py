@@ -17,7 +17,8 @@ early exit).
 Further legs (--extras): point checks of the 4-DOF / 8-box configuration
 with their roofline and the latency of small host-array calls; 64 scenes in
 one graph-replayed batch (the committed tuning table, a fresh tuning run
-beside it) with roofline and CPU baseline; the planner's time-to-path on the
+beside it) with roofline and CPU baseline; the surrogate MLP's training and
+inference throughput against the BF16 MFMA peak; the planner's time-to-path on the
 7-DOF scene, the 7-DOF configuration as written (200,000 uniform samples),
 and plain vs surrogate over 16 tasks of the 6-DOF scene.
 
@@ -522,6 +523,72 @@ def leg_uniform_prm(sp3, o3, start, goal):
             "ms": {"points": 1000 * t_points, "near": 1000 * t_near, "edges": 1000 * t_edges}}
 
 
+PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
+
+
+def leg_mlp():
+    """The surrogate MLP's kernels on the 6-DOF scene: training (the four
+    launches of a step, batch 32,768, Adam) over 8 epochs of 4M exact labels,
+    then inference over the 4M configurations; HIP events on the stream the
+    kernels run on (the null stream), algorithmic FLOPs (forward
+    2 (6 x 256 + 2 x 256^2 + 256) per configuration, a training step three
+    times that), held-out AUC."""
+    import torch
+
+    import gpuprm_amd
+    from oracle import mlp
+
+    d = tempfile.mkdtemp(prefix="gprm_mlp_")
+    gpuprm_amd.generate_scene(2, SEED, d)
+    sp = gpuprm_amd.RobotConfigspace(d, 0.01, 1 << 20)
+    sp.init(0, 1)
+    rng = np.random.default_rng(SEED + 11)
+    N, ntest, B, epochs = 1 << 22, 1 << 16, 32768, 8
+    Q = torch.from_numpy(rng.uniform(-np.pi, np.pi, size=(6, N)).astype(np.float32)).cuda()
+    codes = torch.zeros(N, dtype=torch.int32, device="cuda")
+    sp.indicator_dev(Q, codes)
+    model = gpuprm_amd.Surrogate(ndof=6, batch=B, seed=3)
+    ntr = N - ntest
+    flop = 2 * (6 * 256 + 2 * 256 * 256 + 256)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    steps = 0
+    for ep in range(epochs):
+        for it in range(ntr // B):
+            model.train_step(Q[:, it * B:(it + 1) * B], codes[it * B:(it + 1) * B],
+                             lr=4e-3 if ep < epochs - 1 else 1e-3, offset=N, want_loss=False)
+            steps += 1
+    e1.record()
+    torch.cuda.synchronize()
+    t_train = e0.elapsed_time(e1) / 1000.0
+    pt = torch.zeros(ntest, dtype=torch.float32, device="cuda")
+    model.predict(Q[:, ntr:].contiguous(), pt)
+    auc = mlp.auc(pt.cpu().numpy(), codes[ntr:].cpu().numpy() != 0)
+    P = torch.zeros(N, dtype=torch.float32, device="cuda")
+    model.predict(Q, P)
+    ts = []
+    for _ in range(5):
+        e0.record()
+        model.predict(Q, P)
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1) / 1000.0)
+    t_pred = float(np.median(ts))
+    train_tf = steps * B * 3 * flop / t_train / 1e12
+    pred_tf = N * flop / t_pred / 1e12
+    return {"workload": "6-DOF arm scene: 4,194,304 exact labels; model 6 -> 256 -> 256 -> 256 -> 1, BF16 MFMA",
+            "train": {"batch": B, "steps": steps, "ms_per_step": 1000 * t_train / steps,
+                      "roofline": {"bound": "mfma", "pipe": "bf16 mfma", "unit": "TFLOP/s", "peak": PEAK_BF16_TFLOPS,
+                                   "achieved": round(train_tf, 2), "frac": round(train_tf / PEAK_BF16_TFLOPS, 4),
+                                   "flops_per_sample": 3 * flop}},
+            "predict": {"configs": N, "ms": 1000 * t_pred, "configs_per_s": N / t_pred,
+                        "roofline": {"bound": "mfma", "pipe": "bf16 mfma", "unit": "TFLOP/s",
+                                     "peak": PEAK_BF16_TFLOPS, "achieved": round(pred_tf, 2),
+                                     "frac": round(pred_tf / PEAK_BF16_TFLOPS, 4), "flops_per_config": flop}},
+            "auc_heldout": auc}
+
+
 def leg_prm():
     """Time-to-path of the single-GPU planner on the 7-DOF scene (32 hulls of
     16 points, radius 0.5, 8192 new nodes per step): the three tasks of the
@@ -640,9 +707,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--verify-edges", type=int, default=3000)
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the point-check, multi-scene and planner legs")
-    ap.add_argument("--extras", default="points,multi,prm",
-                    help="comma-separated legs after the headline: points, multi, prm")
+                    help="skip the point-check, multi-scene, MLP and planner legs")
+    ap.add_argument("--extras", default="points,multi,mlp,prm",
+                    help="comma-separated legs after the headline: points, multi, mlp, prm")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the timed steps, run a plain elementwise kernel that reads and writes "
                          "MIB MiB once each (the known-size kernel that scales HBM byte counters)")
@@ -803,6 +870,9 @@ def main():
     if "multi" in extras:
         log("multi-scene batch")
         line["multi_scene"] = leg_multi_scene(10)
+    if "mlp" in extras:
+        log("surrogate MLP kernels")
+        line["mlp"] = leg_mlp()
     if "prm" in extras:
         log("planner")
         line["prm"] = leg_prm()
