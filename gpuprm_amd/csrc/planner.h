@@ -75,20 +75,21 @@ hipError_t launch_plan_sample(int ndof, int batch, PlanTree t0, PlanTree t1, int
 hipError_t launch_plan_bias_mask(const float* prob, int batch, unsigned char* skip, hipStream_t st);
 // New nodes: in slot order, the free (code 0), unskipped candidates, at most
 // want0 of tree 0 and want1 of tree 1, into qnew (SoA, stride num);
-// info->nnew, info->accepted; state point_checks /
+// info->nnew, info->accepted (info->E_all reset); state point_checks /
 // dropped.  tile: 3 ints per 1024 candidates.
 hipError_t launch_plan_select(int ndof, int batch, const int* codes, const unsigned char* skip, const float* cand,
                               int want0, int want1, int num, float* qnew, int* tile, StepInfo* info, PlanState* state,
                               hipStream_t st);
 // Radius search of the step's new nodes in both trees, one group b = 2j + g
 // per (new node j, tree g): every (node, squared distance, group) with
-// distance^2 < D2 (the reference's get_near_vertices float sum), in group
-// order, group b's at off[b]; the first nbuf of them in that order are
-// stored (enode, ed2, egroup) and cnt[b] = the group's stored edges.
-// info->E_all = all matches, info->E = min(E_all, nbuf).  With budget B > 0
-// also sc_off = exclusive scan of the stored edges of the groups with more
-// than B (the groups the model scores) and info->S their total.  off and
-// sc_off have groups + 1 entries.
+// distance^2 < D2 (the reference's get_near_vertices float sum) is a match;
+// up to nbuf of them are stored (enode, ed2, egroup), cnt[b] = the group's
+// stored edges, info->E_all = all matches, info->E = min(E_all, nbuf).
+// With budget B > 0 the matches are in group order (group b's at off[b];
+// the first nbuf in that order are stored), and sc_off = exclusive scan of
+// the stored edges of the groups with more than B (the groups the model
+// scores), info->S their total; off and sc_off have groups + 1 entries.
+// Without a budget they are in arrival order (off and sc_off unused).
 hipError_t launch_plan_near(int ndof, int num, const float* qnew, StepInfo* info, PlanTree t0, PlanTree t1, int cap,
                             const CellGrid& g, float D2, int* cnt, int* off, int* sc_off, int nbuf, int budget,
                             int* enode, float* ed2, int* egroup, hipStream_t st);

@@ -210,6 +210,7 @@ __global__ void __launch_bounds__(kScanBlock)
   if (threadIdx.x == 0) {
     info->nnew = min(tot[0], want0) + min(tot[1], want1);
     info->accepted = tot[0] + tot[1];
+    info->E_all = 0;  // the radius search's edge counter (its appending pass)
     atomicAdd(&state->point_checks, (unsigned long long)(batch - tot[2]));
     atomicAdd(&state->dropped, (unsigned long long)tot[2]);
   }
@@ -248,12 +249,21 @@ static_assert(kNeighbourCells <= 256, "one thread per neighbour cell in a 256-th
 // WRITE stores the group's first cnt[b] matches, in traversal order, at
 // off[b].  Edges are thus in group order and the kept ones (when the step
 // has more than nbuf) do not depend on scheduling.
-template <bool WRITE>
+//
+// APPEND (no edge budget, so no group order needed): one pass; each chunk of
+// 256 candidates takes its matches' slots from info->E_all with one atomic
+// per block (after a block scan), the first nbuf slots are stored, cnt[b]
+// gets the group's stored edges.  On sparse trees (few matches per group)
+// one traversal beats two; on dense ones the per-block atomics stay few.
+enum NearMode { kNearCount = 0, kNearWrite = 1, kNearAppend = 2 };
+
+template <int MODE>
 __global__ void __launch_bounds__(256)
-    plan_near_kernel(int ndof, int num, const float* __restrict__ qnew, const StepInfo* __restrict__ info,
+    plan_near_kernel(int ndof, int num, const float* __restrict__ qnew, StepInfo* __restrict__ info,
                      PlanTree t0, PlanTree t1, int cap, CellGrid g, float D2, int* __restrict__ cnt,
-                     const int* __restrict__ off, int* __restrict__ enode, float* __restrict__ ed2,
+                     const int* __restrict__ off, int nbuf, int* __restrict__ enode, float* __restrict__ ed2,
                      int* __restrict__ egroup) {
+  constexpr bool WRITE = MODE == kNearWrite, APPEND = MODE == kNearAppend;
   constexpr int NK = kNeighbourCells;
   __shared__ int s_start[NK], s_pre[NK + 1];
   __shared__ int lds[kMaxWaves + 1];
@@ -263,6 +273,8 @@ __global__ void __launch_bounds__(256)
     if (!WRITE && threadIdx.x == 0) cnt[b] = 0;
     return;
   }
+  __shared__ int s_base, s_chunk[2];
+  int chunk_no = 0;  // APPEND: chunks done (the same in every thread)
   const int keep = WRITE ? cnt[b] : 0;
   if (WRITE && keep == 0) return;
   const PlanTree& T = tree_of(t0, t1, b & 1);
@@ -303,6 +315,7 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) {
     s_pre[K] = total;
     s_count = 0;
+    s_chunk[0] = 0;
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -335,6 +348,32 @@ __global__ void __launch_bounds__(256)
       }
       run += chunk;
       if (run >= keep) break;  // uniform: run and keep are the same in every thread
+    } else if (APPEND) {
+      // each wave's matches take a block-local range (LDS atomic), the
+      // block's total one global range; the two counters alternate between
+      // chunks, so each is cleared a chunk before it is used again
+      const unsigned long long m = __ballot(hit);
+      int* const cur = &s_chunk[chunk_no & 1];
+      int wl = 0;
+      if (lane == 0 && m) wl = atomicAdd(cur, __popcll(m));
+      if (threadIdx.x == 0) s_chunk[(chunk_no + 1) & 1] = 0;
+      __syncthreads();
+      const int tot = *cur;
+      if (threadIdx.x == 0 && tot > 0) {
+        s_base = atomicAdd(&info->E_all, tot);
+        s_count += max(0, min(tot, nbuf - s_base));
+      }
+      __syncthreads();
+      ++chunk_no;
+      const int wbase = __shfl(wl, 0, 64);  // all lanes take part
+      if (hit) {
+        const int o = s_base + wbase + __popcll(m & ((1ull << lane) - 1ull));
+        if (o < nbuf) {
+          enode[o] = node;
+          ed2[o] = normsq;
+          egroup[o] = b;
+        }
+      }
     } else {
       const unsigned long long m = __ballot(hit);
       if (lane == 0 && m) atomicAdd(&s_count, __popcll(m));
@@ -343,6 +382,15 @@ __global__ void __launch_bounds__(256)
   if (!WRITE) {
     __syncthreads();
     if (threadIdx.x == 0) cnt[b] = s_count;
+  }
+}
+
+// After an APPEND pass: info->E = min(E_all, nbuf), no scored groups.
+__global__ void plan_near_append_done(int nbuf, StepInfo* info) {
+  if (threadIdx.x == 0) {
+    const int all = info->E_all;
+    info->E = all < nbuf ? all : nbuf;
+    info->S = 0;
   }
 }
 
@@ -815,12 +863,19 @@ hipError_t launch_plan_near(int ndof, int num, const float* qnew, StepInfo* info
                             const CellGrid& g, float D2, int* cnt, int* off, int* sc_off, int nbuf, int budget,
                             int* enode, float* ed2, int* egroup, hipStream_t st) {
   const int groups = 2 * num;
-  hipLaunchKernelGGL(plan_near_kernel<false>, dim3(groups), dim3(256), 0, st, ndof, num, qnew, info, t0, t1, cap, g,
-                     D2, cnt, off, enode, ed2, egroup);
+  if (budget <= 0) {
+    // info->E_all was reset by the step's selection
+    hipLaunchKernelGGL(plan_near_kernel<kNearAppend>, dim3(groups), dim3(256), 0, st, ndof, num, qnew, info, t0, t1,
+                       cap, g, D2, cnt, off, nbuf, enode, ed2, egroup);
+    hipLaunchKernelGGL(plan_near_append_done, dim3(1), dim3(64), 0, st, nbuf, info);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL(plan_near_kernel<kNearCount>, dim3(groups), dim3(256), 0, st, ndof, num, qnew, info, t0, t1,
+                     cap, g, D2, cnt, off, nbuf, enode, ed2, egroup);
   hipLaunchKernelGGL(plan_scan_groups_kernel, dim3(1), dim3(kScanBlock), 0, st, groups, cnt, nbuf, budget, off,
                      sc_off, info);
-  hipLaunchKernelGGL(plan_near_kernel<true>, dim3(groups), dim3(256), 0, st, ndof, num, qnew, info, t0, t1, cap, g,
-                     D2, cnt, off, enode, ed2, egroup);
+  hipLaunchKernelGGL(plan_near_kernel<kNearWrite>, dim3(groups), dim3(256), 0, st, ndof, num, qnew, info, t0, t1,
+                     cap, g, D2, cnt, off, nbuf, enode, ed2, egroup);
   return hipGetLastError();
 }
 
