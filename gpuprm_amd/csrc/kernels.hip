@@ -423,119 +423,6 @@ __global__ void __launch_bounds__(kFkBlock) fk_edges_multi_kernel(const EdgeTask
   if (i < n) fk_edge_item<NDOF>(T.rb, T.ea, T.item_edge, T.item_pos, i, T.frames, (long long)T.cap);
 }
 
-// Planner candidates: slot k of a batch (tree k & 1) is node base[k] of its
-// tree (SoA tree0 / tree1, stride cap) plus an offset uniform in [-D, D) per
-// coordinate, q - D + 2 D u.  The u come from a counter-based generator
-// (splitmix64 of key + k ndof + i, top 53 bits), so a batch is a function of
-// its key.
-__device__ __forceinline__ double counter_uniform(unsigned long long key, unsigned long long ctr) {
-  unsigned long long z = key + (ctr + 1ull) * 0x9e3779b97f4a7c15ull;
-  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
-  z ^= z >> 31;
-  return (double)(z >> 11) * 0x1.0p-53;
-}
-__global__ void __launch_bounds__(256)
-    sample_candidates_kernel(int ndof, int batch, const int* __restrict__ base, const float* __restrict__ tree0,
-                             const float* __restrict__ tree1, long long cap, float D, unsigned long long key,
-                             float* __restrict__ cand) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= batch) return;
-  const float* t = (k & 1) ? tree1 : tree0;
-  const int m = base[k];
-  for (int i = 0; i < ndof; ++i) {
-    const float u = (float)counter_uniform(key, (unsigned long long)k * ndof + i);
-    cand[k + (size_t)batch * i] = t[m + cap * i] - D + 2 * D * u;
-  }
-}
-
-// Points scored along planner edges: for candidate a (new node idx[a], tree
-// idx[n + a], node idx[2n + a]) and s < S, the point qa + t (qb - qa) with
-// t = (s + 1) / (S + 1), into out (SoA, stride n S, column a S + s).
-__global__ void __launch_bounds__(256)
-    score_points_kernel(int ndof, int n, int S, const int* __restrict__ idx, const float* __restrict__ qnew, int nnew,
-                        const float* __restrict__ tree0, const float* __restrict__ tree1, long long cap,
-                        float* __restrict__ out) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= n * S) return;
-  const int a = x / S, s = x - a * S;
-  const int j = idx[a], g = idx[n + a], m = idx[2 * (size_t)n + a];
-  const float* t = g ? tree1 : tree0;
-  const float w = (float)(s + 1) / (float)(S + 1);
-  const size_t np = (size_t)n * S;
-  for (int i = 0; i < ndof; ++i) {
-    const float qa = qnew[j + (size_t)nnew * i], qb = t[m + cap * i];
-    out[x + np * i] = qa + w * (qb - qa);
-  }
-}
-
-// Columns idx[0..n) of the SoA array src (stride ns) into dst (stride n).
-__global__ void __launch_bounds__(256)
-    gather_columns_kernel(int ndof, int n, const int* __restrict__ idx, const float* __restrict__ src, int ns,
-                          float* __restrict__ dst) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= n) return;
-  const int k = idx[a];
-  for (int i = 0; i < ndof; ++i) dst[a + (size_t)n * i] = src[k + (size_t)ns * i];
-}
-
-// Planner edges: edge e joins the step's new node enew[e] (SoA qnew, stride
-// nnew) and node enode[e] of tree etree[e] (SoA tree0 / tree1, stride cap);
-// both endpoints are copied into the SoA edge arrays qs, qe (stride E).
-__global__ void __launch_bounds__(256)
-    gather_edges_kernel(int ndof, int E, const int* __restrict__ eidx, const float* __restrict__ qnew, int nnew,
-                        const float* __restrict__ tree0, const float* __restrict__ tree1, long long cap,
-                        float* __restrict__ qs, float* __restrict__ qe) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= E) return;
-  const int j = eidx[e], g = eidx[E + e], m = eidx[2 * (size_t)E + e];
-  const float* t = g ? tree1 : tree0;
-  for (int i = 0; i < ndof; ++i) {
-    qs[e + (size_t)E * i] = qnew[j + (size_t)nnew * i];
-    qe[e + (size_t)E * i] = t[m + cap * i];
-  }
-}
-
-// Radius neighbours (get_near_vertices, prm/src/lib/prm/prmsolver.cpp:
-// 2532-2574): for query j and stored node l, |q_l - query_j|^2 < D^2 in float
-// with the reference's summation order.  One block per query; matches are
-// appended (query, node, distance) with one atomic per wave.
-__global__ void __launch_bounds__(256)
-    near_kernel(int ndof, const float* __restrict__ queries, int nq, long long qoff,
-                const float* __restrict__ nodes, int nn, long long noff, float D2, int* __restrict__ out_q,
-                int* __restrict__ out_node, float* __restrict__ out_d2, int* __restrict__ count, int cap) {
-  const int j = blockIdx.x;
-  if (j >= nq) return;
-  float qr[kMaxDofDev];
-  for (int i = 0; i < ndof; ++i) qr[i] = queries[j + qoff * i];
-  const int lane = threadIdx.x & 63;
-  for (int base = 0; base < nn; base += blockDim.x) {
-    const int l = base + threadIdx.x;
-    float normsq = 0.0f;
-    bool hit = false;
-    if (l < nn) {
-      for (int i = 0; i < ndof; ++i) {
-        float diff = nodes[l + noff * i] - qr[i];
-        normsq += diff * diff;
-      }
-      hit = normsq < D2;
-    }
-    unsigned long long m = __ballot(hit);
-    if (m == 0ull) continue;
-    int b = 0;
-    if (lane == 0) b = atomicAdd(count, __popcll(m));
-    b = __shfl(b, 0, 64);
-    if (hit) {
-      int o = b + __popcll(m & ((1ull << lane) - 1ull));
-      if (o < cap) {
-        out_q[o] = j;
-        out_node[o] = l;
-        out_d2[o] = normsq;
-      }
-    }
-  }
-}
-
 // ---- cell index for the radius search -------------------------------------
 // Nodes are binned by their first g.dims coordinates into cells at least D
 // wide, so every node within D of a query lies in the 3^dims cells around the
@@ -597,10 +484,12 @@ constexpr int pow3(int d) { return d == 0 ? 1 : 3 * pow3(d - 1); }
 constexpr int kNeighbourCells = pow3(kGridDims);
 static_assert(kNeighbourCells <= 256, "one thread per neighbour cell in a 256-thread block");
 
-// Radius search over the cell index: one block per query; the nodes of the
-// (up to 3^kGridDims) neighbouring cells form one index range that the
-// block's threads stride through.  Same squared distance (same summation
-// order) and same append as near_kernel.
+// Radius search over the cell index (get_near_vertices,
+// prm/src/lib/prm/prmsolver.cpp:2532-2574): one block per query; the nodes
+// of the (up to 3^kGridDims) neighbouring cells form one index range that the
+// block's threads stride through.  |q_l - query_j|^2 < D^2 in float with the
+// reference's summation order; matches (query, node, distance) are appended
+// with one atomic per wave.
 __global__ void __launch_bounds__(256)
     near_grid_kernel(int ndof, const float* __restrict__ queries, int nq, long long qoff, CellGrid g,
                      const int* __restrict__ starts, const int* __restrict__ counts, const float* __restrict__ sorted,
@@ -1728,47 +1617,6 @@ hipError_t launch_edge_fk_multi(const EdgeTask* d_tasks, int ntasks, int nblocks
     case 7: hipLaunchKernelGGL(fk_edges_multi_kernel<7>, grid, block, 0, st, d_tasks, ntasks); break;
     default: hipLaunchKernelGGL(fk_edges_multi_kernel<0>, grid, block, 0, st, d_tasks, ntasks); break;
   }
-  return hipGetLastError();
-}
-
-hipError_t launch_sample_candidates(int ndof, int batch, const int* base, const float* tree0, const float* tree1,
-                                    long long cap, float D, unsigned long long key, float* cand, hipStream_t st) {
-  if (batch <= 0) return hipSuccess;
-  hipLaunchKernelGGL(sample_candidates_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, ndof, batch, base, tree0,
-                     tree1, cap, D, key, cand);
-  return hipGetLastError();
-}
-
-hipError_t launch_score_points(int ndof, int n, int S, const int* idx, const float* qnew, int nnew, const float* tree0,
-                               const float* tree1, long long cap, float* out, hipStream_t st) {
-  if (n <= 0 || S <= 0) return hipSuccess;
-  const long long total = (long long)n * S;
-  hipLaunchKernelGGL(score_points_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, ndof, n, S, idx, qnew,
-                     nnew, tree0, tree1, cap, out);
-  return hipGetLastError();
-}
-
-hipError_t launch_gather_columns(int ndof, int n, const int* idx, const float* src, int ns, float* dst,
-                                 hipStream_t st) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(gather_columns_kernel, dim3((n + 255) / 256), dim3(256), 0, st, ndof, n, idx, src, ns, dst);
-  return hipGetLastError();
-}
-
-hipError_t launch_gather_edges(int ndof, int E, const int* eidx, const float* qnew, int nnew, const float* tree0,
-                               const float* tree1, long long cap, float* qs, float* qe, hipStream_t st) {
-  if (E <= 0) return hipSuccess;
-  hipLaunchKernelGGL(gather_edges_kernel, dim3((E + 255) / 256), dim3(256), 0, st, ndof, E, eidx, qnew, nnew, tree0,
-                     tree1, cap, qs, qe);
-  return hipGetLastError();
-}
-
-hipError_t launch_near(int ndof, const float* queries, int nq, long long qoff, const float* nodes, int nn,
-                       long long noff, float D2, int* out_q, int* out_node, float* out_d2, int* count, int cap,
-                       hipStream_t st) {
-  if (nq <= 0 || nn <= 0) return hipSuccess;
-  hipLaunchKernelGGL(near_kernel, dim3(nq), dim3(256), 0, st, ndof, queries, nq, qoff, nodes, nn, noff, D2, out_q,
-                     out_node, out_d2, count, cap);
   return hipGetLastError();
 }
 

@@ -224,35 +224,13 @@ CellGrid make_cell_grid(int ndof, const float* mins, const float* maxs, float D)
 // sorted coordinates (SoA, stride n) and node ids (n).
 hipError_t launch_grid_build(const CellGrid& g, int ndof, const float* nodes, int n, long long noff, int* cell_id,
                              int* counts, int* starts, int* cursor, float* sorted, int* sid, hipStream_t st);
-// launch_near over the cell index (same pairs; append order may differ).
+// Radius neighbours over the cell index: every (query j, node l) with
+// |node_l - query_j|^2 < D2, appended to (out_q, out_node, out_d2) up to cap
+// entries; *count receives the total number of matches (may exceed cap).
 hipError_t launch_near_grid(int ndof, const float* queries, int nq, long long qoff, const CellGrid& g,
                             const int* starts, const int* counts, const float* sorted, const int* sid, int n, float D2,
                             int* out_q, int* out_node, float* out_d2, int* count, int cap, hipStream_t st);
-// Radius neighbours: every (query j, node l) with |node_l - query_j|^2 < D2,
-// appended to (out_q, out_node, out_d2) up to cap entries; *count receives the
-// total number of matches (may exceed cap).
-hipError_t launch_near(int ndof, const float* queries, int nq, long long qoff, const float* nodes, int nn,
-                       long long noff, float D2, int* out_q, int* out_node, float* out_d2, int* count, int cap,
-                       hipStream_t st);
 hipError_t launch_fill_u32(unsigned int* p, unsigned int v, long long n, hipStream_t st);
-// Planner candidates: slot k (tree k & 1) = node base[k] of its tree (SoA,
-// stride cap) plus a uniform offset in [-D, D) per coordinate from a
-// counter-based generator keyed by key; cand is SoA with stride batch.
-hipError_t launch_sample_candidates(int ndof, int batch, const int* base, const float* tree0, const float* tree1,
-                                    long long cap, float D, unsigned long long key, float* cand, hipStream_t st);
-// Points scored along planner edges: candidate a (new node idx[a], tree
-// idx[n + a], node idx[2n + a]), s < S: qa + t (qb - qa), t = (s+1)/(S+1),
-// into out (SoA, stride n S, column a S + s).
-hipError_t launch_score_points(int ndof, int n, int S, const int* idx, const float* qnew, int nnew, const float* tree0,
-                               const float* tree1, long long cap, float* out, hipStream_t st);
-// Columns idx[0..n) of the SoA array src (stride ns) into dst (stride n).
-hipError_t launch_gather_columns(int ndof, int n, const int* idx, const float* src, int ns, float* dst,
-                                 hipStream_t st);
-// Planner edge endpoints: eidx holds E new-node indices, then E tree
-// indices (0/1), then E node indices; qnew is SoA with stride nnew, the trees
-// SoA with stride cap; qs, qe receive SoA with stride E.
-hipError_t launch_gather_edges(int ndof, int E, const int* eidx, const float* qnew, int nnew, const float* tree0,
-                               const float* tree1, long long cap, float* qs, float* qe, hipStream_t st);
 
 // Launch shape of the persistent separating-vector kernel: a kernel variant
 // (block size, occupancy target in waves per SIMD), the pairs of one work
