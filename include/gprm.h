@@ -181,7 +181,7 @@ typedef struct {
   int bias_sampling;        /* with a model: every second pair of candidates must be
                                predicted free (p < 0.5); the rest are dropped unchecked */
   int order_edges;          /* with a model and edge_budget: edges the model predicts
-                               blocked (max probability at 3 interior points above 1/2)
+                               blocked (probability at the edge's midpoint above 1/2)
                                rank after the others */
   int edge_budget;          /* 0: check every candidate edge (the reference's rule);
                                k > 0: at most k per (new node, tree), shortest first
@@ -208,7 +208,9 @@ typedef struct {
   double ms_sampling, ms_near, ms_edges, ms_insert;
   double ms_model;               /* model inference inside the steps (device time) */
   double ms_wait;                /* host time blocked on the steps' read-backs */
-  double ms_final;               /* trees read back, Dijkstra, path */
+  double ms_final;               /* shortest paths on the device (Bellman-Ford rounds over
+                                    the edge lists), the path read back; with output_dir
+                                    also the trees */
 } gprm_prm_result;
 int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* qgoal, const gprm_prm_params* params,
                    gprm_prm_result* res, float* path, int path_cap);
@@ -220,7 +222,8 @@ int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* qgoal, const
  * [k + off*i], off >= count); mins/maxs (host, ndof) bound the nodes.  Up to
  * cap pairs go to the device arrays d_out_* in no particular order; *total
  * receives the number of pairs found (may exceed cap).  Through a cell index
- * over the first three coordinates instead of the reference's q[0] blocks. */
+ * over the first coordinates (up to five) instead of the reference's q[0]
+ * blocks. */
 int gprm_radius_neighbours(int ndof, const float* d_nodes, int n, long long noff, const float* d_queries, int nq,
                            long long qoff, const float* mins, const float* maxs, float D, int* d_out_q,
                            int* d_out_node, float* d_out_d2, int cap, long long* total, void* stream);
@@ -238,6 +241,9 @@ int gprm_surrogate_get_params(void* model, float* out);
 int gprm_surrogate_set_params(void* model, const float* in);
 int gprm_surrogate_gradients(void* model, const float* d_q, long long offset, const int* d_codes, int n,
                              float* grads_out, float* loss_out);
+/* One Adam step on n <= batch rows; *loss_out gets the batch loss before the
+ * step.  With loss_out NULL the step is only enqueued (on the null stream):
+ * synchronise before reading the parameters. */
 int gprm_surrogate_train_step(void* model, const float* d_q, long long offset, const int* d_codes, int n, float lr,
                               float* loss_out);
 int gprm_surrogate_predict(void* model, const float* d_q, int n, long long offset, float* d_prob, void* stream);
