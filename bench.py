@@ -594,9 +594,10 @@ def leg_prm():
     16 points, radius 0.5, 8192 new nodes per step): the three tasks of the
     earlier measurements and six more, one worker (steps in issue order) and
     the reference's two-worker scheme; the configuration as written (200,000
-    uniform samples); and plain vs surrogate on the 6-DOF edge-checking scene
-    with a model trained on 4M exact labels, 16 tasks.  Every path is
-    re-verified by the oracle."""
+    uniform samples); plain vs surrogate on the 7-DOF scene (a model trained
+    on 4M exact labels of it, the same 9 tasks); and plain vs surrogate on the
+    6-DOF edge-checking scene with a model trained on 4M exact labels, 16
+    tasks.  Every path is re-verified by the oracle."""
     import torch
 
     import gpuprm_amd
@@ -621,6 +622,37 @@ def leg_prm():
         "all_paths_valid": all(x["valid"] for x in runs + runs2),
         "runs": runs, "runs_workers2": runs2}
     out["7dof_uniform_as_written"] = leg_uniform_prm(sp3, o3, *tasks[0])
+
+    # the model on the 7-DOF scene too: 4M exact labels, the same 9 tasks
+    rng7 = np.random.default_rng(SEED + 19)
+    N7, ntest7, B7, ep7 = 1 << 22, 1 << 16, 32768, 8
+    Q7 = torch.from_numpy(rng7.uniform(-np.pi, np.pi, size=(7, N7)).astype(np.float32)).cuda()
+    codes7 = torch.zeros(N7, dtype=torch.int32, device="cuda")
+    sp3.indicator_dev(Q7, codes7)
+    model7 = gpuprm_amd.Surrogate(ndof=7, batch=B7, seed=3)
+    ntr7 = N7 - ntest7
+    for ep in range(ep7):
+        for it in range(ntr7 // B7):
+            model7.train_step(Q7[:, it * B7:(it + 1) * B7], codes7[it * B7:(it + 1) * B7],
+                              lr=4e-3 if ep < ep7 - 1 else 1e-3, offset=N7, want_loss=False)
+    pt7 = torch.zeros(ntest7, dtype=torch.float32, device="cuda")
+    model7.predict(Q7[:, ntr7:].contiguous(), pt7)
+    auc7 = mlp.auc(pt7.cpu().numpy(), codes7[ntr7:].cpu().numpy() != 0)
+    modes7 = {"surrogate": {"surrogate": model7, "order_edges": True, "bias_sampling": True, "edge_budget": 4},
+              "budget_shortest_no_model": {"edge_budget": 4}}
+    res7 = {"plain": runs}
+    for k, kw in modes7.items():
+        res7[k] = [_prm_run(sp3, o3, a, b, seed=i, workers=1, **kw3, **kw) for i, (a, b) in enumerate(tasks)]
+    med7 = {k: float(np.median([x["ms"] for x in v])) for k, v in res7.items()}
+    out["surrogate_vs_plain_7dof"] = {
+        "workload": "7-DOF arm, 32 hulls of 16 points, D=0.5, 8192 new nodes/step, the 9 tasks above",
+        "auc": auc7, "collision_rate": float((codes7 != 0).float().mean().item()),
+        "ms_median": med7, "speedup": med7["plain"] / med7["surrogate"],
+        "speedup_budget_no_model": med7["plain"] / med7["budget_shortest_no_model"],
+        "edge_samples_saved": 1 - sum(x["edge_samples"] for x in res7["surrogate"]) /
+        max(1, sum(x["edge_samples"] for x in res7["plain"])),
+        "all_paths_valid": all(x["valid"] for v in res7.values() for x in v),
+        "runs": {k: v for k, v in res7.items() if k != "plain"}}
 
     d2 = tempfile.mkdtemp(prefix="gprm_prm2_")
     gpuprm_amd.generate_scene(2, SEED, d2)
@@ -666,7 +698,7 @@ def leg_prm():
         res = {k: [] for k in modes}
         for i, (a, b) in enumerate(tasks2):
             for k, kw in modes.items():
-                res[k].append(_prm_run(sp2, o2, a, b, verify=(num == 1024), num=num, D=1.0, max_nodes=1 << 18,
+                res[k].append(_prm_run(sp2, o2, a, b, num=num, D=1.0, max_nodes=1 << 18,
                                        max_steps=300, seed=i, **kw))
         med = {k: float(np.median([x["ms"] for x in v])) for k, v in res.items()}
         phases = {k: {ph: float(np.median([x["ms_device"][ph] for x in v])) for ph in v[0]["ms_device"]}
@@ -678,8 +710,7 @@ def leg_prm():
                       "point_checks_saved": 1 - sum(x["point_checks"] for x in res["surrogate"]) /
                       max(1, sum(x["point_checks"] for x in res["plain"])),
                       "all_found": all(x["found"] for v in res.values() for x in v),
-                      "all_paths_valid": all(x["valid"] for v in res.values() for x in v)
-                      if num == 1024 else None,
+                      "all_paths_valid": all(x["valid"] for v in res.values() for x in v),
                       "runs": res}
     best = {k: min(sweep[n]["ms_median"][k] for n in sweep) for k in modes}
     out["surrogate_vs_plain"] = {

@@ -85,11 +85,13 @@ hipError_t launch_plan_select(int ndof, int batch, const int* codes, const unsig
 // distance^2 < D2 (the reference's get_near_vertices float sum) is a match;
 // up to nbuf of them are stored (enode, ed2, egroup), cnt[b] = the group's
 // stored edges, info->E_all = all matches, info->E = min(E_all, nbuf).
-// With budget B > 0 the matches are in group order (group b's at off[b];
-// the first nbuf in that order are stored), and sc_off = exclusive scan of
-// the stored edges of the groups with more than B (the groups the model
-// scores), info->S their total; off and sc_off have groups + 1 entries.
-// Without a budget they are in arrival order (off and sc_off unused).
+// With budget B > 0 a group keeps at most min(512, nbuf / groups) of its
+// matches, its shortest (ties by traversal order), so a step never has
+// more than nbuf; they are contiguous at off[b] (groups placed as their
+// blocks finish), and sc_off = exclusive scan of the stored edges of the
+// groups with more than B (the groups the model scores), info->S its
+// total; sc_off has groups + 1 entries.  Without a budget the matches are
+// in arrival order (off and sc_off unused).
 hipError_t launch_plan_near(int ndof, int num, const float* qnew, StepInfo* info, PlanTree t0, PlanTree t1, int cap,
                             const CellGrid& g, float D2, int* cnt, int* off, int* sc_off, int nbuf, int budget,
                             int* enode, float* ed2, int* egroup, hipStream_t st);

@@ -18,6 +18,8 @@
 // tree) group, so every group's edges are contiguous and in a fixed order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "planner.h"
 
 namespace gprm {
@@ -243,40 +245,84 @@ static_assert(kNeighbourCells <= 256, "one thread per neighbour cell in a 256-th
 
 // One block per group b = 2 j + g: the nonempty cells among the 3^dims
 // around new node j are compacted (block scans) and the block strides
-// through their nodes in cell order.  Two passes, no atomics on a shared
-// counter: COUNT stores the group's matches in cnt[b]; after the scan
-// (plan_scan_groups_kernel: off[b], cnt[b] clipped to the nbuf slots),
-// WRITE stores the group's first cnt[b] matches, in traversal order, at
-// off[b].  Edges are thus in group order and the kept ones (when the step
-// has more than nbuf) do not depend on scheduling.
+// through their nodes in cell order, 256 candidates a chunk.  One pass in
+// either mode.
 //
-// APPEND (no edge budget, so no group order needed): one pass; each chunk of
-// 256 candidates takes its matches' slots from info->E_all with one atomic
-// per block (after a block scan), the first nbuf slots are stored, cnt[b]
-// gets the group's stored edges.  On sparse trees (few matches per group)
-// one traversal beats two; on dense ones the per-block atomics stay few.
-enum NearMode { kNearCount = 0, kNearWrite = 1, kNearAppend = 2 };
+// APPEND (no edge budget, so no group order needed): each chunk's matches
+// take their slots from info->E_all with one atomic per block (waves take
+// block-local ranges by LDS atomic), the first nbuf slots are stored,
+// cnt[b] gets the group's stored edges.
+//
+// GROUP (edge budget: the ranking needs each group's edges contiguous):
+// the matches are staged in LDS with their traversal index (per-wave
+// ballots, one barrier a chunk); a group keeps at most `keep` of them, its
+// shortest (ties by traversal index), keep = min(kGroupKeep, nbuf / groups),
+// so a step never has more than nbuf and the kept edges do not depend on
+// scheduling.  When the stage would overflow, and at the end when more than
+// keep are staged, a bitonic sort in LDS orders the stage and cuts it to
+// keep.  At the end the block takes one range of info->E_all and copies its
+// edges out: group b's are contiguous at off[b] (in traversal order, or
+// sorted when the group was cut).
+enum NearMode { kNearAppend = 0, kNearGroup = 1 };
+// 12 KB of stage: eight blocks of four waves still fit a CU's LDS (the
+// search is latency-bound, so occupancy counts)
+constexpr int kGroupStage = 1024;
+constexpr int kGroupKeep = kGroupStage / 2;
+
+// Ascending bitonic sort of the first n stage entries by (squared distance,
+// traversal index), padded to a power of two with +inf; n <= kGroupStage.
+__device__ void stage_sort(float* d2, int* idx, int* node, int n) {
+  int P = 1;
+  while (P < n) P <<= 1;
+  for (int i = n + (int)threadIdx.x; i < P; i += blockDim.x) {
+    d2[i] = __builtin_inff();
+    idx[i] = 0x7fffffff;
+  }
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = threadIdx.x; t < P / 2; t += blockDim.x) {
+        const int a = ((t & ~(j - 1)) << 1) | (t & (j - 1)), b = a | j;  // a has bit j clear
+        const float da = d2[a], db = d2[b];
+        const int ia = idx[a], ib = idx[b];
+        const bool gt = da > db || (da == db && ia > ib);
+        if (gt == ((a & k) == 0)) {
+          d2[a] = db;
+          d2[b] = da;
+          idx[a] = ib;
+          idx[b] = ia;
+          const int na = node[a];
+          node[a] = node[b];
+          node[b] = na;
+        }
+      }
+      __syncthreads();
+    }
+}
 
 template <int MODE>
 __global__ void __launch_bounds__(256)
     plan_near_kernel(int ndof, int num, const float* __restrict__ qnew, StepInfo* __restrict__ info,
                      PlanTree t0, PlanTree t1, int cap, CellGrid g, float D2, int* __restrict__ cnt,
-                     const int* __restrict__ off, int nbuf, int* __restrict__ enode, float* __restrict__ ed2,
-                     int* __restrict__ egroup) {
-  constexpr bool WRITE = MODE == kNearWrite, APPEND = MODE == kNearAppend;
+                     int* __restrict__ off, int nbuf, int keep, int* __restrict__ enode,
+                     float* __restrict__ ed2, int* __restrict__ egroup) {
+  constexpr bool GROUP = MODE == kNearGroup;
   constexpr int NK = kNeighbourCells;
+  constexpr int kWaves = 256 / 64;
   __shared__ int s_start[NK], s_pre[NK + 1];
   __shared__ int lds[kMaxWaves + 1];
   __shared__ int s_count;
+  __shared__ int s_base, s_chunk[2];
+  __shared__ int s_wcnt[2][kWaves];
+  __shared__ int s_node[GROUP ? kGroupStage : 1];
+  __shared__ float s_d2[GROUP ? kGroupStage : 1];
+  __shared__ int s_idx[GROUP ? kGroupStage : 1];
   const int b = blockIdx.x, j = b >> 1;
   if (j >= info->nnew) {
-    if (!WRITE && threadIdx.x == 0) cnt[b] = 0;
+    if (threadIdx.x == 0) cnt[b] = 0;
     return;
   }
-  __shared__ int s_base, s_chunk[2];
-  int chunk_no = 0;  // APPEND: chunks done (the same in every thread)
-  const int keep = WRITE ? cnt[b] : 0;
-  if (WRITE && keep == 0) return;
+  int chunk_no = 0;  // chunks done (the same in every thread)
   const PlanTree& T = tree_of(t0, t1, b & 1);
   float qr[kMaxDofDev];
   for (int i = 0; i < ndof; ++i) qr[i] = qnew[j + (size_t)num * i];
@@ -318,79 +364,141 @@ __global__ void __launch_bounds__(256)
     s_chunk[0] = 0;
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63;
-  int run = 0;  // WRITE: matches of the earlier chunks
-  for (int base = 0; base < total; base += blockDim.x) {
-    const int idx = base + threadIdx.x;
-    float normsq = 0.0f;
-    bool hit = false;
-    int node = 0;
-    if (idx < total) {
-      int k = 0;  // the nonempty cell holding idx: last k with s_pre[k] <= idx
-      for (int step = 128; step > 0; step >>= 1)
-        if (k + step < K && s_pre[k + step] <= idx) k += step;
-      const int pos = s_start[k] + (idx - s_pre[k]);
-      for (int i = 0; i < ndof; ++i) {
-        const float diff = T.sorted[pos + (size_t)cap * i] - qr[i];
-        normsq += diff * diff;
-      }
-      hit = normsq < D2;
-      node = T.sid[pos];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  // candidate idx of the group's traversal: its node and squared distance
+  auto visit = [&](int idx, int& node, float& normsq) -> bool {
+    normsq = 0.0f;
+    node = 0;
+    if (idx >= total) return false;
+    int k = 0;  // the nonempty cell holding idx: last k with s_pre[k] <= idx
+    for (int step = 128; step > 0; step >>= 1)
+      if (k + step < K && s_pre[k + step] <= idx) k += step;
+    const int pos = s_start[k] + (idx - s_pre[k]);
+    for (int i = 0; i < ndof; ++i) {
+      const float diff = T.sorted[pos + (size_t)cap * i] - qr[i];
+      normsq += diff * diff;
     }
-    if (WRITE) {
-      int chunk;
-      const int r = run + block_excl_scan((int)hit, &chunk, lds);
-      if (hit && r < keep) {
-        const int o = off[b] + r;
+    node = T.sid[pos];
+    return normsq < D2;
+  };
+  // rank of this thread's match among the group's matches so far: the waves'
+  // counts of the chunk go to one of two LDS rows (alternating, so a row is
+  // rewritten only after every thread has passed the next chunk's barrier)
+  auto rank_in_chunk = [&](bool hit, int& tot) -> int {
+    const unsigned long long m = __ballot(hit);
+    int* const row = s_wcnt[chunk_no & 1];
+    if (lane == 0) row[wv] = __popcll(m);
+    __syncthreads();
+    int before = 0;
+    tot = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      const int c = row[w];
+      before += w < wv ? c : 0;
+      tot += c;
+    }
+    ++chunk_no;
+    return before + __popcll(m & below);
+  };
+  if (GROUP) {
+    int run = 0, n = 0;  // matches so far, staged entries (the same in every thread)
+    for (int base = 0; base < total; base += blockDim.x) {
+      if (n + (int)blockDim.x > kGroupStage) {
+        stage_sort(s_d2, s_idx, s_node, n);
+        n = min(n, keep);
+      }
+      int node;
+      float normsq;
+      const bool hit = visit(base + threadIdx.x, node, normsq);
+      int tot;
+      const int r = rank_in_chunk(hit, tot);
+      if (hit) {
+        s_node[n + r] = node;
+        s_d2[n + r] = normsq;
+        s_idx[n + r] = run + r;
+      }
+      run += tot;
+      n += tot;
+    }
+    if (n > keep) {
+      __syncthreads();
+      stage_sort(s_d2, s_idx, s_node, n);
+      n = keep;
+    }
+    if (threadIdx.x == 0) s_base = n > 0 ? atomicAdd(&info->E_all, n) : 0;
+    __syncthreads();
+    const int o = s_base;
+    const int kept = max(0, min(n, nbuf - o));  // n <= keep <= nbuf / groups: never cut
+    if (threadIdx.x == 0) {
+      cnt[b] = kept;
+      off[b] = min(o, nbuf);
+    }
+    for (int r = threadIdx.x; r < kept; r += blockDim.x) {
+      enode[o + r] = s_node[r];
+      ed2[o + r] = s_d2[r];
+      egroup[o + r] = b;
+    }
+    return;
+  }
+  for (int base = 0; base < total; base += blockDim.x) {
+    int node;
+    float normsq;
+    const bool hit = visit(base + threadIdx.x, node, normsq);
+    // each wave's matches take a block-local range (LDS atomic), the
+    // block's total one global range; the two counters alternate between
+    // chunks, so each is cleared a chunk before it is used again
+    const unsigned long long m = __ballot(hit);
+    int* const cur = &s_chunk[chunk_no & 1];
+    int wl = 0;
+    if (lane == 0 && m) wl = atomicAdd(cur, __popcll(m));
+    if (threadIdx.x == 0) s_chunk[(chunk_no + 1) & 1] = 0;
+    __syncthreads();
+    const int tot = *cur;
+    if (threadIdx.x == 0 && tot > 0) {
+      s_base = atomicAdd(&info->E_all, tot);
+      s_count += max(0, min(tot, nbuf - s_base));
+    }
+    __syncthreads();
+    ++chunk_no;
+    const int wbase = __shfl(wl, 0, 64);  // all lanes take part
+    if (hit) {
+      const int o = s_base + wbase + __popcll(m & below);
+      if (o < nbuf) {
         enode[o] = node;
         ed2[o] = normsq;
         egroup[o] = b;
       }
-      run += chunk;
-      if (run >= keep) break;  // uniform: run and keep are the same in every thread
-    } else if (APPEND) {
-      // each wave's matches take a block-local range (LDS atomic), the
-      // block's total one global range; the two counters alternate between
-      // chunks, so each is cleared a chunk before it is used again
-      const unsigned long long m = __ballot(hit);
-      int* const cur = &s_chunk[chunk_no & 1];
-      int wl = 0;
-      if (lane == 0 && m) wl = atomicAdd(cur, __popcll(m));
-      if (threadIdx.x == 0) s_chunk[(chunk_no + 1) & 1] = 0;
-      __syncthreads();
-      const int tot = *cur;
-      if (threadIdx.x == 0 && tot > 0) {
-        s_base = atomicAdd(&info->E_all, tot);
-        s_count += max(0, min(tot, nbuf - s_base));
-      }
-      __syncthreads();
-      ++chunk_no;
-      const int wbase = __shfl(wl, 0, 64);  // all lanes take part
-      if (hit) {
-        const int o = s_base + wbase + __popcll(m & ((1ull << lane) - 1ull));
-        if (o < nbuf) {
-          enode[o] = node;
-          ed2[o] = normsq;
-          egroup[o] = b;
-        }
-      }
-    } else {
-      const unsigned long long m = __ballot(hit);
-      if (lane == 0 && m) atomicAdd(&s_count, __popcll(m));
     }
   }
-  if (!WRITE) {
-    __syncthreads();
-    if (threadIdx.x == 0) cnt[b] = s_count;
-  }
+  __syncthreads();
+  if (threadIdx.x == 0) cnt[b] = s_count;
 }
 
-// After an APPEND pass: info->E = min(E_all, nbuf), no scored groups.
-__global__ void plan_near_append_done(int nbuf, StepInfo* info) {
+// After the radius search: info->E = min(E_all, nbuf); with budget B > 0
+// sc_off = exclusive scan over the groups of their stored edges where they
+// are more than B (the groups the model scores), info->S its total.
+__global__ void __launch_bounds__(kScanBlock)
+    plan_near_done_kernel(int groups, const int* __restrict__ cnt, int nbuf, int budget, int* __restrict__ sc_off,
+                          StepInfo* info) {
+  __shared__ int lds[kMaxWaves + 1];
+  int S = 0;
+  if (budget > 0) {
+    const int per = (groups + kScanBlock - 1) / kScanBlock;
+    const int b0 = threadIdx.x * per, b1 = min(groups, b0 + per);
+    int sc = 0;
+    for (int b = b0; b < b1; ++b) sc += cnt[b] > budget ? cnt[b] : 0;
+    int srun = block_excl_scan(sc, &S, lds);
+    for (int b = b0; b < b1; ++b) {
+      sc_off[b] = srun;
+      srun += cnt[b] > budget ? cnt[b] : 0;
+    }
+  }
   if (threadIdx.x == 0) {
     const int all = info->E_all;
     info->E = all < nbuf ? all : nbuf;
-    info->S = 0;
+    info->S = S;
+    if (budget > 0) sc_off[groups] = S;
   }
 }
 
@@ -399,47 +507,6 @@ __device__ __forceinline__ int stored_edges(const int* off, const int* cnt, int 
   (void)off;
   (void)nbuf;
   return cnt[b];
-}
-
-// After the counting pass: off[b] = exclusive scan of the groups' matches
-// (clipped to nbuf), info->E_all = their total, info->E = min(E_all, nbuf),
-// cnt[b] = the group's matches that fit below nbuf; with budget B > 0 also
-// sc_off over the groups with more than B stored edges and info->S.
-__global__ void __launch_bounds__(kScanBlock)
-    plan_scan_groups_kernel(int groups, int* __restrict__ cnt, int nbuf, int budget, int* __restrict__ off,
-                            int* __restrict__ sc_off, StepInfo* info) {
-  __shared__ int lds[kMaxWaves + 1];
-  const int per = (groups + kScanBlock - 1) / kScanBlock;
-  const int b0 = threadIdx.x * per, b1 = min(groups, b0 + per);
-  int s = 0;
-  for (int b = b0; b < b1; ++b) s += cnt[b];
-  int all;
-  int run = block_excl_scan(s, &all, lds);
-  int sc = 0;
-  for (int b = b0; b < b1; ++b) {
-    const int c = cnt[b];
-    const int kept = max(0, min(c, nbuf - run));
-    off[b] = min(run, nbuf);
-    cnt[b] = kept;
-    sc += kept > budget ? kept : 0;
-    run += c;
-  }
-  int S = 0;
-  if (budget > 0) {
-    int srun = block_excl_scan(sc, &S, lds);
-    for (int b = b0; b < b1; ++b) {
-      sc_off[b] = srun;
-      srun += cnt[b] > budget ? cnt[b] : 0;
-    }
-  }
-  if (threadIdx.x == 0) {
-    const int E = all < nbuf ? all : nbuf;
-    off[groups] = E;
-    if (budget > 0) sc_off[groups] = S;
-    info->E_all = all;
-    info->E = E;
-    info->S = S;
-  }
 }
 
 // ---- edge budget ----------------------------------------------------------------
@@ -863,19 +930,16 @@ hipError_t launch_plan_near(int ndof, int num, const float* qnew, StepInfo* info
                             const CellGrid& g, float D2, int* cnt, int* off, int* sc_off, int nbuf, int budget,
                             int* enode, float* ed2, int* egroup, hipStream_t st) {
   const int groups = 2 * num;
-  if (budget <= 0) {
-    // info->E_all was reset by the step's selection
+  const int keep = std::max(1, std::min(kGroupKeep, nbuf / groups));
+  // info->E_all was reset by the step's selection
+  if (budget <= 0)
     hipLaunchKernelGGL(plan_near_kernel<kNearAppend>, dim3(groups), dim3(256), 0, st, ndof, num, qnew, info, t0, t1,
-                       cap, g, D2, cnt, off, nbuf, enode, ed2, egroup);
-    hipLaunchKernelGGL(plan_near_append_done, dim3(1), dim3(64), 0, st, nbuf, info);
-    return hipGetLastError();
-  }
-  hipLaunchKernelGGL(plan_near_kernel<kNearCount>, dim3(groups), dim3(256), 0, st, ndof, num, qnew, info, t0, t1,
-                     cap, g, D2, cnt, off, nbuf, enode, ed2, egroup);
-  hipLaunchKernelGGL(plan_scan_groups_kernel, dim3(1), dim3(kScanBlock), 0, st, groups, cnt, nbuf, budget, off,
-                     sc_off, info);
-  hipLaunchKernelGGL(plan_near_kernel<kNearWrite>, dim3(groups), dim3(256), 0, st, ndof, num, qnew, info, t0, t1,
-                     cap, g, D2, cnt, off, nbuf, enode, ed2, egroup);
+                       cap, g, D2, cnt, off, nbuf, keep, enode, ed2, egroup);
+  else
+    hipLaunchKernelGGL(plan_near_kernel<kNearGroup>, dim3(groups), dim3(256), 0, st, ndof, num, qnew, info, t0, t1,
+                       cap, g, D2, cnt, off, nbuf, keep, enode, ed2, egroup);
+  hipLaunchKernelGGL(plan_near_done_kernel, dim3(1), dim3(kScanBlock), 0, st, groups, cnt, nbuf, budget, sc_off,
+                     info);
   return hipGetLastError();
 }
 

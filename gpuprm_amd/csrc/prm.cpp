@@ -458,13 +458,13 @@ extern "C" int gprm_prm_solve(gprm_ctx* ctx, const float* qstart, const float* q
                            w.qnew.as<float>(), w.tile.as<int>(), info, state, st) != hipSuccess ||
         !end_iv(iv_s, st))
       return GPRM_E_HIP;
-    // the radius search stores its edges in group order (the edge budget
-    // ranks within groups)
+    // with an edge budget the radius search stores each group's edges
+    // contiguously (the budget ranks within groups)
     const int iv_n = begin_iv(st, &res->ms_near);
     if (iv_n < 0 ||
         launch_plan_near(nd, num, w.qnew.as<float>(), info, T[0], T[1], cap, grid, D2, w.cnt.as<int>(),
-                         w.off.as<int>(), w.sc_off.as<int>(), nbuf, budget, w.enode.as<int>(), w.ed2.as<float>(),
-                         w.egroup.as<int>(), st) != hipSuccess ||
+                         w.off.as<int>(), w.sc_off.as<int>(), nbuf, budget, w.enode.as<int>(),
+                         w.ed2.as<float>(), w.egroup.as<int>(), st) != hipSuccess ||
         !end_iv(iv_n, st) || hipEventRecord(ev_chain, st) != hipSuccess ||
         hipMemcpyAsync(&w.pin->info, info, sizeof(StepInfo), hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipMemcpyAsync(&w.pin->state, state, sizeof(PlanState), hipMemcpyDeviceToHost, st) != hipSuccess ||

@@ -139,3 +139,49 @@ def test_prm_every_tree_node_and_edge_verified(golden_dir, tmp_path, workers):
     assert r.nodes_left == len(g["Ql"]) and r.nodes_right == len(g["Qr"])
     # free edges of the last step's nodes after the connection are not inserted
     assert r.edges_free >= len(g["El"]) + len(g["Er"])
+
+
+def _edge_set(E):
+    return E[np.lexsort((E[:, 1], E[:, 0]))]
+
+
+@pytest.mark.parametrize("budget", [0, 4])
+def test_prm_radius_search_placement_independent(golden_dir, tmp_path, budget):
+    """The radius search places groups in the order its blocks finish, so the
+    step's edge arrays differ between runs; the trees must not.  Without a
+    budget: two runs.  With an edge budget (shortest first, no model) a group
+    keeps its min(512, nbuf / groups) shortest matches, cut by a sort in LDS
+    when it has more (config1's groups hold hundreds): nbuf 2^20 (keep 512)
+    against 2^17 (keep 128) must pick the same budget edges.  Identical node
+    arrays, the same edge sets and counts, every tree edge free by the
+    oracle."""
+    import gpuprm_amd
+    z, d = golden_dir("config1")
+    s = _space(d)
+    o = OracleScene(d)
+    qs = np.zeros(4, np.float32)
+    qg = np.full(4, 3.0, np.float32)
+    runs = []
+    for nbuf in ((1 << 20, 1 << 20) if budget == 0 else (1 << 20, 1 << 17)):
+        out = str(tmp_path / ("trees_%d_%d" % (nbuf, len(runs))))
+        print("planner: config1, budget %d, nbuf %d" % (budget, nbuf), flush=True)
+        r, path = s.prm_solve(qs, qg, num=512, D=1.0, max_nodes=1 << 16, max_steps=300, seed=11, output_dir=out,
+                              edge_budget=budget, nbuf=nbuf)
+        print(r.as_dict())
+        runs.append((r, path, gpuprm_amd.read_prm_output(out, 4)))
+    (r0, p0, g0), (r1, p1, g1) = runs
+    for k in ("found", "steps", "nodes_left", "nodes_right", "point_checks", "edges_checked", "edges_free",
+              "edge_samples"):
+        assert getattr(r0, k) == getattr(r1, k), k
+    if budget:
+        assert r0.edges_skipped > r1.edges_skipped > 0  # the smaller keep stores fewer
+    for side in ("l", "r"):
+        assert np.array_equal(g0["Q" + side], g1["Q" + side])
+        assert np.array_equal(_edge_set(g0["E" + side]), _edge_set(g1["E" + side]))
+        Q, E = g0["Q" + side], g0["E" + side]
+        A, B = gpuprm_amd.canonical_edges(Q[E[:, 0]], Q[E[:, 1]])
+        res, _ = o.indicator2(A.T.copy(), B.T.copy(), nthreads=8)
+        assert np.all(res == 0)
+    if r0.found:
+        assert np.array_equal(p0, p1)
+        _verify(o, p0, qs, qg, 1.0)
