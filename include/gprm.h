@@ -185,7 +185,9 @@ typedef struct {
                                rank after the others */
   int edge_budget;          /* 0: check every candidate edge (the reference's rule);
                                k > 0: at most k per (new node, tree), shortest first
-                               (among predicted-free edges first with order_edges) */
+                               (among predicted-free edges first with order_edges),
+                               chosen from the group's min(512, nbuf / (2 num))
+                               shortest candidates */
   const char* output_dir;   /* NULL, or a directory that receives both trees and the
                                path in the reference's output layout (store_results,
                                prmsolver.cpp:2947-3055; prm/matlab/lib/prmoutputread.m) */
