@@ -725,8 +725,42 @@ def leg_prm():
         "best_of_num_ms": best, "speedup_best_of_num": best["plain"] / best["surrogate"],
         "edge_samples_saved": sweep[1024]["edge_samples_saved"],
         "point_checks_saved": sweep[1024]["point_checks_saved"],
-        "all_paths_valid": sweep[1024]["all_paths_valid"], "by_num": sweep}
+        "all_paths_valid": all(sweep[n]["all_paths_valid"] for n in sweep), "by_num": sweep}
+    out["config1_reference_task"] = leg_config1()
     return out
+
+
+def leg_config1(seeds: int = 10):
+    """The reference's own planner task (prm/src/main.cpp:122-133): the shipped
+    4-DOF scene config1, start (0,0,0,0), goal (3,3,3,3), D = 1.0, dq = 0.01,
+    at the reference's default of 32 new nodes per step and at 1,024; mean and
+    median time-to-path over 10 seeds (the paper's rows are means of 10
+    seeds), every path re-verified by the oracle."""
+    import gpuprm_amd
+    from oracle.oracle import OracleScene
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import scenes
+
+    d = tempfile.mkdtemp(prefix="gprm_config1_")
+    z = dict(np.load(os.path.join(ROOT, "tests", "golden", "config1.npz")))
+    scenes.write_dir(scenes.scene_from_npz(z), d)
+    sp = gpuprm_amd.RobotConfigspace(d, 0.01, 1 << 20)
+    sp.init(0, 1)
+    o = OracleScene(d)
+    qs, qg = np.zeros(4, np.float32), np.full(4, 3.0, np.float32)
+    res = {}
+    for num in (32, 1024):
+        runs = [_prm_run(sp, o, qs, qg, num=num, D=1.0, max_nodes=1 << 18, max_steps=10000, seed=s)
+                for s in range(seeds)]
+        res[num] = {"ms_mean": float(np.mean([x["ms"] for x in runs])),
+                    "ms_median": float(np.median([x["ms"] for x in runs])),
+                    "checks_mean": float(np.mean([x["edge_samples"] + x["point_checks"] for x in runs])),
+                    "steps_mean": float(np.mean([x["steps"] for x in runs])),
+                    "all_found": all(x["found"] for x in runs), "all_paths_valid": all(x["valid"] for x in runs),
+                    "runs": runs}
+    return {"workload": "config1 (4-DOF, the reference's shipped scene), start 0, goal 3, D=1.0, 10 seeds",
+            "by_num": res}
 
 
 def main():

@@ -187,6 +187,7 @@ def lib():
     L.gprm_numthreads_all.restype = C.c_longlong
     L.gprm_indicator_dev.argtypes = [vp, vp, vp, C.c_int, C.c_longlong, vp]
     L.gprm_indicator_dev_noalloc.argtypes = [vp, vp, vp, C.c_int, C.c_longlong, vp]
+    L.gprm_kinematics_dev.argtypes = [vp, vp, vp, C.c_int, C.c_longlong, vp]
     L.gprm_indicator2_dev.argtypes = [vp, vp, vp, vp, C.c_int, C.c_longlong, vp, C.POINTER(EdgeStats)]
     L.gprm_last_error.argtypes = [vp]
     L.gprm_last_error.restype = C.c_char_p
@@ -651,6 +652,15 @@ class RobotConfigspace:
         N = int(q.shape[1])
         _check(lib().gprm_indicator_dev(self._ctx, C.c_void_p(q.data_ptr()), C.c_void_p(res.data_ptr()), N,
                                         N, C.c_void_p(stream)), self._ctx)
+
+    def kinematics_dev(self, q, frames, stream=None) -> None:
+        """Kinematics<ndof>::calculate (kinematics.h:28-37) for every column of
+        q (float32 device tensor (ndof, N), contiguous): frames float32 device
+        tensor (N, ndof + 1, 4, 3), frame s, column c, row x/y/z."""
+        N = int(q.shape[1])
+        assert tuple(frames.shape) == (N, self.ndof + 1, 4, 3) and frames.is_contiguous()
+        _check(lib().gprm_kinematics_dev(self._ctx, C.c_void_p(q.data_ptr()), C.c_void_p(frames.data_ptr()), N,
+                                         N, C.c_void_p(stream)), self._ctx)
 
     def indicator2_dev(self, qs, qe, res, stream=None) -> EdgeStats:
         """qs, qe: float32 device tensors (ndof, N); res: int32 device tensor (N,)."""

@@ -895,6 +895,35 @@ int gprm_indicator_dev(gprm_ctx* c, const float* d_q, int* d_res, int N, long lo
   return prof_harvest(c, sl);
 }
 
+int gprm_kinematics_dev(gprm_ctx* c, const float* d_q, float* d_frames, int N, long long offset, void* stream) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (N < 0 || (N > 0 && (!d_q || !d_frames)) || offset < 0) return fail(c, GPRM_E_INVALID, "bad arguments");
+  if (N == 0) return GPRM_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  int si;
+  if ((rc = acquire_slot(c, &si))) return rc;
+  Slot& sl = *c->slots[si];
+  struct Release { Slot& s; ~Release() { s.busy = false; } } rel{sl};
+  hipStream_t st = stream ? (hipStream_t)stream : sl.stream;
+  const int nd = c->scene.ndof;
+  const int chunk = std::min(N, item_budget(c));
+  GPRM_HIP(sl.frames.ensure(sizeof(float) * 12 * nd * (size_t)frame_stride(chunk)), "hipMalloc frames");
+  GPRM_HIP(sl.status.ensure(sizeof(int) * (size_t)chunk), "hipMalloc words");
+  for (int c0 = 0; c0 < N; c0 += chunk) {
+    const int cn = std::min(chunk, N - c0);
+    // the kinematics kernel of the point checks, then its frames unpacked
+    GPRM_HIP(launch_fk_points(c->rb, d_q + c0, offset, cn, sl.frames.as<float>(), frame_stride(cn),
+                              sl.status.as<int>(), 0, nullptr, st),
+             "kinematics launch");
+    GPRM_HIP(launch_frames_unpack(c->rb, sl.frames.as<float>(), frame_stride(cn), cn,
+                                  d_frames + (size_t)c0 * (nd + 1) * 12, st),
+             "frames launch");
+  }
+  GPRM_HIP(hipStreamSynchronize(st), "sync");
+  return GPRM_OK;
+}
+
 int gprm_indicator2_async(gprm_ctx* c, const float* qs, const float* qe, int* res, int N, int offset,
                           int* request) {
   int rc = check_ready(c);

@@ -117,18 +117,22 @@ __device__ __forceinline__ T12 t_lapply(const T12& self, const T12& T) {
 // to the frame array, link-major and quad-planar: float4 j (0..2) of link
 // frame i of item at out4[(3 i + j) stride + item], so each quad of one link
 // is a contiguous array over the items.
+// T0 = set(0, 0, 0, 0) (kinematics.h:27-37; trafo4::set, geo4.h:32-47),
+// from cos/sin(0) held in rb
+__device__ __forceinline__ T12 t_base(const DevRobot& rb) {
+  const float cq = rb.c0, sq = rb.s0, ca = rb.c0, sa = rb.s0, a = 0.0f, d = 0.0f;
+  T12 T;
+  T.m[0] = cq;      T.m[3] = -sq;     T.m[6] = 0.0f;  T.m[9] = a;
+  T.m[1] = sq * ca; T.m[4] = cq * ca; T.m[7] = -sa;   T.m[10] = -sa * d;
+  T.m[2] = sq * sa; T.m[5] = cq * sa; T.m[8] = ca;    T.m[11] = ca * d;
+  return T;
+}
+
 template <int NDOF>
 __device__ __forceinline__ void fk_store(const DevRobot& rb, const float* q, float* __restrict__ out, int item,
                                          long long stride) {
   const int nd = NDOF > 0 ? NDOF : rb.ndof;
-  // T0 = set(0, 0, 0, 0) (kinematics.h:27-37), from cos/sin(0) held in rb
-  T12 T;
-  {
-    const float cq = rb.c0, sq = rb.s0, ca = rb.c0, sa = rb.s0, a = 0.0f, d = 0.0f;
-    T.m[0] = cq;      T.m[3] = -sq;     T.m[6] = 0.0f;  T.m[9] = a;
-    T.m[1] = sq * ca; T.m[4] = cq * ca; T.m[7] = -sa;   T.m[10] = -sa * d;
-    T.m[2] = sq * sa; T.m[5] = cq * sa; T.m[8] = ca;    T.m[11] = ca * d;
-  }
+  T12 T = t_base(rb);
 #pragma unroll
   for (int i = 0; i < (NDOF > 0 ? NDOF : kMaxDofDev); ++i) {
     if (NDOF == 0 && i >= nd) break;
@@ -202,6 +206,24 @@ __global__ void __launch_bounds__(kFkBlock)
     word[i] = kWordFree;
   }
   fk_store<NDOF>(rb, qq, frames, i, fstride);
+}
+
+// The frames of n configurations as Kinematics<ndof>::calculate leaves its
+// trafos (kinematics.h:28-37): the base frame and the link frames the
+// kinematics kernel stored, unpacked to out[(i (ndof + 1) + s) 12 + m] with
+// m = 3 column + row (trafo4's four columns of x, y, z).
+__global__ void __launch_bounds__(256)
+    frames_unpack_kernel(DevRobot rb, const float* __restrict__ frames, long long fstride, int n,
+                         float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float* o = out + (size_t)i * (rb.ndof + 1) * 12;
+  const T12 T0 = t_base(rb);
+  for (int m = 0; m < 12; ++m) o[m] = T0.m[m];
+  for (int s = 1; s <= rb.ndof; ++s) {
+    const T12 t = load_frame(frames, i, fstride, s);
+    for (int m = 0; m < 12; ++m) o[12 * s + m] = t.m[m];
+  }
 }
 
 // codes[i] from the point mode result word.
@@ -1233,13 +1255,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVE
 // Frames 1..ndof of one configuration (kinematics.h:27-37), the same
 // operations as fk_store.
 __device__ void fk_frames(const DevRobot& rb, const float* q, T12* out) {
-  T12 T;
-  {
-    const float cq = rb.c0, sq = rb.s0, ca = rb.c0, sa = rb.s0, a = 0.0f, d = 0.0f;
-    T.m[0] = cq;      T.m[3] = -sq;     T.m[6] = 0.0f;  T.m[9] = a;
-    T.m[1] = sq * ca; T.m[4] = cq * ca; T.m[7] = -sa;   T.m[10] = -sa * d;
-    T.m[2] = sq * sa; T.m[5] = cq * sa; T.m[8] = ca;    T.m[11] = ca * d;
-  }
+  T12 T = t_base(rb);
   for (int i = 0; i < rb.ndof; ++i) {
     float qset = rb.q[i], dset = rb.d[i];
     if (rb.types[i] == 0) qset = q[i];
@@ -1401,6 +1417,13 @@ hipError_t launch_fk_points(const DevRobot& rb, const float* q, long long offset
   dim3 grid((n + fb - 1) / fb), block(fb);
   GPRM_FK_DISPATCH(fk_points_kernel, grid, block, st, rb, q, offset, n, frames, fstride, word,
                    check_bounds, zero_counter, skip, group_done);
+  return hipGetLastError();
+}
+
+hipError_t launch_frames_unpack(const DevRobot& rb, const float* frames, long long fstride, int n, float* out,
+                               hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(frames_unpack_kernel, dim3((n + 255) / 256), dim3(256), 0, st, rb, frames, fstride, n, out);
   return hipGetLastError();
 }
 

@@ -189,6 +189,10 @@ hipError_t launch_fk_points(const DevRobot& rb, const float* q, long long offset
                             long long fstride, int* word, int check_bounds, unsigned* zero_counter, hipStream_t st,
                             const unsigned char* skip = nullptr, int* group_done = nullptr);
 hipError_t launch_finalize_points(const int* word, int* codes, int n, hipStream_t st);
+// The stored link frames of n items (and the base frame) unpacked to
+// out[(i (ndof + 1) + s) 12 + 3 column + row].
+hipError_t launch_frames_unpack(const DevRobot& rb, const float* frames, long long fstride, int n, float* out,
+                               hipStream_t st);
 struct SvaShape;
 hipError_t launch_sva(const SvaArgs& a, int edge_mode, const SvaShape& sh, hipStream_t st);
 // Multi-scene launches (d_tasks in device memory).  The kinematics and

@@ -138,6 +138,13 @@ typedef struct {
 } gprm_edge_stats;
 int gprm_indicator2_dev(gprm_ctx* ctx, const float* d_qs, const float* d_qe, int* d_res, int N,
                         long long offset, void* stream, gprm_edge_stats* stats);
+/* Kinematics<ndof>::calculate(q, offset) (kinematics.h:28-37) for N
+ * configurations in device memory (d_q[i + offset j], offset >= N): the
+ * ndof + 1 frames it leaves in trafos, as the collision kernels use them,
+ * to d_frames[(i (ndof + 1) + s) 12 + 3 c + r] (frame s, column c of four,
+ * row r of x, y, z; trafo4, geo4.h:22-24). */
+int gprm_kinematics_dev(gprm_ctx* ctx, const float* d_q, float* d_frames, int N, long long offset,
+                        void* stream);
 
 /* Kernel timing and work counters (no reference counterpart; the reference's
  * only counter is numthreads_all).  While enabled, every kinematics and
