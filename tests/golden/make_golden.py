@@ -91,9 +91,9 @@ def make(name, scene_dir, seed, nq, nfk, npair_q, nedges, spread):
 def main():
     import gpuprm_amd
 
-    make("config1", CONFIG1, 11, 4000, 32, 24, 120, 0.3)
+    make("config1", CONFIG1, 11, 4000, 32, 24, 1200, 0.3)
     tmp = tempfile.mkdtemp(prefix="gprm_golden_")
-    for wl, nq, npq, ne in ((1, 4000, 24, 120), (2, 3000, 12, 90), (3, 2000, 8, 60)):
+    for wl, nq, npq, ne in ((1, 4000, 24, 1200), (2, 3000, 12, 900), (3, 2000, 8, 600)):
         d = os.path.join(tmp, "wl%d" % wl)
         gpuprm_amd.generate_scene(wl, 2024, d)
         make("wl%d" % wl, d, 100 + wl, nq, 32, npq, ne, 0.2)
