@@ -657,63 +657,83 @@ __device__ __forceinline__ V2 mult23(const Orth& o, const V3& x) {
   return V2{(o.M0 * x.x + o.M1 * x.y) + o.M2 * x.z, (o.M3 * x.x + o.M4 * x.y) + o.M5 * x.z};
 }
 
-// find_half_plane (collision4.h:136-223), R[0..k-1] in the lane's history.
-__device__ __forceinline__ bool find_half_plane(const float4* __restrict__ H, size_t hs, int k, V3& w,
-                                                const V3& rk) {
-  float dp = dot3(w, rk);
-  if (dp > 0) return true;
-  Orth o;
-  {
-    float x = rk.x, y = rk.y, z = rk.z;
+// Iteration that did not separate (collision4.h:301-323): repetition test,
+// w update, half plane (find_half_plane, collision4.h:136-223), reflection.
+// Returns a nonzero code when the pair is decided (l+1: no half plane), else
+// 0.  Taken by few iterations, but it sets the latency of a full test: the
+// history R[0..k-1] lives in memory, so it is read in one pass with
+// kHistBatch loads in flight, and the repetition test and the half-plane scan run
+// together on each entry.  The scan's outcome counts only when no (p,q)
+// repeats, as in the reference, where the repetition test comes first; the
+// scan stops changing ra/rb where find_half_plane returns false.
+constexpr int kHistBatch = 2;  // history loads in flight (four spill registers at five waves per SIMD)
+__device__ __forceinline__ int sva_slow(float4* __restrict__ H, size_t hs, int& k, int l, int pq, const V3& r,
+                                        float dp, V3& S, V3& w) {
+  H[k * hs] = make_float4(r.x, r.y, r.z, __int_as_float(pq));
+  // find_half_plane returns at once (w unchanged) when w.r > 0
+  const bool scan = k >= 2 && !(dot3(w, r) > 0);
+  Orth o{};
+  if (scan) {
+    // orthtrafo23 M(rk)
+    float x = r.x, y = r.y, z = r.z;
     float d2 = x * x + y * y;
     float d = __builtin_sqrtf(d2);
     o.M0 = x * z / d; o.M1 = y * z / d; o.M2 = -d;
     o.M3 = -y / d;    o.M4 = x / d;     o.M5 = 0.0f;
   }
-  V2 ra = mult23(o, v3(H[0]));
-  V2 rb = mult23(o, v3(H[hs]));
-  if (cross2(ra, rb) < 0) {
-    V2 t = ra;
-    ra = rb;
-    rb = t;
-  }
-  for (int i = 2; i < k; ++i) {
-    V2 T = mult23(o, v3(H[i * hs]));
-    float rbcross = cross2(rb, T);
-    if (cross2(ra, T) > 0) {
-      if (rbcross > 0) rb = T;
-    } else {
-      if (rbcross < 0) ra = T;
-      else return false;
+  bool rep = false, fail = false;
+  V3 r0{0.0f, 0.0f, 0.0f};
+  V2 ra{0.0f, 0.0f}, rb{0.0f, 0.0f};
+  for (int j0 = 0; j0 < k; j0 += kHistBatch) {
+    float4 e[kHistBatch];
+#pragma unroll
+    for (int u = 0; u < kHistBatch; ++u)
+      e[u] = j0 + u < k ? H[(j0 + u) * hs] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int u = 0; u < kHistBatch; ++u) {
+      const int j = j0 + u;
+      if (j >= k) break;
+      rep = rep || __float_as_int(e[u].w) == pq;
+      if (j == 0) r0 = v3(e[u]);
+      if (!scan || fail) continue;
+      const V2 T = mult23(o, v3(e[u]));
+      if (j == 0) {
+        ra = T;
+      } else if (j == 1) {
+        rb = T;
+        if (cross2(ra, rb) < 0) {
+          rb = ra;
+          ra = T;
+        }
+      } else {
+        const float rbcross = cross2(rb, T);
+        if (cross2(ra, T) > 0) {
+          if (rbcross > 0) rb = T;
+        } else {
+          if (rbcross < 0) ra = T;
+          else fail = true;
+        }
+      }
     }
   }
-  normalize2(ra);
-  normalize2(rb);
-  V2 T{ra.x + rb.x, ra.y + rb.y};
-  normalize2(T);
-  w.x = o.M0 * T.x + o.M3 * T.y;
-  w.y = o.M1 * T.x + o.M4 * T.y;
-  w.z = o.M2 * T.x + o.M5 * T.y;
-  return true;
-}
-
-// Iteration that did not separate (collision4.h:301-323): repetition test,
-// w update, half plane, reflection.  Returns a nonzero code when the pair is
-// decided (l+1: no half plane), else 0.  Taken by few iterations.
-__device__ __forceinline__ int sva_slow(float4* __restrict__ H, size_t hs, int& k, int l, int pq, const V3& r,
-                                        float dp, V3& S, V3& w) {
-  H[k * hs] = make_float4(r.x, r.y, r.z, __int_as_float(pq));
-  for (int j = 0; j < k; ++j)
-    if (__float_as_int(H[j * hs].w) == pq) {
-      S = w;  // (p,q) seen before
-      return 0;
-    }
+  if (rep) {
+    S = w;  // (p,q) seen before
+    return 0;
+  }
   if (k == 1) {
-    V3 r0 = v3(H[0]);
     w = V3{r0.x + r.x, r0.y + r.y, r0.z + r.z};
     normalize3(w);
   }
-  if (k >= 2 && !find_half_plane(H, hs, k, w, r)) return l + 1;
+  if (scan) {
+    if (fail) return l + 1;
+    normalize2(ra);
+    normalize2(rb);
+    V2 T{ra.x + rb.x, ra.y + rb.y};
+    normalize2(T);
+    w.x = o.M0 * T.x + o.M3 * T.y;
+    w.y = o.M1 * T.x + o.M4 * T.y;
+    w.z = o.M2 * T.x + o.M5 * T.y;
+  }
   float b = -2.0f * dp;
   S.x = S.x + b * r.x;
   S.y = S.y + b * r.y;
