@@ -17,8 +17,10 @@
 namespace gprm {
 
 // Order-position windows of the edge rounds for a largest testnum maxT:
-// [0, 12), then widths growing fourfold (defined in gprm_api.cpp).
-std::vector<std::pair<int, int>> edge_windows(int maxT);
+// [0, 12), then widths growing fourfold, or one window [0, maxT) when
+// pair_tests (a bound on the call's pair tests, -1 if unknown) is small
+// (defined in gprm_api.cpp).
+std::vector<std::pair<int, int>> edge_windows(int maxT, long long pair_tests = -1);
 
 struct DevBuf {
   void* p = nullptr;

@@ -50,14 +50,29 @@ constexpr int kItemBudget = 1 << 24;
 // longest edges.
 constexpr int kFirstWindow = 12;
 constexpr int kWindowGrowth = 4;
+// A call whose whole plan is known (the planner's, bounded by D / dq) and
+// holds at most this many pair tests (edges x largest testnum x pairs) runs
+// as one round.  Each round costs a launch whose end waits for its longest
+// full test (tens of microseconds), and on the planner's steps almost every
+// candidate edge is free, so the early exit between rounds saves little.
+// Median time-to-path with three rounds / one round up to 2^28 / 2^30 /
+// 2^32 pair tests: config1 at 32 nodes per step 226 / 162 / 163 / 162 ms,
+// 7-DOF 153 / 151 / 140 / 140 ms, 6-DOF at 1,024 nodes 8.1 / 5.7-5.8 /
+// 5.6 / 5.8 ms, with an edge budget at 4,096 nodes 5.7 / 5.8 / 5.1 / 4.9 ms.
+constexpr long long kOneRoundPairTests = 1LL << 32;
 
 thread_local std::string g_create_error;
 
 }  // namespace
 
-// Sample-window rounds: positions [0,w0), then widths growing kWindowGrowth-fold.
-std::vector<std::pair<int, int>> edge_windows(int maxT) {
+// Sample-window rounds: positions [0,w0), then widths growing kWindowGrowth-fold;
+// one window when pair_tests (a bound on the call's, or -1) is small.
+std::vector<std::pair<int, int>> edge_windows(int maxT, long long pair_tests) {
   std::vector<std::pair<int, int>> w;
+  if (pair_tests >= 0 && pair_tests <= kOneRoundPairTests && maxT > 0) {
+    w.push_back({0, maxT});
+    return w;
+  }
   int lo = 0, width = kFirstWindow;
   while (lo < maxT) {
     int hi = (int)std::min<long long>((long long)lo + width, maxT);
@@ -601,7 +616,7 @@ int run_edges(gprm_ctx* c, Slot& sl, const float* d_qs, const float* d_qe, long 
   int maxT = host_maxT;
   if (host_maxT >= 0) {
     // the whole plan is known: one enqueue sizes and launches it
-    for (const auto& w : edge_windows(host_maxT)) add_window(w.first, w.second);
+    for (const auto& w : edge_windows(host_maxT, (long long)N * host_maxT * c->img.M)) add_window(w.first, w.second);
   } else {
     add_window(0, kFirstWindow);
     const int hint = std::max(sl.maxT_hint, maxT_hint);
@@ -722,7 +737,7 @@ int reserve_slot(gprm_ctx* c, Slot& sl, int n_points, int N, int maxT_bound) {
     // buffer set i & 1 for launch i
     long long cap[2] = {0, 0};
     size_t launches = 0;
-    for (const auto& w : edge_windows(maxT_bound)) {
+    for (const auto& w : edge_windows(maxT_bound, (long long)N * maxT_bound * c->img.M)) {
       const int per = std::max(1, item_budget(c) / (w.second - w.first));
       for (int e0 = 0; e0 < N; e0 += per, ++launches)
         cap[launches & 1] = std::max(cap[launches & 1],
