@@ -418,7 +418,7 @@ class SceneBatch:
 
 
 def generate_scene(workload: int, seed: int, out_dir: str) -> str:
-    """Write the seeded scene of a benchmark workload (1, 2, 3 or 5) to out_dir."""
+    """Write the seeded scene of a benchmark workload (1, 2, 3, 5 or 6) to out_dir."""
     _check(lib().gprm_scene_generate(int(workload), int(seed), out_dir.encode()))
     return out_dir
 

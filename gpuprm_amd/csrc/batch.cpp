@@ -665,6 +665,13 @@ int gprm_batch_create(gprm_ctx* const* ctxs, int nscenes, gprm_batch** out) {
     else if (4 * (bk.lds + sva_ring_bytes(256)) <= cap) { bk.shape.block = 256; bk.shape.waves = 4; }
     else { bk.shape.block = 1024; bk.shape.waves = 4; }
     bk.shape.ppu = 64;
+    if (bk.lds + sva_ring_bytes(1024) > cap) {
+      // every scene of a batch is staged in LDS (single-scene contexts read
+      // larger images from HBM)
+      g_batch_error = "scene image of " + std::to_string(bk.lds) + " bytes beyond LDS for the multi-scene kernel";
+      delete b;
+      return GPRM_E_CAPACITY;
+    }
     e = sva_multi_shape_finish(bk.lds, &bk.shape, dev);
     if (e != hipSuccess) {
       int rc = bhip(nullptr, e, "scene too large for the multi-scene kernel");

@@ -279,8 +279,6 @@ int build_image(gprm_ctx* c, std::vector<char>* out) {
                  s.from[i] | (s.to[i] << 16)};
     std::memcpy(b + img.off_pairs + 16 * (size_t)j, pr, 16);
   }
-  if (sva_lds_bytes(img, 1024) > (size_t)sva_max_lds_bytes())
-    return fail(c, GPRM_E_CAPACITY, "scene image of " + std::to_string(img.bytes) + " bytes does not fit in LDS");
   return GPRM_OK;
 }
 

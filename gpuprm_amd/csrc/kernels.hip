@@ -763,6 +763,18 @@ __device__ __forceinline__ Lds stage_scene(const SceneImg& img, int4* smem) {
   return L;
 }
 
+// The scene read where it lies in the global image (scenes beyond LDS): the
+// same layout, through the caches.
+__device__ __forceinline__ Lds scene_global(const SceneImg& img) {
+  const char* base = reinterpret_cast<const char*>(img.blob);
+  Lds L;
+  L.verts = reinterpret_cast<const float4*>(base + img.off_verts);
+  L.pairs = reinterpret_cast<const int4*>(base + img.off_pairs);
+  L.adj = reinterpret_cast<const unsigned short*>(base + img.off_adj);
+  L.spheres = reinterpret_cast<const float4*>(base + img.off_spheres);
+  return L;
+}
+
 // Deferred pair tests of one wave: a ring of CAP = sva_ring_cap(BLOCK)
 // entries in LDS, (item, pair) in ent and the item's edge (edge mode) in
 // edge.  Phase B starts as soon as CAP - 64 entries wait, so the ring never
@@ -1205,12 +1217,14 @@ __device__ __forceinline__ void add_tally(const SvaCounts& c, int4* smem, unsign
 // WAVES: occupancy target in waves per SIMD; the compiler keeps the register
 // budget at 512/WAVES VGPRs (5 waves: 96 VGPRs; the edge-mode build spills
 // two dwords).
-template <int EDGE, int BLOCK, int WAVES>
+// GS: the scene is read from the global image instead of an LDS copy (for
+// scenes whose image does not fit).
+template <int EDGE, int BLOCK, int WAVES, int GS>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) sva_kernel(SvaArgs A) {
   extern __shared__ int4 smem[];
   // dynamic LDS: the waves' deferral rings (sva_ring_bytes), then the
-  // scene image
-  const Lds L = stage_scene<BLOCK>(A.img, smem + sva_ring_bytes(BLOCK) / 16);
+  // scene image (unless GS)
+  const Lds L = GS ? scene_global(A.img) : stage_scene<BLOCK>(A.img, smem + sva_ring_bytes(BLOCK) / 16);
   const Ring R = wave_ring<BLOCK>(smem);
   __syncthreads();
   float4* __restrict__ H = A.hist + (size_t)blockIdx.x * BLOCK + threadIdx.x;
@@ -1408,7 +1422,9 @@ __global__ void finalize_points_multi_kernel(const FkTask* __restrict__ tasks, i
 
 // Dynamic LDS of a separating-vector block: the deferral rings (16 B per
 // thread) and the scene image.
-size_t sva_lds_bytes(const SceneImg& img, int block) { return (size_t)img.bytes + sva_ring_bytes(block); }
+size_t sva_lds_bytes(const SceneImg& img, int block, int gscene) {
+  return (gscene ? 0 : (size_t)img.bytes) + sva_ring_bytes(block);
+}
 
 int sva_max_lds_bytes() { return 160 * 1024; }
 
@@ -1465,17 +1481,23 @@ hipError_t launch_edge_fk(const DevRobot& rb, EdgeArgs ea, const int* item_edge,
   return hipGetLastError();
 }
 
-template <int EDGE, int BLOCK, int WAVES>
+template <int EDGE, int BLOCK, int WAVES, int GS>
 static const void* sva_fn() {
-  return reinterpret_cast<const void*>(&sva_kernel<EDGE, BLOCK, WAVES>);
+  return reinterpret_cast<const void*>(&sva_kernel<EDGE, BLOCK, WAVES, GS>);
 }
 
-// The kernel variants a shape may name: (block, waves per SIMD).
-static const void* sva_variant(int edge_mode, int block, int waves) {
-  if (block == 256 && waves == 5) return edge_mode ? sva_fn<1, 256, 5>() : sva_fn<0, 256, 5>();
-  if (block == 256 && waves == 4) return edge_mode ? sva_fn<1, 256, 4>() : sva_fn<0, 256, 4>();
-  if (block == 1024 && waves == 4) return edge_mode ? sva_fn<1, 1024, 4>() : sva_fn<0, 1024, 4>();
+template <int GS>
+static const void* sva_variant_gs(int edge_mode, int block, int waves) {
+  if (block == 256 && waves == 5) return edge_mode ? sva_fn<1, 256, 5, GS>() : sva_fn<0, 256, 5, GS>();
+  if (block == 256 && waves == 4) return edge_mode ? sva_fn<1, 256, 4, GS>() : sva_fn<0, 256, 4, GS>();
+  if (block == 1024 && waves == 4) return edge_mode ? sva_fn<1, 1024, 4, GS>() : sva_fn<0, 1024, 4, GS>();
   return nullptr;
+}
+
+// The kernel variants a shape may name: (block, waves per SIMD), the scene
+// in LDS or (gscene) in the global image.
+static const void* sva_variant(int edge_mode, int block, int waves, int gscene) {
+  return gscene ? sva_variant_gs<1>(edge_mode, block, waves) : sva_variant_gs<0>(edge_mode, block, waves);
 }
 
 template <int EDGE, int BLOCK, int WAVES>
@@ -1518,10 +1540,12 @@ hipError_t sva_multi_shape_finish(size_t img_bytes, SvaShape* sh, int device) {
 }
 
 hipError_t sva_shape_finish(const SceneImg& img, SvaShape* sh, int device) {
-  const size_t lds = sva_lds_bytes(img, sh->block);
+  sh->gscene = sva_lds_bytes(img, sh->block) > (size_t)sva_max_lds_bytes() ? 1 : 0;
+  const size_t lds = sva_lds_bytes(img, sh->block, sh->gscene);
   if (lds > (size_t)sva_max_lds_bytes()) return hipErrorInvalidValue;
   if (sh->ppu < 1 || sh->ppu > 256) return hipErrorInvalidValue;
-  const void* fns[2] = {sva_variant(0, sh->block, sh->waves), sva_variant(1, sh->block, sh->waves)};
+  const void* fns[2] = {sva_variant(0, sh->block, sh->waves, sh->gscene),
+                        sva_variant(1, sh->block, sh->waves, sh->gscene)};
   return resident_grid(fns, sh->block, lds, device, &sh->grid);
 }
 
@@ -1538,18 +1562,23 @@ SvaShape sva_default_shape(const SceneImg& img) {
   } else if (4 * sva_lds_bytes(img, 256) <= (size_t)sva_max_lds_bytes()) {
     sh.block = 256;
     sh.waves = 4;
-  } else {
+  } else if (sva_lds_bytes(img, 1024) <= (size_t)sva_max_lds_bytes()) {
     sh.block = 1024;
     sh.waves = 4;
+  } else {
+    // beyond LDS: the scene is read from the global image (sva_shape_finish
+    // sets gscene), and LDS holds only the rings
+    sh.block = 256;
+    sh.waves = 5;
   }
   sh.ppu = 128;
   return sh;
 }
 
 hipError_t launch_sva(const SvaArgs& a, int edge_mode, const SvaShape& sh, hipStream_t st) {
-  const size_t lds = sva_lds_bytes(a.img, sh.block);
+  const size_t lds = sva_lds_bytes(a.img, sh.block, sh.gscene);
   if (lds > (size_t)sva_max_lds_bytes() || a.hist == nullptr || sh.grid < 1) return hipErrorInvalidValue;
-  const void* fn = sva_variant(edge_mode, sh.block, sh.waves);
+  const void* fn = sva_variant(edge_mode, sh.block, sh.waves, sh.gscene);
   if (!fn) return hipErrorInvalidValue;
   void* args[] = {const_cast<SvaArgs*>(&a)};
   return hipLaunchKernel(fn, dim3(sh.grid), dim3(sh.block), args, lds, st);

@@ -176,8 +176,10 @@ hipError_t launch_edge_fk_multi(const EdgeTask* d_tasks, int ntasks, int nblocks
 // 1024-thread variant keeps smaller rings to leave LDS to large scenes.
 constexpr int sva_ring_cap(int block) { return block >= 1024 ? 128 : 256; }
 constexpr size_t sva_ring_bytes(int block) { return (size_t)(block / 64) * (size_t)sva_ring_cap(block) * 12; }
-// Dynamic LDS of a separating-vector block of `block` threads on this scene.
-size_t sva_lds_bytes(const SceneImg& img, int block);
+// Dynamic LDS of a separating-vector block of `block` threads on this scene:
+// its rings and, unless the kernel reads the scene from the global image
+// (gscene), a copy of the image.
+size_t sva_lds_bytes(const SceneImg& img, int block, int gscene = 0);
 int sva_max_lds_bytes();
 
 // Kinematics of n configurations; frames link-major with item stride n.
@@ -245,9 +247,12 @@ struct SvaShape {
   int waves = 5;
   int ppu = 32;
   int grid = 0;
+  int gscene = 0;  // 1: the kernel reads the scene from the global image (beyond LDS)
 };
-// Variants: (256, 5), (256, 4), (1024, 4).  Fills in grid; an error for an
-// unknown variant, a scene beyond LDS or ppu outside [1, 256].
+// Variants: (256, 5), (256, 4), (1024, 4), each with the scene in LDS or,
+// when the image and the block's rings do not fit, read from the global
+// image (gscene, set here).  Fills in grid and gscene; an error for an
+// unknown variant or ppu outside [1, 256].
 hipError_t sva_shape_finish(const SceneImg& img, SvaShape* sh, int device);
 // The same for the multi-scene kernel whose largest scene image has img_bytes.
 hipError_t sva_multi_shape_finish(size_t img_bytes, SvaShape* sh, int device);

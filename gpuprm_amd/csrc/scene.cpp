@@ -301,6 +301,30 @@ Polytope obstacle_hull(Rng& r, int npts) {
   }
 }
 
+// Hull of npts points on a sphere of radius in [rmin, rmax] (every point a
+// vertex): many vertices per obstacle.
+Polytope obstacle_shell(Rng& r, int npts, double rmin, double rmax) {
+  for (;;) {
+    double c[3];
+    random_center(r, c);
+    double rad = r.uniform(rmin, rmax);
+    std::vector<double> pts;
+    for (int i = 0; i < npts; ++i) {
+      double x, y, z, n2;
+      do {
+        x = r.uniform(-1, 1); y = r.uniform(-1, 1); z = r.uniform(-1, 1);
+        n2 = x * x + y * y + z * z;
+      } while (n2 > 1.0 || n2 < 0.05);
+      const double inv = 1.0 / std::sqrt(n2);
+      pts.push_back(c[0] + rad * x * inv);
+      pts.push_back(c[1] + rad * y * inv);
+      pts.push_back(c[2] + rad * z * inv);
+    }
+    Polytope p;
+    if (hull_polytope(pts, &p)) return p;
+  }
+}
+
 void set_arm(Scene* s, int ndof, double a, const double* alpha_cycle, int ncycle, double link_w,
              int nobs_polys) {
   s->ndof = ndof;
@@ -351,6 +375,11 @@ std::string generate(int workload, uint64_t seed, Scene* out) {
     ndof = 6; a = 0.8; w = 0.15; cyc = {0.0, hp, -hp};
   } else if (workload == 3) {
     for (int i = 0; i < 32; ++i) obs.push_back(obstacle_hull(r, 16));
+    ndof = 7; a = 0.6; w = 0.12; cyc = {0.0, hp, -hp};
+  } else if (workload == 6) {
+    // a scene whose image exceeds LDS (about 300 KB): 256 small obstacles of
+    // 32 vertices each
+    for (int i = 0; i < 256; ++i) obs.push_back(obstacle_shell(r, 32, 0.06, 0.15));
     ndof = 7; a = 0.6; w = 0.12; cyc = {0.0, hp, -hp};
   } else if (workload == 5) {
     int nobs = r.integer(8, 128);

@@ -28,7 +28,9 @@ extern "C" {
 #define GPRM_E_INVALID (-1)   /* bad argument (null pointer, N < 0, bad index) */
 #define GPRM_E_SCENE (-2)     /* scene could not be read or is malformed */
 #define GPRM_E_HIP (-3)       /* HIP runtime error (no device, launch failure) */
-#define GPRM_E_CAPACITY (-4)  /* scene exceeds what the kernels stage in LDS */
+#define GPRM_E_CAPACITY (-4)  /* scene or call beyond a stated limit (vertex count,
+                                 degree, a multi-scene batch's LDS, the no-alloc
+                                 maximum) */
 #define GPRM_E_REQUEST (-5)   /* unknown or already completed request id */
 #define GPRM_E_STATE (-6)     /* context not initialised (call gprm_init) */
 
@@ -330,8 +332,9 @@ const char* gprm_batch_last_error(gprm_batch* batch);  /* NULL: error of the las
 const char* gprm_last_error(gprm_ctx* ctx);
 
 /* Scene generators of the benchmark workloads (1: 4-DOF boxes, 2: 6-DOF
- * hulls, 3: 7-DOF hulls, 5: 7-DOF with random obstacle/hull counts), written
- * in the reference's directory format. */
+ * hulls, 3: 7-DOF hulls, 5: 7-DOF with random obstacle/hull counts, 6: 7-DOF
+ * among 256 obstacles of 32 vertices, a scene image beyond LDS), written in
+ * the reference's directory format. */
 int gprm_scene_generate(int workload, unsigned long long seed, const char* out_dir);
 
 #ifdef __cplusplus
