@@ -452,10 +452,6 @@ __global__ void __launch_bounds__(kFkBlock) fk_edges_multi_kernel(const EdgeTask
 // exclusive scan, and a scatter of the coordinates (SoA, stride n) and node
 // ids into cell order.
 
-__device__ __forceinline__ int grid_coord(const CellGrid& g, int d, float x) {
-  int c = (int)floorf((x - g.lo[d]) * g.inv[d]);
-  return c < 0 ? 0 : (c >= g.cells[d] ? g.cells[d] - 1 : c);
-}
 
 __global__ void grid_count_kernel(CellGrid g, const float* __restrict__ nodes, int n, long long noff,
                                   int* __restrict__ cell_id, int* __restrict__ counts) {
@@ -525,11 +521,13 @@ __global__ void __launch_bounds__(256)
   for (int i = 0; i < ndof; ++i) qr[i] = queries[j + qoff * i];
   if (threadIdx.x < NK) {
     // neighbour k: offset digit[d] - 1 in grid dimension d; a dimension
-    // without grid (d >= dims) takes only offset 0
+    // without grid (d >= dims) takes only offset 0; a cell whose faces lie
+    // D or more from the query is skipped (cell_gap2)
     int digit[kGridDims];
     for (int d = 0, rem = threadIdx.x; d < kGridDims; ++d, rem /= 3) digit[d] = rem % 3;
     int c = 0;
     bool ok = true;
+    float lb = 0.0f;
     for (int d = kGridDims - 1; d >= 0; --d) {
       if (d >= g.dims) {
         if (digit[d] != 1) ok = false;
@@ -538,7 +536,9 @@ __global__ void __launch_bounds__(256)
       const int x = grid_coord(g, d, qr[d]) + digit[d] - 1;
       if (x < 0 || x >= g.cells[d]) ok = false;
       c = c * g.cells[d] + x;
+      lb += cell_gap2(g, d, qr[d], digit[d]);
     }
+    if (lb >= D2) ok = false;
     s_start[threadIdx.x] = ok ? starts[c] : 0;
     s_pre[threadIdx.x + 1] = ok ? counts[c] : 0;
   }

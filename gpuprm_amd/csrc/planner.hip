@@ -234,10 +234,6 @@ __global__ void __launch_bounds__(kScanBlock)
 }
 
 // ---- radius search -----------------------------------------------------------
-__device__ __forceinline__ int grid_coord(const CellGrid& g, int d, float x) {
-  int c = (int)floorf((x - g.lo[d]) * g.inv[d]);
-  return c < 0 ? 0 : (c >= g.cells[d] ? g.cells[d] - 1 : c);
-}
 
 constexpr int pow3(int d) { return d == 0 ? 1 : 3 * pow3(d - 1); }
 constexpr int kNeighbourCells = pow3(kGridDims);
@@ -329,7 +325,9 @@ __global__ void __launch_bounds__(256)
   int cstart = 0, ccount = 0;
   if (threadIdx.x < NK) {
     // neighbour k: offset digit(d) - 1 in grid dimension d; a dimension
-    // without grid (d >= dims) takes only offset 0
+    // without grid (d >= dims) takes only offset 0.  A cell whose faces lie
+    // D or more from the query is skipped (on a grid of cells D wide about
+    // half of the 3^5).
     int rem = threadIdx.x, c = 0;
     bool ok = true;
     int digit[kGridDims];
@@ -337,6 +335,7 @@ __global__ void __launch_bounds__(256)
       digit[d] = rem % 3;
       rem /= 3;
     }
+    float lb = 0.0f;
     for (int d = kGridDims - 1; d >= 0; --d) {
       if (d >= g.dims) {
         if (digit[d] != 1) ok = false;
@@ -345,7 +344,9 @@ __global__ void __launch_bounds__(256)
       const int x = grid_coord(g, d, qr[d]) + digit[d] - 1;
       if (x < 0 || x >= g.cells[d]) ok = false;
       c = c * g.cells[d] + x;
+      lb += cell_gap2(g, d, qr[d], digit[d]);
     }
+    if (lb >= D2) ok = false;
     if (ok) {
       cstart = T.starts[c];
       ccount = T.counts[c];

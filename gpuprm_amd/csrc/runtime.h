@@ -127,6 +127,29 @@ struct CellGrid {
   int ncells;
 };
 
+// Cell coordinate of x in grid dimension d (clamped to the grid).
+__device__ __forceinline__ int grid_coord(const CellGrid& g, int d, float x) {
+  int c = (int)floorf((x - g.lo[d]) * g.inv[d]);
+  return c < 0 ? 0 : (c >= g.cells[d] ? g.cells[d] - 1 : c);
+}
+
+// Squared lower bound, in dimension d, on the distance from x to any node of
+// the neighbour cell at offset digit - 1 from x's own cell: the gap to the
+// face the two cells share (nodes of the cell below have (y - lo) inv < c,
+// of the cell above >= c + 1, as grid_coord bins them).  The gap is first
+// shortened by a thousandth of a cell, far above the rounding of these
+// quantities, so a cell whose bounds sum to D^2 or more holds no node the
+// exact test (distance^2 < D^2) would match.
+__device__ __forceinline__ float cell_gap2(const CellGrid& g, int d, float x, int digit) {
+  if (digit == 1) return 0.0f;
+  const float t = (x - g.lo[d]) * g.inv[d];
+  const float c = (float)grid_coord(g, d, x);
+  const float gap = (digit == 0 ? t - c : c + 1.0f - t) - 1e-3f;
+  if (!(gap > 0.0f)) return 0.0f;
+  const float len = gap / g.inv[d];
+  return len * len;
+}
+
 // Result of an edge left unchecked by EdgeArgs::skip (nonzero: never taken
 // as free).
 constexpr int kEdgeSkipped = 0x40000000;
