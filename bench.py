@@ -780,7 +780,7 @@ def main():
                          "MIB MiB once each (the known-size kernel that scales HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus %d)"
+        print("bench.py runs on one GPU only: the library has no multi-GPU path (asked for --gpus %d)"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
 

@@ -16,7 +16,7 @@ seperating_vector_algorithm (collision4.h:227-332) for every pair of a few
 configurations; edges with the reference's indicator2 host-loop results
 (robotconfigspace.cpp:389-465,694-705) and its sample count.
 
-Usage (build container only; the reference is not on the GPU box):
+Usage (where the reference sources are, under /root/reference):
     python tests/golden/make_golden.py
 """
 from __future__ import annotations
