@@ -1056,7 +1056,7 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, const 
       V3 cQw{0, 0, 0}, PqW{0, 0, 0};
       // Q's box axes in the world frame, times its half extents
       V3 aQ0{0, 0, 0}, aQ1{0, 0, 0}, aQ2{0, 0, 0};
-      float magQ = 0.0f;
+      float magQ = 0.0f, rQ = 0.0f;
       for (; pair < p1; ++pair) {
         // the record is the same in every lane: keep it in scalar registers
         const int4 pdv = L.pairs[pair];
@@ -1085,6 +1085,7 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, const 
           const float4 b0 = L.spheres[2 * kQ], b1 = L.spheres[2 * kQ + 1];
           cQw = sysQ0 ? v3(b0) : t_apply(tq, v3(b0));
           magQ = fabsf(cQw.x) + fabsf(cQw.y) + fabsf(cQw.z) + b0.w;
+          rQ = b0.w;
           if (sysQ0) {
             aQ0 = V3{b1.x, 0.0f, 0.0f};
             aQ1 = V3{0.0f, b1.y, 0.0f};
@@ -1122,28 +1123,40 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, const 
             const V3 vp = v3(L.verts[vbP]);
             const V3 Pp = sysP0 ? vp : t_apply(tp, vp);
             const V3 S{PqW.x - Pp.x, PqW.y - Pp.y, PqW.z - Pp.z};
-            // support extents along S: the box's sum_a |S.axis_a| h_a (axes:
-            // the frame's rotation columns).  The bounding sphere's |S| r is
-            // not used: on the benchmark scenes it decides under 0.1% more pair
-            // tests than the box alone, for a square root and five operations
-            // per pair.  The magnitude term of the margin is |c|_1 + r per side
-            // (precomputed for world-frame polytopes).
+            // First the bounding spheres: every vertex lies within r of its
+            // polytope's centre c, so S.(q - p) >= S.(cQ - cP) - (rP + rQ)|S|.
+            // That costs a few operations once cP is in the world frame and
+            // decides most pairs; the boxes' support extents along S (sum_a
+            // |S.axis_a| h_a, axes the frame's rotation columns) are tighter
+            // and are computed only for the pairs the spheres leave open.
+            // The margin is the same in both: 2^-10 of |S|_1 times the
+            // magnitudes |c|_1 + r per side (precomputed for world-frame
+            // polytopes); the spheres' comparison is made on squares (no
+            // square root), both sides non-negative, the right one inflated
+            // by 2^-8 against their rounding.
             const float4 b0 = L.spheres[2 * kP], b1 = L.spheres[2 * kP + 1];
             const V3 cPw = sysP0 ? v3(b0) : t_apply(tp, v3(b0));
-            const float boxQ = (fabsf(dot3(S, aQ0)) + fabsf(dot3(S, aQ1))) + fabsf(dot3(S, aQ2));
-            float boxP, magP;
-            if (sysP0) {
-              boxP = (fabsf(S.x) * b1.x + fabsf(S.y) * b1.y) + fabsf(S.z) * b1.z;
-              magP = b1.w;
-            } else {
-              const V3 sp = t_apply_rot_inv(tp, S);
-              boxP = (fabsf(sp.x) * b1.x + fabsf(sp.y) * b1.y) + fabsf(sp.z) * b1.z;
-              magP = fabsf(cPw.x) + fabsf(cPw.y) + fabsf(cPw.z) + b0.w;
-            }
+            const float magP = sysP0 ? b1.w : fabsf(cPw.x) + fabsf(cPw.y) + fabsf(cPw.z) + b0.w;
             const float sn1 = fabsf(S.x) + fabsf(S.y) + fabsf(S.z);
-            const float gap = dot3(S, V3{cQw.x - cPw.x, cQw.y - cPw.y, cQw.z - cPw.z}) - boxQ - boxP;
-            const float mag = magQ + magP;
-            if (sn1 > 1e-5f && gap > 0x1p-10f * mag * sn1) {
+            const float dc = dot3(S, V3{cQw.x - cPw.x, cQw.y - cPw.y, cQw.z - cPw.z});
+            const float thr = 0x1p-10f * (magQ + magP) * sn1;
+            bool sep = false;
+            if (sn1 > 1e-5f && dc > thr) {
+              const float A = dc - thr, R = b0.w + rQ;
+              sep = A * A > (R * R) * dot3(S, S) * (1.0f + 0x1p-8f);
+              if (!sep) {
+                const float boxQ = (fabsf(dot3(S, aQ0)) + fabsf(dot3(S, aQ1))) + fabsf(dot3(S, aQ2));
+                float boxP;
+                if (sysP0) {
+                  boxP = (fabsf(S.x) * b1.x + fabsf(S.y) * b1.y) + fabsf(S.z) * b1.z;
+                } else {
+                  const V3 sp = t_apply_rot_inv(tp, S);
+                  boxP = (fabsf(sp.x) * b1.x + fabsf(sp.y) * b1.y) + fabsf(sp.z) * b1.z;
+                }
+                sep = dc - boxQ - boxP > thr;
+              }
+            }
+            if (sep) {
               ++n_bound;
               ++n_sep;
             } else {
