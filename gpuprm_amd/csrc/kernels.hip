@@ -1099,71 +1099,68 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, const 
         }
         if (EDGE && active && pair == p0 + 2)
           gate = __hip_atomic_load(&A.edge_res[edge], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ? 0 : 0x7fffffff;
-        bool defer = false;
-        if (active) {
-          if (pair >= p0 + 4 && (EDGE ? gate == 0 : gate <= crs)) {
-            ++n_dropped;
+        // Every lane evaluates the bound test (a SIMD runs the whole wave
+        // anyway); the lane's conditions (its item active, its pair not
+        // dropped by the gate) are applied to the result, which keeps the
+        // exec-mask switching out of the loop.
+        const bool drop = active && pair >= p0 + 4 && (EDGE ? gate == 0 : gate <= crs);
+        const bool test = active && !drop;
+        // Start of the pair (collision4.h:228-244) and the bound decision
+        // of iteration 0: iteration 0 returns 0 whenever
+        // S.normalize(q0 - p0) >= -1e-6 for the support vertices p0, q0
+        // the climbs find, S = normalize(D) with D the difference of the
+        // two start vertices.  If the bounding boxes show S.(q - p) > 0
+        // with a wide margin for every vertex p of P and q of Q, that holds
+        // whichever vertices the climbs would pick, so the pair test ends
+        // here exactly as the reference's does (code 0, one
+        // iteration).  The margin, 2^-10 of the magnitudes involved,
+        // exceeds the rounding error of every quantity in the test and in
+        // the reference's iteration by orders of magnitude.  Both sides of
+        // the test scale with |S|, so it runs on D itself: the reference's
+        // S differs from D/|D| by a relative 1e-7, far inside the margin.
+        // The test also requires |D|_1 > 1e-5, so |D|_2 > 5.7e-6 stays
+        // clear of the reference's override of S for |D| <= 1e-6; a NaN D
+        // never passes (such pairs are deferred to the exact test).
+        const V3 vp = v3(L.verts[vbP]);
+        const V3 Pp = sysP0 ? vp : t_apply(tp, vp);
+        const V3 S{PqW.x - Pp.x, PqW.y - Pp.y, PqW.z - Pp.z};
+        // First the bounding spheres: every vertex lies within r of its
+        // polytope's centre c, so S.(q - p) >= S.(cQ - cP) - (rP + rQ)|S|.
+        // That costs a few operations once cP is in the world frame and
+        // decides most pairs; the boxes' support extents along S (sum_a
+        // |S.axis_a| h_a, axes the frame's rotation columns) are tighter
+        // and are computed only when some lane's pair the spheres leave
+        // open.  The margin is the same in both: 2^-10 of |S|_1 times the
+        // magnitudes |c|_1 + r per side (precomputed for world-frame
+        // polytopes); the spheres' comparison is made on squares (no square
+        // root), both sides non-negative, the right one inflated by 2^-8
+        // against their rounding.
+        const float4 b0 = L.spheres[2 * kP], b1 = L.spheres[2 * kP + 1];
+        const V3 cPw = sysP0 ? v3(b0) : t_apply(tp, v3(b0));
+        const float magP = sysP0 ? b1.w : fabsf(cPw.x) + fabsf(cPw.y) + fabsf(cPw.z) + b0.w;
+        const float sn1 = fabsf(S.x) + fabsf(S.y) + fabsf(S.z);
+        const float dc = dot3(S, V3{cQw.x - cPw.x, cQw.y - cPw.y, cQw.z - cPw.z});
+        const float thr = 0x1p-10f * (magQ + magP) * sn1;
+        const bool open = sn1 > 1e-5f && dc > thr;
+        const float A = dc - thr, rsum = b0.w + rQ;
+        bool sep = open && A * A > (rsum * rsum) * dot3(S, S) * (1.0f + 0x1p-8f);
+        const bool need_box = test && open && !sep;
+        if (__ballot(need_box) != 0ull) {
+          const float boxQ = (fabsf(dot3(S, aQ0)) + fabsf(dot3(S, aQ1))) + fabsf(dot3(S, aQ2));
+          float boxP;
+          if (sysP0) {
+            boxP = (fabsf(S.x) * b1.x + fabsf(S.y) * b1.y) + fabsf(S.z) * b1.z;
           } else {
-            // Start of the pair (collision4.h:228-244) and the bound decision
-            // of iteration 0: iteration 0 returns 0 whenever
-            // S.normalize(q0 - p0) >= -1e-6 for the support vertices p0, q0
-            // the climbs find, S = normalize(D) with D the difference of the
-            // two start vertices.  If the bounding boxes show S.(q - p) > 0
-            // with a wide margin for every vertex p of P and q of Q, that holds
-            // whichever vertices the climbs would pick, so the pair test ends
-            // here exactly as the reference's does (code 0, one
-            // iteration).  The margin, 2^-10 of the magnitudes involved,
-            // exceeds the rounding error of every quantity in the test and in
-            // the reference's iteration by orders of magnitude.  Both sides of
-            // the test scale with |S|, so it runs on D itself: the reference's
-            // S differs from D/|D| by a relative 1e-7, far inside the margin.
-            // The test also requires |D|_1 > 1e-5, so |D|_2 > 5.7e-6 stays
-            // clear of the reference's override of S for |D| <= 1e-6; a NaN D
-            // never passes (such pairs are deferred to the exact test).
-            const V3 vp = v3(L.verts[vbP]);
-            const V3 Pp = sysP0 ? vp : t_apply(tp, vp);
-            const V3 S{PqW.x - Pp.x, PqW.y - Pp.y, PqW.z - Pp.z};
-            // First the bounding spheres: every vertex lies within r of its
-            // polytope's centre c, so S.(q - p) >= S.(cQ - cP) - (rP + rQ)|S|.
-            // That costs a few operations once cP is in the world frame and
-            // decides most pairs; the boxes' support extents along S (sum_a
-            // |S.axis_a| h_a, axes the frame's rotation columns) are tighter
-            // and are computed only for the pairs the spheres leave open.
-            // The margin is the same in both: 2^-10 of |S|_1 times the
-            // magnitudes |c|_1 + r per side (precomputed for world-frame
-            // polytopes); the spheres' comparison is made on squares (no
-            // square root), both sides non-negative, the right one inflated
-            // by 2^-8 against their rounding.
-            const float4 b0 = L.spheres[2 * kP], b1 = L.spheres[2 * kP + 1];
-            const V3 cPw = sysP0 ? v3(b0) : t_apply(tp, v3(b0));
-            const float magP = sysP0 ? b1.w : fabsf(cPw.x) + fabsf(cPw.y) + fabsf(cPw.z) + b0.w;
-            const float sn1 = fabsf(S.x) + fabsf(S.y) + fabsf(S.z);
-            const float dc = dot3(S, V3{cQw.x - cPw.x, cQw.y - cPw.y, cQw.z - cPw.z});
-            const float thr = 0x1p-10f * (magQ + magP) * sn1;
-            bool sep = false;
-            if (sn1 > 1e-5f && dc > thr) {
-              const float A = dc - thr, R = b0.w + rQ;
-              sep = A * A > (R * R) * dot3(S, S) * (1.0f + 0x1p-8f);
-              if (!sep) {
-                const float boxQ = (fabsf(dot3(S, aQ0)) + fabsf(dot3(S, aQ1))) + fabsf(dot3(S, aQ2));
-                float boxP;
-                if (sysP0) {
-                  boxP = (fabsf(S.x) * b1.x + fabsf(S.y) * b1.y) + fabsf(S.z) * b1.z;
-                } else {
-                  const V3 sp = t_apply_rot_inv(tp, S);
-                  boxP = (fabsf(sp.x) * b1.x + fabsf(sp.y) * b1.y) + fabsf(sp.z) * b1.z;
-                }
-                sep = dc - boxQ - boxP > thr;
-              }
-            }
-            if (sep) {
-              ++n_bound;
-              ++n_sep;
-            } else {
-              defer = true;
-            }
+            const V3 sp = t_apply_rot_inv(tp, S);
+            boxP = (fabsf(sp.x) * b1.x + fabsf(sp.y) * b1.y) + fabsf(sp.z) * b1.z;
           }
+          sep = sep || (open && dc - boxQ - boxP > thr);
         }
+        sep = sep && test;
+        n_dropped += drop ? 1u : 0u;
+        n_bound += sep ? 1u : 0u;
+        n_sep += sep ? 1u : 0u;
+        const bool defer = test && !sep;
         // append the undecided pairs to the ring; drain it once CAP - 64 wait
         const unsigned long long dm = __ballot(defer);
         if (defer) {
