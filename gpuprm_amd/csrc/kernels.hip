@@ -1066,22 +1066,23 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, const 
         const int vbP = pd.x & 0xffff, vbQ = pd.x >> 16;
         const int sP = pd.y & 255, sQ = pd.y >> 8;
         const bool sysP0 = sP == 0, sysQ0 = sQ == 0;
-        if (!sysP0 && sP != fsP) {
+        // the cache tags are wave-uniform: compared as scalars (readfirstlane)
+        if (!sysP0 && sP != __builtin_amdgcn_readfirstlane(fsP)) {
           if (active) tp = load_frame(A.frames, item, A.fstride, sP);
           fsP = sP;
         }
-        if (!sysQ0 && sQ != fsQ) {
+        if (!sysQ0 && sQ != __builtin_amdgcn_readfirstlane(fsQ)) {
           if (active) tq = load_frame(A.frames, item, A.fstride, sQ);
           fsQ = sQ;
           ckQ = cvQ = -1;
         }
         const int kP = pd.w & 0xffff, kQ = pd.w >> 16;
-        if (vbQ != cvQ) {
+        if (vbQ != __builtin_amdgcn_readfirstlane(cvQ)) {
           const V3 vq = v3(L.verts[vbQ]);
           PqW = sysQ0 ? vq : t_apply(tq, vq);
           cvQ = vbQ;
         }
-        if (kQ != ckQ) {
+        if (kQ != __builtin_amdgcn_readfirstlane(ckQ)) {
           const float4 b0 = L.spheres[2 * kQ], b1 = L.spheres[2 * kQ + 1];
           cQw = sysQ0 ? v3(b0) : t_apply(tq, v3(b0));
           magQ = fabsf(cQw.x) + fabsf(cQw.y) + fabsf(cQw.z) + b0.w;
@@ -1103,8 +1104,9 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, const 
         // anyway); the lane's conditions (its item active, its pair not
         // dropped by the gate) are applied to the result, which keeps the
         // exec-mask switching out of the loop.
-        const bool drop = active && pair >= p0 + 4 && (EDGE ? gate == 0 : gate <= crs);
-        const bool test = active && !drop;
+        // (bitwise & on the conditions: no short-circuit branches)
+        const bool drop = active & (pair >= p0 + 4) & (EDGE ? gate == 0 : gate <= crs);
+        const bool test = active & !drop;
         // Start of the pair (collision4.h:228-244) and the bound decision
         // of iteration 0: iteration 0 returns 0 whenever
         // S.normalize(q0 - p0) >= -1e-6 for the support vertices p0, q0
@@ -1141,10 +1143,10 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, const 
         const float sn1 = fabsf(S.x) + fabsf(S.y) + fabsf(S.z);
         const float dc = dot3(S, V3{cQw.x - cPw.x, cQw.y - cPw.y, cQw.z - cPw.z});
         const float thr = 0x1p-10f * (magQ + magP) * sn1;
-        const bool open = sn1 > 1e-5f && dc > thr;
+        const bool open = (sn1 > 1e-5f) & (dc > thr);
         const float A = dc - thr, rsum = b0.w + rQ;
-        bool sep = open && A * A > (rsum * rsum) * dot3(S, S) * (1.0f + 0x1p-8f);
-        const bool need_box = test && open && !sep;
+        bool sep = open & (A * A > (rsum * rsum) * dot3(S, S) * (1.0f + 0x1p-8f));
+        const bool need_box = test & open & !sep;
         if (__ballot(need_box) != 0ull) {
           const float boxQ = (fabsf(dot3(S, aQ0)) + fabsf(dot3(S, aQ1))) + fabsf(dot3(S, aQ2));
           float boxP;
@@ -1154,13 +1156,13 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, const 
             const V3 sp = t_apply_rot_inv(tp, S);
             boxP = (fabsf(sp.x) * b1.x + fabsf(sp.y) * b1.y) + fabsf(sp.z) * b1.z;
           }
-          sep = sep || (open && dc - boxQ - boxP > thr);
+          sep = sep | (open & (dc - boxQ - boxP > thr));
         }
-        sep = sep && test;
+        sep = sep & test;
         n_dropped += drop ? 1u : 0u;
         n_bound += sep ? 1u : 0u;
         n_sep += sep ? 1u : 0u;
-        const bool defer = test && !sep;
+        const bool defer = test & !sep;
         // append the undecided pairs to the ring; drain it once CAP - 64 wait
         const unsigned long long dm = __ballot(defer);
         if (defer) {
