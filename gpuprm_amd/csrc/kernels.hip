@@ -1066,13 +1066,16 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, const 
         const int vbP = pd.x & 0xffff, vbQ = pd.x >> 16;
         const int sP = pd.y & 255, sQ = pd.y >> 8;
         const bool sysP0 = sP == 0, sysQ0 = sQ == 0;
-        // the cache tags are wave-uniform: compared as scalars (readfirstlane)
+        // the cache tags are wave-uniform: compared as scalars
+        // (readfirstlane).  Frames are loaded by every lane: an inactive
+        // lane's item is below the plane stride (the item count rounded up
+        // to 64), and what it computes from them is masked out below.
         if (!sysP0 && sP != __builtin_amdgcn_readfirstlane(fsP)) {
-          if (active) tp = load_frame(A.frames, item, A.fstride, sP);
+          tp = load_frame(A.frames, item, A.fstride, sP);
           fsP = sP;
         }
         if (!sysQ0 && sQ != __builtin_amdgcn_readfirstlane(fsQ)) {
-          if (active) tq = load_frame(A.frames, item, A.fstride, sQ);
+          tq = load_frame(A.frames, item, A.fstride, sQ);
           fsQ = sQ;
           ckQ = cvQ = -1;
         }
@@ -1098,7 +1101,7 @@ __device__ __forceinline__ void sva_units(const SvaArgs& A, const Lds& L, const 
           }
           ckQ = kQ;
         }
-        if (EDGE && active && pair == p0 + 2)
+        if (EDGE && pair == p0 + 2)  // an inactive lane reads edge 0's result, unused
           gate = __hip_atomic_load(&A.edge_res[edge], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ? 0 : 0x7fffffff;
         // Every lane evaluates the bound test (a SIMD runs the whole wave
         // anyway); the lane's conditions (its item active, its pair not
