@@ -561,10 +561,7 @@ __global__ void __launch_bounds__(256)
       for (int step = 128; step > 0; step >>= 1)
         if (k + step < NK && s_pre[k + step] <= idx) k += step;
       const int pos = s_start[k] + (idx - s_pre[k]);
-      for (int i = 0; i < ndof; ++i) {
-        float diff = sorted[pos + (size_t)n * i] - qr[i];
-        normsq += diff * diff;
-      }
+      node_dist2<1>(sorted, (size_t)n, &pos, ndof, qr, &normsq);
       hit = normsq < D2;
       node = sid[pos];
     }

@@ -241,18 +241,20 @@ static_assert(kNeighbourCells <= 256, "one thread per neighbour cell in a 256-th
 
 // One block per group b = 2 j + g: the nonempty cells among the 3^dims
 // around new node j are compacted (block scans) and the block strides
-// through their nodes in cell order, 256 candidates a chunk.  One pass in
-// either mode.
+// through their nodes in cell order, 256 candidates a chunk (512 in APPEND
+// mode: two a thread, their loads in flight together).  In either mode
+// the matches are staged in LDS (per-wave ballots, one barrier a chunk), so
+// no chunk waits on a global atomic.
 //
-// APPEND (no edge budget, so no group order needed): each chunk's matches
-// take their slots from info->E_all with one atomic per block (waves take
-// block-local ranges by LDS atomic), the first nbuf slots are stored,
-// cnt[b] gets the group's stored edges.
+// APPEND (no edge budget, so no group order needed): the group takes one
+// range of info->E_all for its staged matches at the end, and one more each
+// time the stage fills before that; the first nbuf slots are stored, cnt[b]
+// gets the group's stored edges.
 //
 // GROUP (edge budget: the ranking needs each group's edges contiguous):
-// the matches are staged in LDS with their traversal index (per-wave
-// ballots, one barrier a chunk); a group keeps at most `keep` of them, its
-// shortest (ties by traversal index), keep = min(kGroupKeep, nbuf / groups),
+// the matches are staged with their traversal index; a group keeps at most
+// `keep` of them, its shortest (ties by traversal index),
+// keep = min(kGroupKeep, nbuf / groups),
 // so a step never has more than nbuf and the kept edges do not depend on
 // scheduling.  When the stage would overflow, and at the end when more than
 // keep are staged, a bitonic sort in LDS orders the stage and cuts it to
@@ -264,6 +266,10 @@ enum NearMode { kNearAppend = 0, kNearGroup = 1 };
 // search is latency-bound, so occupancy counts)
 constexpr int kGroupStage = 1024;
 constexpr int kGroupKeep = kGroupStage / 2;
+// APPEND: two candidates per thread and chunk (twice the loads in flight, half
+// the barriers); the stage takes two chunks
+constexpr int kAppendPer = 2;
+static_assert(kGroupStage >= 2 * kAppendPer * 256, "append stage");
 
 // Ascending bitonic sort of the first n stage entries by (squared distance,
 // traversal index), padded to a power of two with +inf; n <= kGroupStage.
@@ -303,15 +309,16 @@ __global__ void __launch_bounds__(256)
                      int* __restrict__ off, int nbuf, int keep, int* __restrict__ enode,
                      float* __restrict__ ed2, int* __restrict__ egroup) {
   constexpr bool GROUP = MODE == kNearGroup;
+  // candidates per thread and chunk
+  constexpr int PER = GROUP ? 1 : kAppendPer;
   constexpr int NK = kNeighbourCells;
   constexpr int kWaves = 256 / 64;
   __shared__ int s_start[NK], s_pre[NK + 1];
   __shared__ int lds[kMaxWaves + 1];
-  __shared__ int s_count;
-  __shared__ int s_base, s_chunk[2];
+  __shared__ int s_base;
   __shared__ int s_wcnt[2][kWaves];
-  __shared__ int s_node[GROUP ? kGroupStage : 1];
-  __shared__ float s_d2[GROUP ? kGroupStage : 1];
+  __shared__ int s_node[kGroupStage];
+  __shared__ float s_d2[kGroupStage];
   __shared__ int s_idx[GROUP ? kGroupStage : 1];
   const int b = blockIdx.x, j = b >> 1;
   if (j >= info->nnew) {
@@ -359,37 +366,25 @@ __global__ void __launch_bounds__(256)
     s_start[slot] = cstart;
     s_pre[slot] = pre;
   }
-  if (threadIdx.x == 0) {
-    s_pre[K] = total;
-    s_count = 0;
-    s_chunk[0] = 0;
-  }
+  if (threadIdx.x == 0) s_pre[K] = total;
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const unsigned long long below = (1ull << lane) - 1ull;
-  // candidate idx of the group's traversal: its node and squared distance
-  auto visit = [&](int idx, int& node, float& normsq) -> bool {
-    normsq = 0.0f;
-    node = 0;
-    if (idx >= total) return false;
+  // position in the tree's cell order of candidate idx < total of the
+  // group's traversal
+  auto cell_pos = [&](int idx) -> int {
     int k = 0;  // the nonempty cell holding idx: last k with s_pre[k] <= idx
     for (int step = 128; step > 0; step >>= 1)
       if (k + step < K && s_pre[k + step] <= idx) k += step;
-    const int pos = s_start[k] + (idx - s_pre[k]);
-    for (int i = 0; i < ndof; ++i) {
-      const float diff = T.sorted[pos + (size_t)cap * i] - qr[i];
-      normsq += diff * diff;
-    }
-    node = T.sid[pos];
-    return normsq < D2;
+    return s_start[k] + (idx - s_pre[k]);
   };
-  // rank of this thread's match among the group's matches so far: the waves'
-  // counts of the chunk go to one of two LDS rows (alternating, so a row is
-  // rewritten only after every thread has passed the next chunk's barrier)
-  auto rank_in_chunk = [&](bool hit, int& tot) -> int {
-    const unsigned long long m = __ballot(hit);
+  // matches of the chunk in the waves before this one (tot: in the block),
+  // from each wave's count cw: the counts go to one of two LDS rows
+  // (alternating, so a row is rewritten only after every thread has passed
+  // the next chunk's barrier)
+  auto waves_before = [&](int cw, int& tot) -> int {
     int* const row = s_wcnt[chunk_no & 1];
-    if (lane == 0) row[wv] = __popcll(m);
+    if (lane == 0) row[wv] = cw;
     __syncthreads();
     int before = 0;
     tot = 0;
@@ -400,80 +395,86 @@ __global__ void __launch_bounds__(256)
       tot += c;
     }
     ++chunk_no;
-    return before + __popcll(m & below);
+    return before;
   };
-  if (GROUP) {
-    int run = 0, n = 0;  // matches so far, staged entries (the same in every thread)
-    for (int base = 0; base < total; base += blockDim.x) {
-      if (n + (int)blockDim.x > kGroupStage) {
+  // matches so far, staged entries, entries stored by earlier flushes
+  // (APPEND); the same in every thread
+  int run = 0, n = 0, stored = 0;
+  for (int base = 0; base < total; base += PER * blockDim.x) {
+    if (n + PER * (int)blockDim.x > kGroupStage) {
+      if (GROUP) {
         stage_sort(s_d2, s_idx, s_node, n);
         n = min(n, keep);
+      } else {
+        __syncthreads();  // the last chunk's entries
+        if (threadIdx.x == 0) s_base = atomicAdd(&info->E_all, n);
+        __syncthreads();
+        const int o = s_base;
+        const int kept = max(0, min(n, nbuf - o));
+        for (int r = threadIdx.x; r < kept; r += blockDim.x) {
+          enode[o + r] = s_node[r];
+          ed2[o + r] = s_d2[r];
+          egroup[o + r] = b;
+        }
+        stored += kept;
+        n = 0;
+        __syncthreads();  // the stage is read before it is refilled
       }
-      int node;
-      float normsq;
-      const bool hit = visit(base + threadIdx.x, node, normsq);
-      int tot;
-      const int r = rank_in_chunk(hit, tot);
-      if (hit) {
-        s_node[n + r] = node;
-        s_d2[n + r] = normsq;
-        s_idx[n + r] = run + r;
+    }
+    // candidates base + u 256 + t for u < PER: their loads are in flight
+    // together (a lane past the end reads the last candidate and matches
+    // nothing); a wave's matches are staged u-major (GROUP, PER = 1: in
+    // traversal order)
+    int pos[PER], node[PER];
+    float normsq[PER];
+    bool hit[PER];
+    unsigned long long m[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) pos[u] = cell_pos(min(base + u * (int)blockDim.x + (int)threadIdx.x, total - 1));
+#pragma unroll
+    for (int u = 0; u < PER; ++u) node[u] = T.sid[pos[u]];
+    node_dist2<PER>(T.sorted, (size_t)cap, pos, ndof, qr, normsq);
+    int cw = 0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      hit[u] = base + u * (int)blockDim.x + (int)threadIdx.x < total && normsq[u] < D2;
+      m[u] = __ballot(hit[u]);
+      cw += __popcll(m[u]);
+    }
+    int tot;
+    int r = n + waves_before(cw, tot);
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      if (hit[u]) {
+        const int at = r + __popcll(m[u] & below);
+        s_node[at] = node[u];
+        s_d2[at] = normsq[u];
+        if (GROUP) s_idx[at] = run + (at - n);
       }
-      run += tot;
-      n += tot;
+      r += __popcll(m[u]);
     }
-    if (n > keep) {
-      __syncthreads();
-      stage_sort(s_d2, s_idx, s_node, n);
-      n = keep;
-    }
-    if (threadIdx.x == 0) s_base = n > 0 ? atomicAdd(&info->E_all, n) : 0;
-    __syncthreads();
-    const int o = s_base;
-    const int kept = max(0, min(n, nbuf - o));  // n <= keep <= nbuf / groups: never cut
-    if (threadIdx.x == 0) {
-      cnt[b] = kept;
-      off[b] = min(o, nbuf);
-    }
-    for (int r = threadIdx.x; r < kept; r += blockDim.x) {
-      enode[o + r] = s_node[r];
-      ed2[o + r] = s_d2[r];
-      egroup[o + r] = b;
-    }
-    return;
+    run += tot;
+    n += tot;
   }
-  for (int base = 0; base < total; base += blockDim.x) {
-    int node;
-    float normsq;
-    const bool hit = visit(base + threadIdx.x, node, normsq);
-    // each wave's matches take a block-local range (LDS atomic), the
-    // block's total one global range; the two counters alternate between
-    // chunks, so each is cleared a chunk before it is used again
-    const unsigned long long m = __ballot(hit);
-    int* const cur = &s_chunk[chunk_no & 1];
-    int wl = 0;
-    if (lane == 0 && m) wl = atomicAdd(cur, __popcll(m));
-    if (threadIdx.x == 0) s_chunk[(chunk_no + 1) & 1] = 0;
+  if (GROUP && n > keep) {
     __syncthreads();
-    const int tot = *cur;
-    if (threadIdx.x == 0 && tot > 0) {
-      s_base = atomicAdd(&info->E_all, tot);
-      s_count += max(0, min(tot, nbuf - s_base));
-    }
-    __syncthreads();
-    ++chunk_no;
-    const int wbase = __shfl(wl, 0, 64);  // all lanes take part
-    if (hit) {
-      const int o = s_base + wbase + __popcll(m & below);
-      if (o < nbuf) {
-        enode[o] = node;
-        ed2[o] = normsq;
-        egroup[o] = b;
-      }
-    }
+    stage_sort(s_d2, s_idx, s_node, n);
+    n = keep;
   }
+  if (threadIdx.x == 0) s_base = n > 0 ? atomicAdd(&info->E_all, n) : 0;
   __syncthreads();
-  if (threadIdx.x == 0) cnt[b] = s_count;
+  const int o = s_base;
+  // GROUP: n <= keep <= nbuf / groups, never cut
+  const int kept = max(0, min(n, nbuf - o));
+  if (threadIdx.x == 0) {
+    cnt[b] = stored + kept;
+    if (GROUP) off[b] = min(o, nbuf);
+  }
+  for (int r = threadIdx.x; r < kept; r += blockDim.x) {
+    enode[o + r] = s_node[r];
+    ed2[o + r] = s_d2[r];
+    egroup[o + r] = b;
+  }
 }
 
 // After the radius search: info->E = min(E_all, nbuf); with budget B > 0

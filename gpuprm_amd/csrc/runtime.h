@@ -150,6 +150,38 @@ __device__ __forceinline__ float cell_gap2(const CellGrid& g, int d, float x, in
   return len * len;
 }
 
+// Squared distances out[u] from q to stored nodes pos[u], u < P (coordinate
+// i of node p at coords[p + stride i]), each summed in coordinate order like
+// the reference's float loop.  The first eight coordinates of all P nodes
+// are loaded before any is used, so their loads are in flight together (a
+// loop over a run-time ndof would wait for each load in turn).  Every pos[u]
+// must be a stored node.
+template <int P>
+__device__ __forceinline__ void node_dist2(const float* __restrict__ coords, size_t stride, const int* pos, int ndof,
+                                           const float* q, float* out) {
+  constexpr int kFirst = 8;
+  float x[P][kFirst];
+#pragma unroll
+  for (int i = 0; i < kFirst; ++i)
+#pragma unroll
+    for (int u = 0; u < P; ++u) x[u][i] = i < ndof ? coords[pos[u] + stride * i] : 0.0f;
+#pragma unroll
+  for (int u = 0; u < P; ++u) {
+    float s = 0.0f;
+#pragma unroll
+    for (int i = 0; i < kFirst; ++i)
+      if (i < ndof) {
+        const float diff = x[u][i] - q[i];
+        s += diff * diff;
+      }
+    for (int i = kFirst; i < ndof; ++i) {
+      const float diff = coords[pos[u] + stride * i] - q[i];
+      s += diff * diff;
+    }
+    out[u] = s;
+  }
+}
+
 // Result of an edge left unchecked by EdgeArgs::skip (nonzero: never taken
 // as free).
 constexpr int kEdgeSkipped = 0x40000000;
