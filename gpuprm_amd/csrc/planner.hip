@@ -241,8 +241,8 @@ static_assert(kNeighbourCells <= 256, "one thread per neighbour cell in a 256-th
 
 // One block per group b = 2 j + g: the nonempty cells among the 3^dims
 // around new node j are compacted (block scans) and the block strides
-// through their nodes in cell order, 256 candidates a chunk (512 in APPEND
-// mode: two a thread, their loads in flight together).  In either mode
+// through their nodes in cell order, 512 candidates a chunk (two a thread,
+// their loads in flight together).  In either mode
 // the matches are staged in LDS (per-wave ballots, one barrier a chunk), so
 // no chunk waits on a global atomic.
 //
@@ -266,10 +266,10 @@ enum NearMode { kNearAppend = 0, kNearGroup = 1 };
 // search is latency-bound, so occupancy counts)
 constexpr int kGroupStage = 1024;
 constexpr int kGroupKeep = kGroupStage / 2;
-// APPEND: two candidates per thread and chunk (twice the loads in flight, half
-// the barriers); the stage takes two chunks
-constexpr int kAppendPer = 2;
-static_assert(kGroupStage >= 2 * kAppendPer * 256, "append stage");
+// Two candidates per thread and chunk (twice the loads in flight, half the
+// barriers).  The stage takes a cut group and one chunk, or two chunks.
+constexpr int kNearPer = 2;
+static_assert(kGroupStage >= kGroupKeep + kNearPer * 256 && kGroupStage >= 2 * kNearPer * 256, "stage");
 
 // Ascending bitonic sort of the first n stage entries by (squared distance,
 // traversal index), padded to a power of two with +inf; n <= kGroupStage.
@@ -309,14 +309,13 @@ __global__ void __launch_bounds__(256)
                      int* __restrict__ off, int nbuf, int keep, int* __restrict__ enode,
                      float* __restrict__ ed2, int* __restrict__ egroup) {
   constexpr bool GROUP = MODE == kNearGroup;
-  // candidates per thread and chunk
-  constexpr int PER = GROUP ? 1 : kAppendPer;
+  constexpr int PER = kNearPer;
   constexpr int NK = kNeighbourCells;
   constexpr int kWaves = 256 / 64;
   __shared__ int s_start[NK], s_pre[NK + 1];
   __shared__ int lds[kMaxWaves + 1];
   __shared__ int s_base;
-  __shared__ int s_wcnt[2][kWaves];
+  __shared__ int s_wcnt[2][PER][kWaves];
   __shared__ int s_node[kGroupStage];
   __shared__ float s_d2[kGroupStage];
   __shared__ int s_idx[GROUP ? kGroupStage : 1];
@@ -378,24 +377,28 @@ __global__ void __launch_bounds__(256)
       if (k + step < K && s_pre[k + step] <= idx) k += step;
     return s_start[k] + (idx - s_pre[k]);
   };
-  // matches of the chunk in the waves before this one (tot: in the block),
-  // from each wave's count cw: the counts go to one of two LDS rows
-  // (alternating, so a row is rewritten only after every thread has passed
-  // the next chunk's barrier)
-  auto waves_before = [&](int cw, int& tot) -> int {
-    int* const row = s_wcnt[chunk_no & 1];
-    if (lane == 0) row[wv] = cw;
-    __syncthreads();
-    int before = 0;
-    tot = 0;
+  // matches of the chunk's u-th candidates in the waves before this one
+  // (tot: in the block), from each wave's counts cw: the counts go to one of
+  // two LDS rows (alternating, so a row is rewritten only after every thread
+  // has passed the next chunk's barrier)
+  auto waves_before = [&](const int* cw, int* before, int* tot) {
+    int(*const row)[kWaves] = s_wcnt[chunk_no & 1];
+    if (lane == 0)
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
-      const int c = row[w];
-      before += w < wv ? c : 0;
-      tot += c;
+      for (int u = 0; u < PER; ++u) row[u][wv] = cw[u];
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      before[u] = 0;
+      tot[u] = 0;
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) {
+        const int c = row[u][w];
+        before[u] += w < wv ? c : 0;
+        tot[u] += c;
+      }
     }
     ++chunk_no;
-    return before;
   };
   // matches so far, staged entries, entries stored by earlier flushes
   // (APPEND); the same in every thread
@@ -423,8 +426,7 @@ __global__ void __launch_bounds__(256)
     }
     // candidates base + u 256 + t for u < PER: their loads are in flight
     // together (a lane past the end reads the last candidate and matches
-    // nothing); a wave's matches are staged u-major (GROUP, PER = 1: in
-    // traversal order)
+    // nothing); the chunk's matches are staged in traversal order
     int pos[PER], node[PER];
     float normsq[PER];
     bool hit[PER];
@@ -434,27 +436,27 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int u = 0; u < PER; ++u) node[u] = T.sid[pos[u]];
     node_dist2<PER>(T.sorted, (size_t)cap, pos, ndof, qr, normsq);
-    int cw = 0;
+    int cw[PER], before[PER], tot[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       hit[u] = base + u * (int)blockDim.x + (int)threadIdx.x < total && normsq[u] < D2;
       m[u] = __ballot(hit[u]);
-      cw += __popcll(m[u]);
+      cw[u] = __popcll(m[u]);
     }
-    int tot;
-    int r = n + waves_before(cw, tot);
+    waves_before(cw, before, tot);
+    int r = 0;  // the chunk's matches among earlier candidates u
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       if (hit[u]) {
-        const int at = r + __popcll(m[u] & below);
-        s_node[at] = node[u];
-        s_d2[at] = normsq[u];
-        if (GROUP) s_idx[at] = run + (at - n);
+        const int rank = r + before[u] + __popcll(m[u] & below);
+        s_node[n + rank] = node[u];
+        s_d2[n + rank] = normsq[u];
+        if (GROUP) s_idx[n + rank] = run + rank;
       }
-      r += __popcll(m[u]);
+      r += tot[u];
     }
-    run += tot;
-    n += tot;
+    run += r;
+    n += r;
   }
   if (GROUP && n > keep) {
     __syncthreads();
