@@ -185,3 +185,57 @@ def test_prm_radius_search_placement_independent(golden_dir, tmp_path, budget):
     if r0.found:
         assert np.array_equal(p0, p1)
         _verify(o, p0, qs, qg, 1.0)
+
+
+def test_prm_radius_search_complete_for_large_groups(golden_dir, tmp_path):
+    """Groups with more matches than the radius search's LDS stage holds
+    (config1 at D = 5 after 11 steps: over a thousand earlier nodes within D
+    of a new node), so a group's matches are stored in several ranges.  The planner is
+    deterministic, so k + 1 steps repeat the k-step trees and add one step's
+    nodes; each of those must have an edge to exactly the earlier nodes of its
+    tree within D (the planner's float32 sum in coordinate order, by brute
+    force here) whose edge the exact checker finds free, and to no other
+    node."""
+    import gpuprm_amd
+    z, d = golden_dir("config1")
+    s = _space(d)
+    qs = np.zeros(4, np.float32)
+    qg = np.full(4, 3.0, np.float32)
+    D, num, k = 5.0, 512, 11
+    runs = []
+    for steps in (k, k + 1):
+        out = str(tmp_path / ("trees_%d" % steps))
+        print("planner: config1, D %.1f, %d steps" % (D, steps), flush=True)
+        r, _ = s.prm_solve(qs, qg, num=num, D=D, max_nodes=1 << 15, max_steps=steps, seed=5, output_dir=out,
+                           nbuf=1 << 22)
+        print(r.as_dict())
+        assert r.found == 0 and r.steps == steps
+        runs.append(gpuprm_amd.read_prm_output(out, 4))
+    ga, gb = runs
+    largest = 0
+    for side in ("l", "r"):
+        Qa, Qb, Eb = ga["Q" + side], gb["Q" + side], gb["E" + side]
+        na = len(Qa)
+        assert np.array_equal(Qb[:na], Qa)
+        assert np.array_equal(_edge_set(Eb[Eb[:, 1] < na]), _edge_set(ga["E" + side]))
+        got = Eb[Eb[:, 1] >= na]
+        assert np.all(got[:, 0] < na)  # the step's nodes connect to earlier ones only
+        if len(Qb) == na:
+            continue
+        new = Qb[na:]
+        diff = Qa[None, :, :] - new[:, None, :]
+        s2 = diff[..., 0] * diff[..., 0]
+        for i in range(1, 4):
+            s2 = s2 + diff[..., i] * diff[..., i]
+        cand = np.argwhere(s2 < np.float32(D * D))  # (new node, earlier node)
+        largest = max(largest, int(np.bincount(cand[:, 0]).max()))
+        A, B = gpuprm_amd.canonical_edges(new[cand[:, 0]], Qa[cand[:, 1]])
+        n = len(cand)
+        res = np.zeros(n, np.int32)
+        s.indicator2(A.T.copy(), B.T.copy(), res, n, n)
+        free = cand[res == 0]
+        want = np.stack([free[:, 1], free[:, 0] + na], axis=1)
+        print(side, "earlier nodes", na, "new", len(new), "candidates", n, "free", len(want), "edges", len(got))
+        assert np.array_equal(_edge_set(got), _edge_set(want))
+    print("largest group", largest)
+    assert largest > 1024  # more than two chunks' matches: the stage was stored mid-group
