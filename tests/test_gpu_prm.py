@@ -187,15 +187,19 @@ def test_prm_radius_search_placement_independent(golden_dir, tmp_path, budget):
         _verify(o, p0, qs, qg, 1.0)
 
 
-def test_prm_radius_search_complete_for_large_groups(golden_dir, tmp_path):
+@pytest.mark.parametrize("budget", [0, 4])
+def test_prm_radius_search_complete_for_large_groups(golden_dir, tmp_path, budget):
     """Groups with more matches than the radius search's LDS stage holds
     (config1 at D = 5 after 11 steps: over a thousand earlier nodes within D
-    of a new node), so a group's matches are stored in several ranges.  The planner is
+    of a new node), so a group's matches are stored in several ranges, or
+    (with an edge budget) sorted and cut in the stage.  The planner is
     deterministic, so k + 1 steps repeat the k-step trees and add one step's
     nodes; each of those must have an edge to exactly the earlier nodes of its
     tree within D (the planner's float32 sum in coordinate order, by brute
-    force here) whose edge the exact checker finds free, and to no other
-    node."""
+    force here) whose edge the exact checker finds free, and to no other node;
+    with a budget of b, to the free ones among its b nearest (a node whose
+    b-th and (b+1)-th distances tie is not checked: the planner breaks the
+    tie by its traversal order)."""
     import gpuprm_amd
     z, d = golden_dir("config1")
     s = _space(d)
@@ -205,9 +209,9 @@ def test_prm_radius_search_complete_for_large_groups(golden_dir, tmp_path):
     runs = []
     for steps in (k, k + 1):
         out = str(tmp_path / ("trees_%d" % steps))
-        print("planner: config1, D %.1f, %d steps" % (D, steps), flush=True)
+        print("planner: config1, D %.1f, %d steps, budget %d" % (D, steps, budget), flush=True)
         r, _ = s.prm_solve(qs, qg, num=num, D=D, max_nodes=1 << 15, max_steps=steps, seed=5, output_dir=out,
-                           nbuf=1 << 22)
+                           nbuf=1 << 22, edge_budget=budget)
         print(r.as_dict())
         assert r.found == 0 and r.steps == steps
         runs.append(gpuprm_amd.read_prm_output(out, 4))
@@ -227,15 +231,29 @@ def test_prm_radius_search_complete_for_large_groups(golden_dir, tmp_path):
         s2 = diff[..., 0] * diff[..., 0]
         for i in range(1, 4):
             s2 = s2 + diff[..., i] * diff[..., i]
-        cand = np.argwhere(s2 < np.float32(D * D))  # (new node, earlier node)
-        largest = max(largest, int(np.bincount(cand[:, 0]).max()))
+        within = s2 < np.float32(D * D)
+        largest = max(largest, int(within.sum(axis=1).max()))
+        checked = np.ones(len(new), bool)
+        if budget:
+            # the b nearest within D; a tie across the cut leaves the node unchecked
+            order = np.argsort(np.where(within, s2, np.inf), axis=1, kind="stable")
+            keep = np.zeros_like(within)
+            for j in range(len(new)):
+                c = int(within[j].sum())
+                if c > budget and s2[j, order[j, budget - 1]] == s2[j, order[j, budget]]:
+                    checked[j] = False
+                keep[j, order[j, :min(c, budget)]] = True
+            within &= keep
+        cand = np.argwhere(within)  # (new node, earlier node)
         A, B = gpuprm_amd.canonical_edges(new[cand[:, 0]], Qa[cand[:, 1]])
         n = len(cand)
         res = np.zeros(n, np.int32)
         s.indicator2(A.T.copy(), B.T.copy(), res, n, n)
-        free = cand[res == 0]
+        free = cand[(res == 0) & checked[cand[:, 0]]]
         want = np.stack([free[:, 1], free[:, 0] + na], axis=1)
-        print(side, "earlier nodes", na, "new", len(new), "candidates", n, "free", len(want), "edges", len(got))
+        got = got[checked[got[:, 1] - na]]
+        print(side, "earlier nodes", na, "new", len(new), "candidates", n, "free", len(want), "edges", len(got),
+              "unchecked (ties)", int((~checked).sum()))
         assert np.array_equal(_edge_set(got), _edge_set(want))
     print("largest group", largest)
-    assert largest > 1024  # more than two chunks' matches: the stage was stored mid-group
+    assert largest > 1024  # more than two chunks' matches: the stage was stored or cut mid-group
