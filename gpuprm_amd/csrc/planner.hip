@@ -242,9 +242,9 @@ static_assert(kNeighbourCells <= 256, "one thread per neighbour cell in a 256-th
 // One block per group b = 2 j + g: the nonempty cells among the 3^dims
 // around new node j are compacted (block scans) and the block strides
 // through their nodes in cell order, 512 candidates a chunk (two a thread,
-// their loads in flight together).  In either mode
-// the matches are staged in LDS (per-wave ballots, one barrier a chunk), so
-// no chunk waits on a global atomic.
+// their loads in flight together).  In either mode the matches are staged
+// in LDS (per-wave ballots, one barrier a chunk), so no chunk waits on a
+// global atomic.
 //
 // APPEND (no edge budget, so no group order needed): the group takes one
 // range of info->E_all for its staged matches at the end, and one more each
